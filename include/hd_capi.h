@@ -92,6 +92,98 @@ typedef struct hd_edit_desc {
 	uint32_t reserved;
 } hd_edit_desc;
 
+
+/* Pool statistics (the reference prints these to stdout / ImGui, main.cpp:275-278,379-385). */
+typedef struct hd_stats {
+	uint64_t total_words, total_buckets, used_words;
+	uint64_t color_nodes, color_leaf_words;
+	uint32_t overflow;    /* bit0: a bucket was full (upsert fell back, NodePool.hpp:195,210), bit1: color capacity */
+	uint32_t node_levels;
+	uint64_t level_used_words[HD_MAX_LEVELS];
+	uint64_t last_entries, last_upserts, last_jobs; /* work of the last hd_edit */
+	double last_ms;
+} hd_stats;
+
+typedef struct hd_pool hd_pool;
+
+/* Last error message of the calling thread. */
+const char *hd_last_error(void);
+
+/* Create / destroy a device-resident pool: flat HBM node words + per-bucket word counters + color
+ * octree nodes + color leaf words (replaces DAGNodePool::Create, src/DAGNodePool.cpp:9-47, and
+ * DAGColorPool::Create, src/DAGColorPool.cpp:9-48). All work runs on the pool's own HIP stream. */
+int hd_pool_create(const hd_config *cfg, hd_pool **out);
+int hd_pool_destroy(hd_pool *pool);
+int hd_pool_sync(hd_pool *pool);
+void *hd_pool_stream(hd_pool *pool); /* the pool's hipStream_t */
+
+/* Heightfield for HD_EDIT_COLUMN_FILL: resolution x resolution u32 heights, row-major, x fastest. */
+int hd_pool_set_heightmap(hd_pool *pool, const uint32_t *heights, uint32_t res);
+
+/* Host interop (the Flush format of src/DAGNodePool.cpp:56-85 without sparse binding):
+ * word ranges of the node pool, per-bucket word counters, color arrays. */
+int hd_pool_read_words(hd_pool *pool, uint64_t first, uint64_t count, uint32_t *out);
+int hd_pool_write_words(hd_pool *pool, uint64_t first, uint64_t count, const uint32_t *in);
+int hd_pool_read_bucket_words(hd_pool *pool, uint64_t first, uint64_t count, uint32_t *out);
+int hd_pool_write_bucket_words(hd_pool *pool, uint64_t first, uint64_t count, const uint32_t *in);
+int hd_pool_read_color_nodes(hd_pool *pool, uint64_t first, uint64_t count, uint32_t *out); /* 8 u32 per node */
+int hd_pool_read_color_leaves(hd_pool *pool, uint64_t first, uint64_t count, uint32_t *out);
+/* NodePoolBase::make_filled_node_pointers (NodePool.hpp:240-262); creates the chain on first use. */
+int hd_pool_filled(hd_pool *pool, uint32_t *out, uint32_t cap, uint32_t *n);
+int hd_pool_stats(hd_pool *pool, hd_stats *stats);
+int hd_pool_clear_overflow(hd_pool *pool);
+
+/* Edit modes. */
+enum {
+	HD_MODE_PLAIN = 0,  /* one level-synchronous pass per edit, edits applied in order: the result
+	                       (pool words, bucket counters, color arrays, roots) equals NodePoolBase::Edit
+	                       applied to each edit in turn (NodePool.hpp:405-417) */
+	HD_MODE_BATCHED = 1 /* all edits in one level-synchronous pass with per-node ordered edit lists;
+	                       the reachable DAG and the colors of occupied voxels equal sequential
+	                       application */
+};
+
+/* Apply n edits to (root, color_root); writes the new roots. Replaces NodePoolBase::Edit and
+ * NodePoolThreadedEdit::ThreadedEdit with the VBREditorWrapper / StatelessEditorWrapper
+ * (main.cpp:214-234). Blocking: returns after the device work completed. */
+int hd_edit(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_edit_desc *edits, uint32_t n,
+            uint32_t mode, uint32_t *root_out, uint32_t *color_root_out);
+
+/* NodePoolThreadedGC::ThreadedGC (NodePoolThreadedGC.hpp:394-403): keeps the DAGs below roots[0..n)
+ * and the filled chain, compacts every bucket, remaps roots in place. */
+int hd_gc(hd_pool *pool, uint32_t *roots, uint32_t n);
+
+/* One camera (trace.frag push constants, trace.frag:15-23 / TracePass.cpp:10-18). World = [0,1]^3. */
+typedef struct hd_view {
+	float pos[3], look[3], side[3], up[3];
+	float proj_factor;  /* TracePass.cpp:115-118; +inf disables the LOD cut-off (trace.frag:148) */
+	uint32_t width, height;
+	uint32_t flags;     /* HD_TRACE_* */
+	uint64_t out_offset; /* first pixel of this view in the output buffers */
+} hd_view;
+
+enum {
+	HD_TRACE_SHADOW = 1, /* one shadow ray per hit toward normalize(4,5,3) (no LOD) */
+	HD_TRACE_RECORDS = 2 /* also write 48-byte per-pixel hit records */
+};
+
+/* Render views on the device (DAG_RayMarch + Color_Fetch + the diffuse shading of trace.frag:374-402).
+ * rgba: device u32 per pixel (RGBA8); records: device, 12 u32 per pixel when HD_TRACE_RECORDS:
+ * vox[3], vox_size | hit << 31, hit_pos[3] (f32), norm[3] (f32), iterations, shadow hit. */
+int hd_trace(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *views, uint32_t n_views,
+             uint32_t *rgba, uint32_t *records);
+
+/* NodePoolTraversal::Traversal<float> (NodePoolTraversal.hpp:93-256) for n rays given as device
+ * arrays o[3n], d[3n]; out: device float[4n] = hit position in [0,1]^3 + hit flag. */
+int hd_traversal(hd_pool *pool, uint32_t root, const float *o, const float *d, uint64_t n, float *out);
+
+/* Device memory helpers for hosts without a GPU runtime of their own. */
+int hd_device_alloc(uint64_t bytes, void **out);
+int hd_device_free(void *ptr);
+int hd_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
+int hd_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
+int hd_device_count(int *n);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -132,6 +132,8 @@ struct VBRColor {
 };
 inline VBRColor from_hd(const hd_color &c) {
 	VBRColor v;
+	if (!c.has_value)
+		return v; // a VBRColor without value is default-constructed (all zero)
 	v.colors = c.colors;
 	v.weight = c.weight;
 	v.bpw = c.bits_per_weight;
@@ -1306,6 +1308,9 @@ Hit ray_march(WordFn &&word, u32 root, u32 leaf_level, float proj_factor, float 
 	if ((oct & 4u) == 0u)
 		n[2] = -n[2];
 	const u32 voxel_level = leaf_level + 1u, voxel_scale = S - voxel_level;
+	out.iter = iter;
+	if (!(scale < S && t_min <= t_max))
+		return out; // a miss reports zeros (trace.frag leaves them unused)
 	u32 vox_size = 1u << (scale - voxel_scale);
 	u32 vp[3] = {(f2u(pos.x) & 0x7FFFFFu) >> voxel_scale, (f2u(pos.y) & 0x7FFFFFu) >> voxel_scale,
 	             (f2u(pos.z) & 0x7FFFFFu) >> voxel_scale};

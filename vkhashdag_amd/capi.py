@@ -1,0 +1,271 @@
+"""ctypes binding of include/hd_capi.h and a thin host object mirroring the reference interface
+(hashdag::NodePoolBase::Edit / NodePoolThreadedEdit::ThreadedEdit / NodePoolThreadedGC::ThreadedGC /
+NodePoolTraversal::Traversal and the DAGColorPool root). The library is the HIP build
+vkhashdag_amd/libhashdag_amd.so; there is no CPU fallback: loading fails loudly without it."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+from .workloads import EDIT_DTYPE, HD_COLOR_NULL, HD_NULL, PoolConfig
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhashdag_amd.so")
+HD_MAX_LEVELS = 24
+MODE_PLAIN, MODE_BATCHED = 0, 1
+TRACE_SHADOW, TRACE_RECORDS = 1, 2
+
+
+class HDError(RuntimeError):
+    pass
+
+
+class hd_config(C.Structure):
+    _fields_ = [("word_bits_per_page", C.c_uint32), ("page_bits_per_bucket", C.c_uint32),
+                ("node_levels", C.c_uint32), ("bucket_bits", C.c_uint32 * HD_MAX_LEVELS),
+                ("color_leaf_level", C.c_uint32), ("color_keep_history", C.c_uint32),
+                ("color_node_capacity", C.c_uint64), ("color_leaf_capacity", C.c_uint64)]
+
+
+class hd_stats(C.Structure):
+    _fields_ = [("total_words", C.c_uint64), ("total_buckets", C.c_uint64), ("used_words", C.c_uint64),
+                ("color_nodes", C.c_uint64), ("color_leaf_words", C.c_uint64), ("overflow", C.c_uint32),
+                ("node_levels", C.c_uint32), ("level_used_words", C.c_uint64 * HD_MAX_LEVELS),
+                ("last_entries", C.c_uint64), ("last_upserts", C.c_uint64), ("last_jobs", C.c_uint64),
+                ("last_ms", C.c_double)]
+
+
+class hd_view(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("look", C.c_float * 3), ("side", C.c_float * 3), ("up", C.c_float * 3),
+                ("proj_factor", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("flags", C.c_uint32),
+                ("out_offset", C.c_uint64)]
+
+
+assert C.sizeof(hd_view) == 72
+
+_lib = None
+VP, U32, U64 = C.c_void_p, C.c_uint32, C.c_uint64
+P32 = C.POINTER(C.c_uint32)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HDError(f"{LIB_PATH} is missing: build it with `python -m vkhashdag_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        sigs = {
+            "hd_last_error": (C.c_char_p, []),
+            "hd_pool_create": (C.c_int, [C.POINTER(hd_config), C.POINTER(VP)]),
+            "hd_pool_destroy": (C.c_int, [VP]),
+            "hd_pool_sync": (C.c_int, [VP]),
+            "hd_pool_stream": (VP, [VP]),
+            "hd_pool_set_heightmap": (C.c_int, [VP, VP, U32]),
+            "hd_pool_read_words": (C.c_int, [VP, U64, U64, VP]),
+            "hd_pool_write_words": (C.c_int, [VP, U64, U64, VP]),
+            "hd_pool_read_bucket_words": (C.c_int, [VP, U64, U64, VP]),
+            "hd_pool_write_bucket_words": (C.c_int, [VP, U64, U64, VP]),
+            "hd_pool_read_color_nodes": (C.c_int, [VP, U64, U64, VP]),
+            "hd_pool_read_color_leaves": (C.c_int, [VP, U64, U64, VP]),
+            "hd_pool_filled": (C.c_int, [VP, VP, U32, P32]),
+            "hd_pool_stats": (C.c_int, [VP, C.POINTER(hd_stats)]),
+            "hd_pool_clear_overflow": (C.c_int, [VP]),
+            "hd_edit": (C.c_int, [VP, U32, U32, VP, U32, U32, P32, P32]),
+            "hd_gc": (C.c_int, [VP, VP, U32]),
+            "hd_trace": (C.c_int, [VP, U32, U32, VP, U32, VP, VP]),
+            "hd_trace_async": (C.c_int, [VP, U32, U32, VP, VP, U32, U32, VP, VP]),
+            "hd_trace_tiles": (C.c_int, [VP, U32, VP]),
+            "hd_traversal": (C.c_int, [VP, U32, VP, VP, U64, VP]),
+            "hd_device_alloc": (C.c_int, [U64, C.POINTER(VP)]),
+            "hd_device_free": (C.c_int, [VP]),
+            "hd_memcpy_h2d": (C.c_int, [VP, VP, U64]),
+            "hd_memcpy_d2h": (C.c_int, [VP, VP, U64]),
+            "hd_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise HDError(f"hd error {rc}: {lib().hd_last_error().decode()}")
+
+
+class DeviceBuffer:
+    """A device allocation (hd_device_alloc); freed with the object."""
+
+    def __init__(self, nbytes):
+        p = VP()
+        check(lib().hd_device_alloc(max(4, int(nbytes)), C.byref(p)))
+        self.ptr, self.nbytes = p.value, int(nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().hd_memcpy_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype)
+        check(lib().hd_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().hd_device_free(self.ptr)
+            self.ptr = None
+
+
+def make_view(vb, width, height, flags=0, out_offset=0) -> hd_view:
+    """vb: 13 floats (pos, look, side, up, proj_factor) as produced by workloads.orbit_view."""
+    v = hd_view()
+    vb = np.asarray(vb, np.float32)
+    for i in range(3):
+        v.pos[i], v.look[i], v.side[i], v.up[i] = vb[i], vb[3 + i], vb[6 + i], vb[9 + i]
+    v.proj_factor = float(vb[12])
+    v.width, v.height, v.flags, v.out_offset = width, height, flags, out_offset
+    return v
+
+
+class DevicePool:
+    """A device-resident HashDAG node pool + color pool (DAGNodePool + DAGColorPool semantics)."""
+
+    def __init__(self, cfg: PoolConfig, heights=None):
+        self.cfg = cfg
+        c = hd_config()
+        c.word_bits_per_page = cfg.word_bits_per_page
+        c.page_bits_per_bucket = cfg.page_bits_per_bucket
+        c.node_levels = cfg.node_levels
+        for i, b in enumerate(cfg.bucket_bits):
+            c.bucket_bits[i] = b
+        c.color_leaf_level = cfg.color_leaf_level
+        c.color_keep_history = int(cfg.keep_history)
+        c.color_node_capacity = cfg.color_node_capacity
+        c.color_leaf_capacity = cfg.color_leaf_capacity
+        h = VP()
+        check(lib().hd_pool_create(C.byref(c), C.byref(h)))
+        self.h = h.value
+        self.root = HD_NULL
+        self.color_root = HD_COLOR_NULL
+        if heights is not None:
+            self.set_heightmap(heights)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hd_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_heightmap(self, heights):
+        heights = np.ascontiguousarray(heights, np.uint32)
+        check(lib().hd_pool_set_heightmap(self.h, heights.ctypes.data, heights.shape[0]))
+
+    # -- NodePoolBase::Edit / ThreadedEdit (one device pass per edit) and the batched pass
+    def edit(self, edits, mode=MODE_PLAIN):
+        e = np.ascontiguousarray(np.atleast_1d(edits).astype(EDIT_DTYPE))
+        r, cr = U32(0), U32(0)
+        check(lib().hd_edit(self.h, self.root, self.color_root, e.ctypes.data, len(e), mode, C.byref(r),
+                            C.byref(cr)))
+        self.root, self.color_root = r.value, cr.value
+        return self.root
+
+    Edit = edit
+    ThreadedEdit = edit
+
+    def gc(self, extra_roots=()):
+        roots = np.asarray([self.root, *extra_roots], np.uint32)
+        check(lib().hd_gc(self.h, roots.ctypes.data, len(roots)))
+        self.root = int(roots[0])
+        return roots
+
+    ThreadedGC = gc
+
+    # -- host interop
+    def read_words(self, first, count):
+        out = np.empty(count, np.uint32)
+        check(lib().hd_pool_read_words(self.h, first, count, out.ctypes.data))
+        return out
+
+    def full_words(self):
+        return self.read_words(0, self.cfg.total_words)
+
+    def bucket_words(self):
+        out = np.empty(self.cfg.total_buckets, np.uint32)
+        check(lib().hd_pool_read_bucket_words(self.h, 0, len(out), out.ctypes.data))
+        return out
+
+    def live_words(self):
+        bw = self.bucket_words()
+        words = self.full_words()
+        wpb = self.cfg.words_per_bucket
+        nz = np.nonzero(bw)[0]
+        parts = [words[int(b) * wpb: int(b) * wpb + int(bw[b])] for b in nz]
+        return bw, (np.concatenate(parts) if parts else np.zeros(0, np.uint32))
+
+    def filled(self):
+        out = np.zeros(64, np.uint32)
+        n = U32(0)
+        check(lib().hd_pool_filled(self.h, out.ctypes.data, 64, C.byref(n)))
+        return out[: n.value]
+
+    def stats(self):
+        s = hd_stats()
+        check(lib().hd_pool_stats(self.h, C.byref(s)))
+        return s
+
+    def color_nodes(self):
+        n = self.stats().color_nodes
+        out = np.zeros(max(8, 8 * n), np.uint32)
+        if n:
+            check(lib().hd_pool_read_color_nodes(self.h, 0, n, out.ctypes.data))
+        return out[: 8 * n]
+
+    def color_leaves(self):
+        n = self.stats().color_leaf_words
+        out = np.zeros(max(1, n), np.uint32)
+        if n:
+            check(lib().hd_pool_read_color_leaves(self.h, 0, n, out.ctypes.data))
+        return out[:n]
+
+    # -- rendering (trace.frag) and picking (NodePoolTraversal)
+    def render(self, vb, width, height, shadow=True, records=False, root=None, color_root=None):
+        flags = (TRACE_SHADOW if shadow else 0) | (TRACE_RECORDS if records else 0)
+        v = make_view(vb, width, height, flags)
+        npx = width * height
+        rgba = DeviceBuffer(npx * 4)
+        rec = DeviceBuffer(npx * 48) if records else None
+        check(lib().hd_trace(self.h, self.root if root is None else root,
+                             self.color_root if color_root is None else color_root, C.byref(v), 1, rgba.ptr,
+                             rec.ptr if rec else None))
+        out = rgba.download(np.uint32, npx)
+        if records:
+            return out, rec.download(np.uint32, npx * 12).reshape(npx, 12)
+        return out
+
+    def traversal(self, o, d, root=None):
+        o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+        n = len(o)
+        bo, bd, bout = DeviceBuffer(o.nbytes), DeviceBuffer(d.nbytes), DeviceBuffer(n * 16)
+        bo.upload(o)
+        bd.upload(d)
+        check(lib().hd_traversal(self.h, self.root if root is None else root, bo.ptr, bd.ptr, n, bout.ptr))
+        return bout.download(np.float32, 4 * n).reshape(n, 4)
+
+    Traversal = traversal
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    try:
+        check(lib().hd_device_count(C.byref(n)))
+    except HDError:
+        return 0
+    return n.value

@@ -1,0 +1,109 @@
+// hd_internal.hpp -- host-side state of a device pool and helpers shared by the HIP sources.
+#pragma once
+
+#include <cstring> // rocPRIM's texture_cache_iterator uses memset
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "hd_device.hpp"
+
+namespace hd {
+
+void set_error(const char *fmt, ...);
+
+#define HD_TRY(expr)                                                                                              \
+	do {                                                                                                          \
+		hipError_t e_ = (expr);                                                                                   \
+		if (e_ != hipSuccess) {                                                                                   \
+			::hd::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));                \
+			return HD_ERR_DEVICE;                                                                                 \
+		}                                                                                                         \
+	} while (0)
+#define HD_RET(expr)                                                                                              \
+	do {                                                                                                          \
+		int r_ = (expr);                                                                                          \
+		if (r_ != HD_OK)                                                                                          \
+			return r_;                                                                                            \
+	} while (0)
+
+// Growable device buffer.
+struct DevBuf {
+	void *p = nullptr;
+	size_t bytes = 0;
+	int ensure(size_t n) {
+		if (n <= bytes && p)
+			return HD_OK;
+		if (p)
+			HD_TRY(hipFree(p));
+		p = nullptr;
+		size_t want = n < 256 ? 256 : n + n / 4;
+		HD_TRY(hipMalloc(&p, want));
+		bytes = want;
+		return HD_OK;
+	}
+	template <typename T> T *as() const { return static_cast<T *>(p); }
+	void release() {
+		if (p)
+			(void)hipFree(p);
+		p = nullptr;
+		bytes = 0;
+	}
+};
+
+// Named scratch buffers, grown on demand and kept for the pool's lifetime.
+struct Scratch {
+	std::map<std::string, DevBuf> bufs;
+	template <typename T> int get(const std::string &name, size_t count, T **out) {
+		HD_RET(bufs[name].ensure(count * sizeof(T)));
+		*out = bufs[name].as<T>();
+		return HD_OK;
+	}
+	void release() {
+		for (auto &kv : bufs)
+			kv.second.release();
+		bufs.clear();
+	}
+};
+
+// Device-side counters of a pool (one small array so a single copy reads them all).
+enum Meta : u32 {
+	kMetaColorNodes = 0,  // color octree node count (SafePagedVector count of DAGColorPool::m_nodes)
+	kMetaColorLeaves = 1, // color leaf word count
+	kMetaOverflow = 2,    // bit 0: a bucket was full (upsert fell back), bit 1: color capacity, bit 2: GC bucket
+	kMetaCount = 8
+};
+
+struct Pool {
+	hd_config cfg{};
+	DevConfig dc{};
+	u64 total_words = 0, total_buckets = 0;
+	u32 *words = nullptr;        // total_words u32: the node pool (pages of every bucket, flat)
+	u32 *bucket_words = nullptr; // total_buckets u32: used words per bucket (incl. page padding)
+	u32 *meta = nullptr;         // kMetaCount u32
+	std::vector<u32> filled;     // filled-node chain (NodePool.hpp:240-262), host copy
+	u32 *d_filled = nullptr;
+	u32 *cnodes = nullptr, *cleaves = nullptr; // color octree nodes (8 u32 each) and VBR leaf words
+	u64 cnode_cap = 0, cleaf_cap = 0;
+	u32 *hm_mn = nullptr, *hm_mx = nullptr;
+	DevHeightmap hm{};
+	hipStream_t stream = nullptr;
+	Scratch scratch;
+	// statistics of the last edit (host)
+	u64 last_entries = 0, last_upserts = 0, last_jobs = 0;
+	double last_ms = 0;
+};
+
+int make_filled(Pool &p);               // hd_pool.hip
+int read_meta(Pool &p, u32 *host_meta); // hd_pool.hip
+
+inline u32 div_up(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
+
+} // namespace hd
+
+struct hd_pool {
+	hd::Pool p;
+};

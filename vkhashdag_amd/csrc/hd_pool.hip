@@ -1,0 +1,333 @@
+// hd_pool.hip -- device-resident node pool + color pool: creation, host interop, heightmap,
+// filled-node chain. Replaces the page store of src/DAGNodePool.hpp:41-99 (+ .cpp:9-85) and the
+// paged color storage of src/DAGColorPool.hpp:51-64 with flat HBM arrays.
+#include "hd_internal.hpp"
+
+#include <cstdarg>
+#include <mutex>
+
+
+namespace hd {
+
+static thread_local char g_err[512] = "";
+void set_error(const char *fmt, ...) {
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof(g_err), fmt, ap);
+	va_end(ap);
+}
+
+// NodePool::make_filled_node_pointers (NodePool.hpp:240-262), one thread: the all-solid leaf,
+// then the all-solid inner node of every level, each found or appended in its bucket.
+__global__ void k_make_filled(u32 *words, u32 *bucket_words, DevConfig c, u32 *out, u32 *meta) {
+	if (threadIdx.x | blockIdx.x)
+		return;
+	u32 node[9];
+	u32 prev = kNull;
+	for (int l = (int)c.L - 1; l >= 0; --l) {
+		bool leaf = l == (int)c.L - 1;
+		u32 n = leaf ? 2u : 9u;
+		if (leaf)
+			node[0] = node[1] = ~0u;
+		else {
+			node[0] = 0xFFu;
+			for (int i = 1; i < 9; ++i)
+				node[i] = prev;
+		}
+		u32 b = bucket_of(c, (u32)l, node, n, leaf);
+		u32 bw = bucket_words[b];
+		u32 p = find_in_bucket(words, c, b, bw, node, n, leaf);
+		if (p == kNull) {
+			p = append_to_bucket(words, c, b, bw, node, n);
+			if (p == kNull)
+				meta[kMetaOverflow] |= 1u;
+			else
+				bucket_words[b] = bw;
+		}
+		out[l] = p;
+		prev = p;
+	}
+}
+
+int make_filled(Pool &p) {
+	if (!p.filled.empty())
+		return HD_OK;
+	k_make_filled<<<1, 64, 0, p.stream>>>(p.words, p.bucket_words, p.dc, p.d_filled, p.meta);
+	HD_TRY(hipGetLastError());
+	p.filled.resize(p.dc.L);
+	HD_TRY(hipMemcpyAsync(p.filled.data(), p.d_filled, p.dc.L * 4, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+
+int read_meta(Pool &p, u32 *host_meta) {
+	HD_TRY(hipMemcpyAsync(host_meta, p.meta, kMetaCount * 4, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+
+// Heightfield min/max pyramid, one level per launch.
+__global__ void k_pyramid(const u32 *src_mn, const u32 *src_mx, u32 *dst_mn, u32 *dst_mx, u32 side) {
+	u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (u64)side * side)
+		return;
+	u32 x = (u32)(i % side), y = (u32)(i / side), ps = side * 2;
+	u64 a = (u64)(2 * y) * ps + 2 * x, b = a + ps;
+	dst_mn[i] = min(min(src_mn[a], src_mn[a + 1]), min(src_mn[b], src_mn[b + 1]));
+	dst_mx[i] = max(max(src_mx[a], src_mx[a + 1]), max(src_mx[b], src_mx[b + 1]));
+}
+
+} // namespace hd
+
+using namespace hd;
+
+extern "C" {
+
+const char *hd_last_error(void) { return g_err; }
+
+int hd_pool_create(const hd_config *cfg, hd_pool **out) {
+	if (!cfg || !out || cfg->node_levels == 0 || cfg->node_levels > HD_MAX_LEVELS - 1 || cfg->word_bits_per_page < 4 ||
+	    cfg->color_leaf_level > cfg->node_levels - 1 || cfg->node_levels + 1 > 22) {
+		set_error("invalid hd_config");
+		return HD_ERR_ARG;
+	}
+	u64 buckets = 0;
+	for (u32 l = 0; l < cfg->node_levels; ++l) {
+		if (cfg->bucket_bits[l] > 30) {
+			set_error("bucket_bits too large");
+			return HD_ERR_ARG;
+		}
+		buckets += 1ull << cfg->bucket_bits[l];
+	}
+	u64 words = buckets << (cfg->word_bits_per_page + cfg->page_bits_per_bucket);
+	if (words - 1 > 0xFFFFFFFEull) { // Config::Validate (Config.hpp:48-56)
+		set_error("pool exceeds 2^32-1 words");
+		return HD_ERR_ARG;
+	}
+	if (cfg->color_node_capacity >= (1ull << 30) - 1 || cfg->color_leaf_capacity > (1ull << 30)) {
+		set_error("color capacities exceed 30-bit color pointers (DAGColorPool.hpp:24)");
+		return HD_ERR_ARG;
+	}
+	auto *h = new hd_pool();
+	Pool &p = h->p;
+	p.cfg = *cfg;
+	p.dc.wbpp = cfg->word_bits_per_page;
+	p.dc.pbpb = cfg->page_bits_per_bucket;
+	p.dc.L = cfg->node_levels;
+	p.dc.color_leaf = cfg->color_leaf_level;
+	p.dc.voxel_level = cfg->node_levels + 1;
+	p.dc.keep_history = cfg->color_keep_history;
+	u32 base = 0;
+	for (u32 l = 0; l < cfg->node_levels; ++l) {
+		p.dc.bucket_bits[l] = cfg->bucket_bits[l];
+		p.dc.level_base[l] = base;
+		base += 1u << cfg->bucket_bits[l];
+	}
+	p.total_buckets = buckets;
+	p.total_words = words;
+	p.cnode_cap = cfg->color_node_capacity;
+	p.cleaf_cap = cfg->color_leaf_capacity;
+	auto fail = [&](hipError_t e) {
+		set_error("hd_pool_create: %s", hipGetErrorString(e));
+		hd_pool_destroy(h);
+		return HD_ERR_DEVICE;
+	};
+	hipError_t e;
+	if ((e = hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMalloc(&p.words, words * 4)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMalloc(&p.bucket_words, buckets * 4)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMalloc(&p.meta, kMetaCount * 4)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMalloc(&p.d_filled, HD_MAX_LEVELS * 4)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMalloc(&p.cnodes, (p.cnode_cap ? p.cnode_cap : 1) * 32)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMalloc(&p.cleaves, (p.cleaf_cap ? p.cleaf_cap : 1) * 4)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMemsetAsync(p.words, 0, words * 4, p.stream)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMemsetAsync(p.bucket_words, 0, buckets * 4, p.stream)) != hipSuccess)
+		return fail(e);
+	if ((e = hipMemsetAsync(p.meta, 0, kMetaCount * 4, p.stream)) != hipSuccess)
+		return fail(e);
+	if ((e = hipStreamSynchronize(p.stream)) != hipSuccess)
+		return fail(e);
+	*out = h;
+	return HD_OK;
+}
+
+int hd_pool_destroy(hd_pool *h) {
+	if (!h)
+		return HD_OK;
+	Pool &p = h->p;
+	if (p.stream)
+		(void)hipStreamSynchronize(p.stream);
+	p.scratch.release();
+	for (void *ptr : {(void *)p.words, (void *)p.bucket_words, (void *)p.meta, (void *)p.d_filled, (void *)p.cnodes,
+	                  (void *)p.cleaves, (void *)p.hm_mn, (void *)p.hm_mx})
+		if (ptr)
+			(void)hipFree(ptr);
+	if (p.stream)
+		(void)hipStreamDestroy(p.stream);
+	delete h;
+	return HD_OK;
+}
+
+int hd_pool_sync(hd_pool *h) {
+	HD_TRY(hipStreamSynchronize(h->p.stream));
+	return HD_OK;
+}
+
+void *hd_pool_stream(hd_pool *h) { return h ? (void *)h->p.stream : nullptr; }
+
+int hd_pool_set_heightmap(hd_pool *h, const uint32_t *heights, uint32_t res) {
+	Pool &p = h->p;
+	if (!heights || res != (1u << p.dc.voxel_level)) {
+		set_error("heightmap must be resolution x resolution (%u)", 1u << p.dc.voxel_level);
+		return HD_ERR_ARG;
+	}
+	u64 total = 0;
+	DevHeightmap hm;
+	hm.res = res;
+	u32 k = 0;
+	for (u32 s = res;; s >>= 1, ++k) {
+		hm.off[k] = total;
+		total += (u64)s * s;
+		if (s == 1)
+			break;
+	}
+	if (p.hm_mn)
+		HD_TRY(hipFree(p.hm_mn));
+	if (p.hm_mx)
+		HD_TRY(hipFree(p.hm_mx));
+	HD_TRY(hipMalloc(&p.hm_mn, total * 4));
+	HD_TRY(hipMalloc(&p.hm_mx, total * 4));
+	HD_TRY(hipMemcpyAsync(p.hm_mn, heights, (u64)res * res * 4, hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipMemcpyAsync(p.hm_mx, heights, (u64)res * res * 4, hipMemcpyHostToDevice, p.stream));
+	k = 1;
+	for (u32 s = res >> 1; s >= 1; s >>= 1, ++k) {
+		u64 n = (u64)s * s;
+		k_pyramid<<<div_up(n, 256), 256, 0, p.stream>>>(p.hm_mn + hm.off[k - 1], p.hm_mx + hm.off[k - 1],
+		                                                 p.hm_mn + hm.off[k], p.hm_mx + hm.off[k], s);
+		HD_TRY(hipGetLastError());
+	}
+	hm.mn = p.hm_mn;
+	hm.mx = p.hm_mx;
+	p.hm = hm;
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+
+int hd_pool_read_words(hd_pool *h, uint64_t first, uint64_t count, uint32_t *out) {
+	Pool &p = h->p;
+	if (first + count > p.total_words)
+		return set_error("read beyond pool"), HD_ERR_ARG;
+	HD_TRY(hipMemcpyAsync(out, p.words + first, count * 4, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+int hd_pool_write_words(hd_pool *h, uint64_t first, uint64_t count, const uint32_t *in) {
+	Pool &p = h->p;
+	if (first + count > p.total_words)
+		return set_error("write beyond pool"), HD_ERR_ARG;
+	HD_TRY(hipMemcpyAsync(p.words + first, in, count * 4, hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+int hd_pool_read_bucket_words(hd_pool *h, uint64_t first, uint64_t count, uint32_t *out) {
+	Pool &p = h->p;
+	if (first + count > p.total_buckets)
+		return set_error("read beyond buckets"), HD_ERR_ARG;
+	HD_TRY(hipMemcpyAsync(out, p.bucket_words + first, count * 4, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+int hd_pool_write_bucket_words(hd_pool *h, uint64_t first, uint64_t count, const uint32_t *in) {
+	Pool &p = h->p;
+	if (first + count > p.total_buckets)
+		return set_error("write beyond buckets"), HD_ERR_ARG;
+	HD_TRY(hipMemcpyAsync(p.bucket_words + first, in, count * 4, hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+int hd_pool_filled(hd_pool *h, uint32_t *out, uint32_t cap, uint32_t *n) {
+	Pool &p = h->p;
+	HD_RET(make_filled(p));
+	*n = (u32)p.filled.size();
+	for (u32 i = 0; i < p.filled.size() && i < cap; ++i)
+		out[i] = p.filled[i];
+	return HD_OK;
+}
+int hd_pool_stats(hd_pool *h, hd_stats *s) {
+	Pool &p = h->p;
+	u32 meta[kMetaCount];
+	HD_RET(read_meta(p, meta));
+	std::memset(s, 0, sizeof(*s));
+	s->total_words = p.total_words;
+	s->total_buckets = p.total_buckets;
+	s->node_levels = p.dc.L;
+	s->color_nodes = meta[kMetaColorNodes];
+	s->color_leaf_words = meta[kMetaColorLeaves];
+	s->overflow = meta[kMetaOverflow];
+	std::vector<u32> bw(p.total_buckets);
+	HD_TRY(hipMemcpy(bw.data(), p.bucket_words, p.total_buckets * 4, hipMemcpyDeviceToHost));
+	for (u32 l = 0; l < p.dc.L; ++l) {
+		u64 sum = 0;
+		for (u32 b = p.dc.level_base[l]; b < p.dc.level_base[l] + (1u << p.dc.bucket_bits[l]); ++b)
+			sum += bw[b];
+		s->level_used_words[l] = sum;
+		s->used_words += sum;
+	}
+	s->last_entries = p.last_entries;
+	s->last_upserts = p.last_upserts;
+	s->last_jobs = p.last_jobs;
+	s->last_ms = p.last_ms;
+	return HD_OK;
+}
+int hd_pool_read_color_nodes(hd_pool *h, uint64_t first, uint64_t count, uint32_t *out) {
+	Pool &p = h->p;
+	if (first + count > p.cnode_cap)
+		return set_error("read beyond color nodes"), HD_ERR_ARG;
+	HD_TRY(hipMemcpyAsync(out, p.cnodes + first * 8, count * 32, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+int hd_pool_read_color_leaves(hd_pool *h, uint64_t first, uint64_t count, uint32_t *out) {
+	Pool &p = h->p;
+	if (first + count > p.cleaf_cap)
+		return set_error("read beyond color leaves"), HD_ERR_ARG;
+	HD_TRY(hipMemcpyAsync(out, p.cleaves + first, count * 4, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+int hd_pool_clear_overflow(hd_pool *h) {
+	HD_TRY(hipMemsetAsync(h->p.meta + kMetaOverflow, 0, 4, h->p.stream));
+	HD_TRY(hipStreamSynchronize(h->p.stream));
+	return HD_OK;
+}
+
+int hd_device_alloc(uint64_t bytes, void **out) {
+	HD_TRY(hipMalloc(out, bytes ? bytes : 4));
+	return HD_OK;
+}
+int hd_device_free(void *ptr) {
+	HD_TRY(hipFree(ptr));
+	return HD_OK;
+}
+int hd_memcpy_h2d(void *dst, const void *src, uint64_t bytes) {
+	HD_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+	return HD_OK;
+}
+int hd_memcpy_d2h(void *dst, const void *src, uint64_t bytes) {
+	HD_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+	return HD_OK;
+}
+int hd_device_count(int *n) {
+	HD_TRY(hipGetDeviceCount(n));
+	return HD_OK;
+}
+
+} // extern "C"

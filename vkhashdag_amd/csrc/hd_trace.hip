@@ -1,0 +1,410 @@
+// hd_trace.hip -- ESVO ray traversal of the device pool (shader/src/trace.frag:69-402 and
+// include/hashdag/NodePoolTraversal.hpp:93-256).
+//
+// The march keeps trace.frag's fp32 operation order (the library is built with
+// -ffp-contract=off and correctly rounded division / sqrt) so hits are bit-identical to the
+// CPU transliteration. Copyright (c) 2009-2011, NVIDIA Corporation, BSD-3 for the ESVO march
+// (the full notice is reproduced at trace.frag:25-50); after Laine & Karras, "Efficient Sparse
+// Voxel Octrees" (I3D 2010).
+//
+// Launch: 256 threads = a 16x16 pixel tile; the per-scale parent stack lives in LDS laid out
+// [scale][thread] so a wavefront's stack accesses hit 64 consecutive banks.
+#include "hd_internal.hpp"
+
+namespace hd {
+
+constexpr u32 kStack = 23;  // STACK_SIZE, fraction bits of float (trace.frag:52-54)
+constexpr u32 kTile = 16;   // 16x16 pixels per workgroup
+constexpr u32 kThreads = kTile * kTile;
+constexpr u32 kMaxIter = 1u << 16;
+
+__device__ inline float fmx(float a, float b) { return a < b ? b : a; } // std::max / GLSL max order
+__device__ inline float fmn(float a, float b) { return b < a ? b : a; }
+__device__ inline u32 f2u(float f) { return __float_as_uint(f); }
+__device__ inline float u2f(u32 u) { return __uint_as_float(u); }
+
+struct MarchOut {
+	bool hit;
+	u32 vox[3], vox_size, iter;
+	float hit_pos[3], norm[3];
+	// NodePoolTraversal result: clamp(o + t_min * d, cube) - 1 (valid when hit)
+	float pick[3];
+};
+
+// DAG_RayMarch (trace.frag:69-270). stack: LDS base for this thread, stride = blockDim.
+__device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_level, float proj_factor,
+                                 float proj_bias, float ox, float oy, float oz, float dx, float dy, float dz,
+                                 u32 *stack, u32 stride) {
+	MarchOut out;
+	out.hit = false;
+	out.iter = 0;
+	out.vox_size = 0;
+	for (int i = 0; i < 3; ++i)
+		out.vox[i] = 0, out.hit_pos[i] = 0.0f, out.norm[i] = 0.0f, out.pick[i] = 0.0f;
+	if (root == kNull)
+		return out;
+	u32 iter = 0;
+	ox += 1.0f, oy += 1.0f, oz += 1.0f;
+	const float eps = u2f((127u - kStack) << 23u);
+	dx = fabsf(dx) > eps ? dx : (dx >= 0 ? eps : -eps);
+	dy = fabsf(dy) > eps ? dy : (dy >= 0 ? eps : -eps);
+	dz = fabsf(dz) > eps ? dz : (dz >= 0 ? eps : -eps);
+	float tcx = 1.0f / -fabsf(dx), tcy = 1.0f / -fabsf(dy), tcz = 1.0f / -fabsf(dz);
+	float tbx = tcx * ox, tby = tcy * oy, tbz = tcz * oz;
+	u32 oct = 0;
+	if (dx > 0.0f)
+		oct ^= 1u, tbx = 3.0f * tcx - tbx;
+	if (dy > 0.0f)
+		oct ^= 2u, tby = 3.0f * tcy - tby;
+	if (dz > 0.0f)
+		oct ^= 4u, tbz = 3.0f * tcz - tbz;
+	float t_min = fmx(fmx(2.0f * tcx - tbx, 2.0f * tcy - tby), 2.0f * tcz - tbz);
+	float t_max = fmn(fmn(tcx - tbx, tcy - tby), tcz - tbz);
+	float h = t_max;
+	t_min = fmx(t_min, 0.0f);
+	t_max = fmn(t_max, 1.0f);
+	u32 parent = root, child_bits = 0;
+	float px = 1.0f, py = 1.0f, pz = 1.0f;
+	u32 idx = 0;
+	if (1.5f * tcx - tbx > t_min)
+		idx ^= 1u, px = 1.5f;
+	if (1.5f * tcy - tby > t_min)
+		idx ^= 2u, py = 1.5f;
+	if (1.5f * tcz - tbz > t_min)
+		idx ^= 4u, pz = 1.5f;
+	u32 scale = kStack - 1;
+	float scale_exp2 = 0.5f;
+	const u32 leaf_scale = kStack - leaf_level;
+	for (;;) {
+		// A valid ray takes at most a few thousand steps; the cap only guards the device against a
+		// malformed pool (it is never reached by the parity cases).
+		if (++iter > kMaxIter)
+			return out;
+		if (child_bits == 0u) {
+			if (scale > leaf_scale)
+				child_bits = W[parent];
+			else if (scale == leaf_scale) { // DAG_GetLeafFirstChildBits (trace.frag:56-67)
+				u32 l0 = W[parent], l1 = W[parent + 1u];
+				child_bits = ((l0 & 0x000000FFu) ? 0x01u : 0u) | ((l0 & 0x0000FF00u) ? 0x02u : 0u) |
+				             ((l0 & 0x00FF0000u) ? 0x04u : 0u) | ((l0 & 0xFF000000u) ? 0x08u : 0u) |
+				             ((l1 & 0x000000FFu) ? 0x10u : 0u) | ((l1 & 0x0000FF00u) ? 0x20u : 0u) |
+				             ((l1 & 0x00FF0000u) ? 0x40u : 0u) | ((l1 & 0xFF000000u) ? 0x80u : 0u);
+			} else
+				child_bits = parent;
+		}
+		float tx = px * tcx - tbx, ty = py * tcy - tby, tz = pz * tcz - tbz;
+		float tc_max = fmn(fmn(tx, ty), tz);
+		u32 child_shift = idx ^ oct, child_mask = 1u << child_shift;
+		if ((child_bits & child_mask) != 0 && t_min <= t_max) {
+			float half = scale_exp2 * 0.5f;
+			float cx = half * tcx + tx, cy = half * tcy + ty, cz = half * tcz + tz;
+			if (scale < leaf_scale || scale_exp2 * proj_factor < tc_max + proj_bias)
+				break;
+			if (tc_max < h)
+				stack[(scale - leaf_scale) * stride] = parent;
+			h = tc_max;
+			parent = scale > leaf_scale ? W[parent + 1u + (u32)__builtin_popcount(child_bits & (child_mask - 1u))]
+			                            : (W[parent + (child_shift >> 2u)] >> ((child_shift & 3u) << 3u)) & 0xFFu;
+			idx = 0u;
+			--scale;
+			scale_exp2 = half;
+			if (cx > t_min)
+				idx ^= 1u, px += scale_exp2;
+			if (cy > t_min)
+				idx ^= 2u, py += scale_exp2;
+			if (cz > t_min)
+				idx ^= 4u, pz += scale_exp2;
+			child_bits = 0;
+			continue;
+		}
+		u32 step = 0;
+		if (tx <= tc_max)
+			step ^= 1u, px -= scale_exp2;
+		if (ty <= tc_max)
+			step ^= 2u, py -= scale_exp2;
+		if (tz <= tc_max)
+			step ^= 4u, pz -= scale_exp2;
+		t_min = tc_max;
+		idx ^= step;
+		if ((idx & step) != 0) {
+			u32 diff = 0;
+			if (step & 1u)
+				diff |= f2u(px) ^ f2u(px + scale_exp2);
+			if (step & 2u)
+				diff |= f2u(py) ^ f2u(py + scale_exp2);
+			if (step & 4u)
+				diff |= f2u(pz) ^ f2u(pz + scale_exp2);
+			scale = diff ? 31u - (u32)__builtin_clz(diff) : 0xFFFFFFFFu; // findMSB(0) = -1
+			if (scale >= kStack)
+				break;
+			scale_exp2 = u2f((scale - kStack + 127u) << 23u);
+			parent = stack[(scale - leaf_scale) * stride];
+			u32 shx = f2u(px) >> scale, shy = f2u(py) >> scale, shz = f2u(pz) >> scale;
+			px = u2f(shx << scale);
+			py = u2f(shy << scale);
+			pz = u2f(shz << scale);
+			idx = (shx & 1u) | ((shy & 1u) << 1u) | ((shz & 1u) << 2u);
+			h = 0.0f;
+			child_bits = 0;
+		}
+	}
+	// normal / voxel / hit position (trace.frag:223-258)
+	float ex = tcx * (px + scale_exp2) - tbx, ey = tcy * (py + scale_exp2) - tby, ez = tcz * (pz + scale_exp2) - tbz;
+	float n[3];
+	if (ex > ey && ex > ez)
+		n[0] = -1, n[1] = 0, n[2] = 0;
+	else if (ey > ez)
+		n[0] = 0, n[1] = -1, n[2] = 0;
+	else
+		n[0] = 0, n[1] = 0, n[2] = -1;
+	if ((oct & 1u) == 0u)
+		n[0] = -n[0];
+	if ((oct & 2u) == 0u)
+		n[1] = -n[1];
+	if ((oct & 4u) == 0u)
+		n[2] = -n[2];
+	const u32 voxel_level = leaf_level + 1u, voxel_scale = kStack - voxel_level;
+	const bool ok = scale < kStack && t_min <= t_max;
+	out.iter = iter;
+	if (!ok) { // a miss reports zeros (trace.frag leaves them unused)
+		for (int i = 0; i < 3; ++i)
+			out.vox[i] = 0, out.hit_pos[i] = 0.0f, out.norm[i] = 0.0f, out.pick[i] = 0.0f;
+		out.vox_size = 0;
+		return out;
+	}
+	u32 vs = 1u << (scale - voxel_scale);
+	float pp[3] = {px, py, pz}, oo[3] = {ox, oy, oz}, dd[3] = {dx, dy, dz};
+	for (int i = 0; i < 3; ++i) {
+		u32 v = (f2u(pp[i]) & 0x7FFFFFu) >> voxel_scale;
+		if (oct & (1u << i))
+			v = (1u << voxel_level) - vs - v;
+		out.vox[i] = v;
+		float vmin = u2f((v << voxel_scale) | (127u << 23u)), vmax = u2f(((v + vs) << voxel_scale) | (127u << 23u));
+		float hp = fmn(fmx(oo[i] + t_min * dd[i], vmin), vmax);
+		if (n[i] != 0)
+			hp = n[i] > 0 ? vmax + eps * 2 : vmin - eps;
+		out.hit_pos[i] = hp - 1.0f;
+		out.norm[i] = n[i];
+		// NodePoolTraversal: undo mirroring of pos, clamp(o + t d, pos, pos + scale) - 1
+		float q = pp[i];
+		if (oct & (1u << i))
+			q = 3.0f - scale_exp2 - q;
+		out.pick[i] = fmn(fmx(oo[i] + t_min * dd[i], q), q + scale_exp2) - 1.0f;
+	}
+	out.vox_size = vs;
+	out.iter = iter;
+	out.hit = ok;
+	return out;
+}
+
+struct RGB {
+	float r, g, b;
+};
+__device__ inline RGB unorm8(u32 c) {
+	return {float(c & 0xFFu) / 255.0f, float((c >> 8) & 0xFFu) / 255.0f, float((c >> 16) & 0xFFu) / 255.0f};
+}
+__device__ inline RGB rgb565(u32 c) {
+	return {float(c & 0x1Fu) / 31.0f, float((c >> 5) & 0x3Fu) / 63.0f, float((c >> 11) & 0x1Fu) / 31.0f};
+}
+// Color_GetLeafColor (trace.frag:304-349)
+__device__ inline RGB leaf_color(const u32 *__restrict__ L, u32 idx, u32 x, u32 y, u32 z) {
+	u32 macro_cnt = L[idx + 1u], block_cnt = L[idx + 2u];
+	u32 macro_offset = idx + 4u, block_offset = macro_offset + (macro_cnt << 1u);
+	u32 weight_offset = block_offset + (block_cnt << 1u);
+	u32 vox_id = morton3(x, y, z);
+	u32 macro_id = vox_id >> 14u;
+	if (macro_id >= macro_cnt)
+		return {0, 0, 0};
+	u32 mx = L[macro_offset + (macro_id << 1u)], my = L[(macro_offset + (macro_id << 1u)) | 1u];
+	block_offset += mx << 1u;
+	block_cnt = macro_id + 1u < macro_cnt ? L[macro_offset + ((macro_id + 1u) << 1u)] - mx : block_cnt - mx;
+	vox_id &= 0x3FFFu;
+	if (block_cnt == 0u)
+		return {0, 0, 0};
+	for (u32 s = 0; s <= 14u && block_cnt != 0u; ++s) {
+		u32 step = block_cnt >> 1u;
+		if ((L[(block_offset + (step << 1u)) | 1u] >> 18u) <= vox_id) {
+			block_cnt -= (step + 1u);
+			block_offset += (step + 1u) << 1u;
+		} else
+			block_cnt = step;
+	}
+	block_offset -= 2u;
+	u32 bx = L[block_offset], by = L[block_offset | 1u];
+	u32 bpw = (by >> 16u) & 0x3u;
+	if (bpw == 0u)
+		return unorm8(bx);
+	vox_id -= (by >> 18u);
+	u32 bit_id = my + (by & 0xFFFFu) + vox_id * bpw;
+	u32 bo = bit_id & 31u;
+	u32 w = L[weight_offset + (bit_id >> 5u)] >> bo;
+	if (bo + bpw <= 32u)
+		w &= (1u << bpw) - 1u;
+	else
+		w = w | ((L[weight_offset + (bit_id >> 5u) + 1u] & ((1u << (bo + bpw - 32u)) - 1u)) << (32u - bo));
+	float a = float(w) / float((1u << bpw) - 1u);
+	RGB p = rgb565(bx), q = rgb565(bx >> 16u);
+	return {p.r * (1.0f - a) + q.r * a, p.g * (1.0f - a) + q.g * a, p.b * (1.0f - a) + q.b * a};
+}
+// Color_Fetch (trace.frag:351-364)
+__device__ inline RGB color_fetch(const u32 *__restrict__ CN, const u32 *__restrict__ CL, u32 root, u32 voxel_level,
+                                  u32 leaf_level, u32 x, u32 y, u32 z) {
+	u32 ptr = root;
+	for (u32 l = 0; l < leaf_level; ++l) {
+		u32 tag = ptr >> 30u, data = ptr & 0x3FFFFFFFu;
+		if (tag != 0)
+			return unorm8(data);
+		u32 sh = voxel_level - 1u - l;
+		ptr = CN[((u64)ptr << 3u) | ((x >> sh) & 1u) | (((y >> sh) & 1u) << 1u) | (((z >> sh) & 1u) << 2u)];
+	}
+	u32 tag = ptr >> 30u, data = ptr & 0x3FFFFFFFu;
+	if (tag == 2u) {
+		u32 m = (1u << (voxel_level - leaf_level)) - 1u;
+		return leaf_color(CL, data, x & m, y & m, z & m);
+	}
+	return unorm8(data);
+}
+
+struct TraceArgs {
+	const u32 *words, *cnodes, *cleaves;
+	u32 root, color_root, leaf_level, color_leaf_level;
+	const hd_view *views;
+	const u32 *tile_start; // prefix of tiles per view (n_views + 1)
+	u32 n_views;
+	u32 *rgba, *records;
+};
+
+__device__ inline u32 q8(float x) { return (u32)(fmn(fmx(x, 0.0f), 1.0f) * 255.0f + 0.5f); }
+
+__global__ void __launch_bounds__(kThreads) k_trace(TraceArgs A) {
+	__shared__ u32 stack_lds[kStack * kThreads];
+	// view of this tile (binary search over the tile prefix)
+	u32 tile = blockIdx.x, lo = 0, hi = A.n_views;
+	while (hi - lo > 1) {
+		u32 mid = (lo + hi) >> 1;
+		if (A.tile_start[mid] <= tile)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	const hd_view V = A.views[lo];
+	u32 t = tile - A.tile_start[lo];
+	u32 tiles_x = (V.width + kTile - 1) / kTile;
+	u32 px = (t % tiles_x) * kTile + (threadIdx.x % kTile), py = (t / tiles_x) * kTile + (threadIdx.x / kTile);
+	if (px >= V.width || py >= V.height)
+		return;
+	u32 *stack = stack_lds + threadIdx.x;
+	// Camera_GenRay (trace.frag:366-370), gl_FragCoord at pixel centres, origin top-left
+	float cx = (float(px) + 0.5f) / float(V.width), cy = (float(py) + 0.5f) / float(V.height);
+	cx = cx * 2.0f - 1.0f;
+	cy = cy * 2.0f - 1.0f;
+	float dx = V.look[0] - V.side[0] * cx - V.up[0] * cy, dy = V.look[1] - V.side[1] * cx - V.up[1] * cy,
+	      dz = V.look[2] - V.side[2] * cx - V.up[2] * cy;
+	float len = sqrtf(dx * dx + dy * dy + dz * dz);
+	dx = dx / len, dy = dy / len, dz = dz / len;
+	MarchOut h = march(A.words, A.root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx, dy, dz,
+	                   stack, kThreads);
+	float lx = 4.0f, ly = 5.0f, lz = 3.0f;
+	float ll = sqrtf(lx * lx + ly * ly + lz * lz);
+	lx = lx / ll, ly = ly / ll, lz = lz / ll;
+	bool shadow = false;
+	if (h.hit && (V.flags & HD_TRACE_SHADOW)) {
+		MarchOut s = march(A.words, A.root, A.leaf_level, __builtin_inff(), 0.0f, h.hit_pos[0], h.hit_pos[1],
+		                   h.hit_pos[2], lx, ly, lz, stack, kThreads);
+		shadow = s.hit;
+	}
+	float r = 0, g = 0, b = 0;
+	if (h.hit) {
+		float nd = h.norm[0] * lx + h.norm[1] * ly + h.norm[2] * lz;
+		float diffuse = shadow ? 0.5f : fmx(nd, 0.0f) * 0.5f + 0.5f;
+		RGB c = color_fetch(A.cnodes, A.cleaves, A.color_root, A.leaf_level + 1u, A.color_leaf_level, h.vox[0],
+		                    h.vox[1], h.vox[2]);
+		r = diffuse * c.r, g = diffuse * c.g, b = diffuse * c.b;
+	}
+	u64 o = V.out_offset + (u64)py * V.width + px;
+	A.rgba[o] = q8(r) | (q8(g) << 8) | (q8(b) << 16) | (255u << 24);
+	if (A.records && (V.flags & HD_TRACE_RECORDS)) {
+		u32 *rec = A.records + o * 12u;
+		rec[0] = h.vox[0], rec[1] = h.vox[1], rec[2] = h.vox[2];
+		rec[3] = h.vox_size | (h.hit ? 0x80000000u : 0u);
+		rec[4] = f2u(h.hit_pos[0]), rec[5] = f2u(h.hit_pos[1]), rec[6] = f2u(h.hit_pos[2]);
+		rec[7] = f2u(h.norm[0]), rec[8] = f2u(h.norm[1]), rec[9] = f2u(h.norm[2]);
+		rec[10] = h.iter;
+		rec[11] = shadow ? 1u : 0u;
+	}
+}
+
+__global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ W, u32 root, u32 leaf_level,
+                                                         const float *__restrict__ o, const float *__restrict__ d,
+                                                         u64 n, float *__restrict__ out) {
+	__shared__ u32 stack_lds[kStack * kThreads];
+	u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	MarchOut h = march(W, root, leaf_level, __builtin_inff(), 0.0f, o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i],
+	                   d[3 * i + 1], d[3 * i + 2], stack_lds + threadIdx.x, kThreads);
+	out[4 * i] = h.hit ? h.pick[0] : 0.0f;
+	out[4 * i + 1] = h.hit ? h.pick[1] : 0.0f;
+	out[4 * i + 2] = h.hit ? h.pick[2] : 0.0f;
+	out[4 * i + 3] = h.hit ? 1.0f : 0.0f;
+}
+
+} // namespace hd
+
+using namespace hd;
+
+extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd_view *views, uint32_t n_views,
+                        uint32_t *rgba, uint32_t *records) {
+	if (!h || !views || !n_views || !rgba)
+		return set_error("hd_trace: null argument"), HD_ERR_ARG;
+	Pool &p = h->p;
+	std::vector<u32> tiles(n_views + 1, 0);
+	for (u32 v = 0; v < n_views; ++v) {
+		if (!views[v].width || !views[v].height)
+			return set_error("hd_trace: empty view"), HD_ERR_ARG;
+		if ((views[v].flags & HD_TRACE_RECORDS) && !records)
+			return set_error("hd_trace: records requested without a buffer"), HD_ERR_ARG;
+		tiles[v + 1] = tiles[v] + div_up(views[v].width, kTile) * div_up(views[v].height, kTile);
+	}
+	hd_view *dv;
+	u32 *dt;
+	HD_RET(p.scratch.get<hd_view>("trace_views", n_views, &dv));
+	HD_RET(p.scratch.get<u32>("trace_tiles", n_views + 1, &dt));
+	HD_TRY(hipMemcpyAsync(dv, views, n_views * sizeof(hd_view), hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipMemcpyAsync(dt, tiles.data(), (n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
+	TraceArgs A{p.words, p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, dv, dt, n_views, rgba, records};
+	k_trace<<<tiles[n_views], kThreads, 0, p.stream>>>(A);
+	HD_TRY(hipGetLastError());
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+
+// Asynchronous variant for timing loops and graph capture: views and the tile prefix already on the
+// device (hd_trace_prepare), nothing copied, no synchronisation.
+extern "C" int hd_trace_async(hd_pool *h, uint32_t root, uint32_t color_root, const hd_view *d_views,
+                              const uint32_t *d_tile_start, uint32_t n_views, uint32_t n_tiles, uint32_t *rgba,
+                              uint32_t *records) {
+	Pool &p = h->p;
+	TraceArgs A{p.words, p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
+	            n_views, rgba, records};
+	k_trace<<<n_tiles, kThreads, 0, p.stream>>>(A);
+	HD_TRY(hipGetLastError());
+	return HD_OK;
+}
+
+extern "C" int hd_trace_tiles(const hd_view *views, uint32_t n_views, uint32_t *tile_start) {
+	tile_start[0] = 0;
+	for (u32 v = 0; v < n_views; ++v)
+		tile_start[v + 1] = tile_start[v] + div_up(views[v].width, kTile) * div_up(views[v].height, kTile);
+	return HD_OK;
+}
+
+extern "C" int hd_traversal(hd_pool *h, uint32_t root, const float *o, const float *d, uint64_t n, float *out) {
+	Pool &p = h->p;
+	if (!n)
+		return HD_OK;
+	k_traversal<<<div_up(n, kThreads), kThreads, 0, p.stream>>>(p.words, root, p.dc.L, o, d, n, out);
+	HD_TRY(hipGetLastError());
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
