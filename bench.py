@@ -1,0 +1,209 @@
+"""Headline benchmark: edit -> render steps on a 4096^3 value-noise terrain on one MI355X.
+
+A step = one batch of 256 sphere add/remove edits (radii log-uniform in [4, 128], centred on the
+terrain surface, adds colored) applied through the plain per-edit device path, then one
+1920x1080 frame with a primary ray per pixel and one shadow ray per hit. Every step uses its own
+seeded batch, so the pool keeps growing as in interactive editing.
+
+Prints ONE JSON line. value = edited voxels/s of the timed steps (the lattice points inside each
+edit's shape, clipped to the grid, summed); rays/s is reported beside it.
+The CPU baseline is the compiled reference's NodePoolThreadedEdit::ThreadedEdit (oracle/_ref,
+built from the VkHashDAG headers) on the host's cores, started from the device pool's exact
+state and fed the same edits (a bounded prefix of the first batch)."""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "voxel edits/s + Mrays/s (4096³ terrain, 1080p); batched vs plain; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--res", type=int, default=4096)
+    ap.add_argument("--edits", type=int, default=256)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--mode", choices=["plain", "batched"], default="plain")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed, not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from vkhashdag_amd import capi
+    from vkhashdag_amd import workloads as W
+
+    res = args.res
+    cfg = W.config_for_resolution(res, bucket_bits=16, wbpp=9, pbpb=2, color_leaf_depth=4,
+                                  color_node_capacity=1 << 24, color_leaf_capacity=1 << 28)
+    t = time.time()
+    heights = W.value_noise_heightmap(res, 6, 1, res // 8, (res * 3) // 4)
+    log(f"heightmap {res}^2 in {time.time() - t:.1f}s")
+    pool = capi.DevicePool(cfg, heights=heights)
+    t = time.time()
+    pool.edit(W.terrain_edit(res, seed=1, skin=8))
+    st = pool.stats()
+    log(f"terrain build {time.time() - t:.2f}s, used words {st.used_words}, color leaves {st.color_leaf_words}")
+    state0 = None if args.no_cpu else pool.state()
+
+    batches = [W.terrain_sphere_edits(args.edits, res, heights, 4, 128, seed=1000 + s)
+               for s in range(args.warmup + args.steps)]
+    vox = [W.edited_voxels(b, res) for b in batches]
+    mode = capi.MODE_BATCHED if args.mode == "batched" else capi.MODE_PLAIN
+
+    # camera: seeded orbit at 0.9 * res around the terrain centre, LOD on (TracePass.cpp:115-118)
+    view_block = W.orbit_view((res / 2, res / 2, res * 0.3), 0.9 * res, 7, args.width, args.height, res=res,
+                              elevation=(0.35, 0.6))
+    v = capi.make_view(view_block, args.width, args.height, capi.TRACE_SHADOW)
+    tiles = np.zeros(2, np.uint32)
+    capi.check(capi.lib().hd_trace_tiles(C_ref(v), 1, tiles.ctypes.data))
+    dview = capi.DeviceBuffer(72)
+    dview.upload(np.frombuffer(bytes(v), np.uint8))
+    dtiles = capi.DeviceBuffer(8)
+    dtiles.upload(tiles)
+    npx = args.width * args.height
+    rgba = capi.DeviceBuffer(npx * 4)
+    counters = capi.DeviceBuffer(32)
+
+    def render(count=False):
+        if count:
+            counters.upload(np.zeros(4, np.uint64))
+        pool.timer_begin()
+        capi.check(capi.lib().hd_trace_async(pool.h, pool.root, pool.color_root, dview.ptr, dtiles.ptr, 1,
+                                             int(tiles[1]), rgba.ptr, None, counters.ptr if count else None))
+        return pool.timer_end()
+
+    edit_ms, trace_ms, step_ms = [], [], []
+    for s in range(args.warmup + args.steps):
+        pool.sync()
+        t0 = time.perf_counter()
+        pool.edit(batches[s], mode)
+        t1 = time.perf_counter()
+        kms = render()
+        pool.sync()
+        t2 = time.perf_counter()
+        if s >= args.warmup:
+            edit_ms.append((t1 - t0) * 1e3)
+            trace_ms.append(kms)
+            step_ms.append((t2 - t0) * 1e3)
+        log(f"step {s}: edit {1e3 * (t1 - t0):.1f} ms ({vox[s] / 1e6:.1f} M voxels), trace kernel {kms:.3f} ms")
+
+    timed_vox = sum(vox[args.warmup:])
+    edit_s = sum(edit_ms) / 1e3
+    value = timed_vox / edit_s
+    # instrumented frame (untimed): rays, iterations, pool words read
+    render(count=True)
+    c = counters.download(np.uint64, 4)
+    primary, shadow, iters, words = (int(x) for x in c)
+    rays = primary + shadow
+    trace_avg_ms = float(np.mean(trace_ms))
+    rays_per_s = rays / (trace_avg_ms / 1e3)
+    bytes_frame = 4.0 * words + 4.0 * npx  # words read by the march + color fetch, RGBA8 output
+    achieved = bytes_frame / (trace_avg_ms / 1e3) / 1e9
+    st = pool.stats()
+    log(f"edits: {value / 1e9:.3f} G voxels/s; trace: {rays_per_s / 1e9:.3f} G rays/s "
+        f"({primary} primary + {shadow} shadow, {iters / max(rays, 1):.1f} iterations/ray, "
+        f"{words / max(rays, 1):.1f} words/ray)")
+
+    cpu = None
+    if state0 is not None:
+        cpu = cpu_baseline(cfg, heights, state0, batches[0], args.cpu_seconds, res)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "edited voxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(np.mean(step_ms)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{res}^3 value-noise terrain (6 octaves, seed 1, z in [{res // 8},{(res * 3) // 4}]), "
+                        f"{args.edits} sphere add/remove edits per step (r log-uniform 4..128, adds VBR-colored), "
+                        f"{args.mode} device edit path, then a {args.width}x{args.height} frame "
+                        f"(primary + 1 shadow ray per hit, LOD on)",
+            "edits_per_step": args.edits,
+            "resolution": res,
+            "frame": [args.width, args.height],
+            "edit_mode": args.mode,
+        },
+        "edit_ms_per_step": float(np.mean(edit_ms)),
+        "rays_per_s": rays_per_s,
+        "mrays_per_s": rays_per_s / 1e6,
+        "trace_ms": trace_avg_ms,
+        "rays_per_frame": rays,
+        "roofline": {
+            "kernel": "k_trace (DAG_RayMarch + shadow + Color_Fetch)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_frame,
+        },
+        "cpu_baseline": cpu,
+        "pool": {"used_words": int(st.used_words), "color_nodes": int(st.color_nodes),
+                 "color_leaf_words": int(st.color_leaf_words), "overflow": int(st.overflow)},
+    }
+    print(json.dumps(out))
+
+
+def C_ref(v):
+    import ctypes
+    return ctypes.byref(v)
+
+
+def cpu_baseline(cfg, heights, state0, batch, budget_s, res):
+    """The reference's ThreadedEdit (libfork busy_pool over the host cores) from the same state."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O  # the checker (oracle/_ref)
+    from vkhashdag_amd import workloads as W
+
+    if not O.have_ref():
+        log("oracle/_ref not built: no reference CPU baseline")
+        return None
+    workers = min(16, os.cpu_count() or 1)
+    ref = O.RefPool(cfg, heights=heights, workers=workers)
+    ref.load_state(state0)
+    done, secs = 0, 0.0
+    while done < len(batch) and secs < budget_s:
+        chunk = batch[done:done + 8]
+        secs += ref.edit_timed(chunk, threaded=True)
+        done += len(chunk)
+        log(f"cpu baseline: {done} edits in {secs:.1f}s")
+    vox = W.edited_voxels(batch[:done], res)
+    return {"value": vox / secs, "unit": "edited voxels/s", "cores": workers, "kind": "reference",
+            "sample": f"first {done} edits of the first batch ({vox / 1e6:.1f} M voxels), "
+                      f"NodePoolThreadedEdit::ThreadedEdit with VBREditorWrapper, libfork busy_pool({workers}), "
+                      f"max_task_level = color leaf level, from the device pool's state after the terrain build",
+            "seconds": secs}
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,10 @@ int hd_pool_read_bucket_words(hd_pool *pool, uint64_t first, uint64_t count, uin
 int hd_pool_write_bucket_words(hd_pool *pool, uint64_t first, uint64_t count, const uint32_t *in);
 int hd_pool_read_color_nodes(hd_pool *pool, uint64_t first, uint64_t count, uint32_t *out); /* 8 u32 per node */
 int hd_pool_read_color_leaves(hd_pool *pool, uint64_t first, uint64_t count, uint32_t *out);
+/* Used words of every bucket packed in bucket order (*n = their count; out may be null to query). */
+int hd_pool_read_live_words(hd_pool *pool, uint32_t *out, uint64_t cap, uint64_t *n);
+/* Sets all bucket counters and the used words of every bucket (inverse of the above). */
+int hd_pool_write_live_words(hd_pool *pool, const uint32_t *bucket_words, const uint32_t *live, uint64_t n);
 /* NodePoolBase::make_filled_node_pointers (NodePool.hpp:240-262); creates the chain on first use. */
 int hd_pool_filled(hd_pool *pool, uint32_t *out, uint32_t cap, uint32_t *n);
 int hd_pool_stats(hd_pool *pool, hd_stats *stats);
@@ -172,6 +176,18 @@ enum {
  * vox[3], vox_size | hit << 31, hit_pos[3] (f32), norm[3] (f32), iterations, shadow hit. */
 int hd_trace(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *views, uint32_t n_views,
              uint32_t *rgba, uint32_t *records);
+
+/* Asynchronous trace with views and the per-view tile prefix (hd_trace_tiles) already on the device;
+ * no copies, no synchronisation. counters: device u64[4] (zeroed by the caller) selects the
+ * instrumented variant counting [primary rays, shadow rays, march iterations, pool words read]. */
+int hd_trace_async(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *d_views,
+                   const uint32_t *d_tile_start, uint32_t n_views, uint32_t n_tiles, uint32_t *rgba,
+                   uint32_t *records, uint64_t *counters);
+/* Host helper: tile_start[0..n_views] prefix of 16x16 tiles per view. */
+int hd_trace_tiles(const hd_view *views, uint32_t n_views, uint32_t *tile_start);
+/* HIP-event timing of the work queued on the pool's stream between the two calls. */
+int hd_timer_begin(hd_pool *pool);
+int hd_timer_end(hd_pool *pool, float *ms);
 
 /* NodePoolTraversal::Traversal<float> (NodePoolTraversal.hpp:93-256) for n rays given as device
  * arrays o[3n], d[3n]; out: device float[4n] = hit position in [0,1]^3 + hit flag. */

@@ -314,4 +314,27 @@ void hdo_dda(const uint8_t *grid, u32 res, const float *o, const float *d, u64 n
 	}
 }
 
+// Same state loader as hdr_load_live for the oracle.
+void hdo_load_live(void *p, const u32 *bucket_words, const u32 *live, u64 nlive, const u32 *filled, u32 nfilled,
+                   const u32 *cnodes, u64 ncnodes, const u32 *cleaves, u64 ncleaves) {
+	auto *o = static_cast<OPool *>(p);
+	auto &pool = o->pool;
+	const auto &cfg = pool.cfg;
+	u64 nb = cfg.total_buckets(), at = 0;
+	for (u64 b = 0; b < nb; ++b) {
+		u32 bw = bucket_words[b];
+		pool.bucket_words[b] = bw;
+		for (u32 i = 0; i < bw; ++i) {
+			u64 w = (b << cfg.word_bits_per_bucket()) + i;
+			pool.page((u32)(w >> cfg.wbpp))[w & (cfg.words_per_page() - 1u)] = live[at++];
+		}
+	}
+	(void)nlive;
+	pool.filled.assign(filled, filled + nfilled);
+	o->color.nodes.resize(ncnodes);
+	for (u64 i = 0; i < ncnodes; ++i)
+		std::copy(cnodes + 8 * i, cnodes + 8 * i + 8, o->color.nodes[i].begin());
+	o->color.leaves.assign(cleaves, cleaves + ncleaves);
+}
+
 } // extern "C"

@@ -419,4 +419,37 @@ u64 hdr_vbr_write(const u32 *src, u32 src_words, const u32 *ops, u32 nops, u32 *
 	return o.size();
 }
 
+// Load a pool state (per-bucket counters + their used words packed in bucket order, the filled
+// chain and the color arrays), e.g. one downloaded from the device, so the reference continues
+// from exactly the same state.
+void hdr_load_live(void *p, const u32 *bucket_words, const u32 *live, u64 nlive, const u32 *filled, u32 nfilled,
+                   const u32 *cnodes, u64 ncnodes, const u32 *cleaves, u64 ncleaves) {
+	auto *r = static_cast<Ref *>(p);
+	auto &pool = r->pool;
+	const auto &cfg = pool.GetConfig();
+	u64 nb = cfg.GetTotalBuckets(), at = 0;
+	for (u64 b = 0; b < nb; ++b) {
+		u32 bw = bucket_words[b];
+		pool.m_bucket_words[b] = bw;
+		for (u32 off = 0; off < bw;) {
+			u32 page = (u32)((b << cfg.page_bits_per_bucket) | (off >> cfg.word_bits_per_page));
+			u32 po = off & (cfg.GetWordsPerPage() - 1u);
+			u32 n = std::min(bw - off, cfg.GetWordsPerPage() - po);
+			pool.WritePage(page, po, std::span<const u32>{live + at, n});
+			at += n;
+			off += n;
+		}
+	}
+	(void)nlive;
+	pool.m_filled_node_pointers.clear();
+	for (u32 i = 0; i < nfilled; ++i)
+		pool.m_filled_node_pointers.emplace_back(filled[i]);
+	for (u64 i = 0; i < ncnodes; ++i)
+		for (int k = 0; k < 8; ++k)
+			r->octree.nodes[i][k].pointer = cnodes[8 * i + k];
+	r->octree.node_count = ncnodes;
+	std::copy(cleaves, cleaves + ncleaves, r->octree.leaves.get());
+	r->octree.leaf_count = ncleaves;
+}
+
 } // extern "C"

@@ -76,6 +76,7 @@ def olib():
         _sig(L, "hdo_render_arrays", None,
              [u32p, U64, U32, U32, u32p, U64, u32p, U64, U32, U32, f32p, U32, U32, U32, u32p, VP])
         _sig(L, "hdo_dda", None, [u8p, U32, f32p, f32p, U64, u32p])
+        _sig(L, "hdo_load_live", None, [VP, u32p, u32p, U64, u32p, U32, u32p, U64, u32p, U64])
         _olib = L
     return _olib
 
@@ -105,6 +106,7 @@ def rlib():
         _sig(L, "hdr_hash_span", U32, [u32p, U32])
         _sig(L, "hdr_hash_leaf", U32, [U32, U32])
         _sig(L, "hdr_vbr_write", U64, [VP, U32, u32p, U32, u32p, U64])
+        _sig(L, "hdr_load_live", None, [VP, u32p, u32p, U64, u32p, U32, u32p, U64, u32p, U64])
         _rlib = L
     return _rlib
 
@@ -132,6 +134,15 @@ class _PoolBase:
 
     def full_words(self):
         return self.read_words(0, self.cfg.total_words)
+
+    def load_state(self, st):
+        """Continue from a state dict (vkhashdag_amd.capi.DevicePool.state())."""
+        f = olib().hdo_load_live if isinstance(self, OraclePool) else rlib().hdr_load_live
+        a = lambda x: np.ascontiguousarray(x if len(x) else np.zeros(1, np.uint32), np.uint32)  # noqa: E731
+        cn, cl = st["color_nodes"], st["color_leaves"]
+        f(self.h, a(st["bucket_words"]), a(st["live"]), len(st["live"]), a(st["filled"]), len(st["filled"]), a(cn),
+          len(cn) // 8, a(cl), len(cl))
+        self.root, self.color_root = int(st["root"]), int(st["color_root"])
 
     def canonical(self, roots=None):
         roots = np.asarray([self.root] if roots is None else roots, np.uint32)

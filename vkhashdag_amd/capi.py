@@ -75,7 +75,11 @@ def lib():
             "hd_edit": (C.c_int, [VP, U32, U32, VP, U32, U32, P32, P32]),
             "hd_gc": (C.c_int, [VP, VP, U32]),
             "hd_trace": (C.c_int, [VP, U32, U32, VP, U32, VP, VP]),
-            "hd_trace_async": (C.c_int, [VP, U32, U32, VP, VP, U32, U32, VP, VP]),
+            "hd_trace_async": (C.c_int, [VP, U32, U32, VP, VP, U32, U32, VP, VP, VP]),
+            "hd_timer_begin": (C.c_int, [VP]),
+            "hd_timer_end": (C.c_int, [VP, C.POINTER(C.c_float)]),
+            "hd_pool_read_live_words": (C.c_int, [VP, VP, U64, C.POINTER(U64)]),
+            "hd_pool_write_live_words": (C.c_int, [VP, VP, VP, U64]),
             "hd_trace_tiles": (C.c_int, [VP, U32, VP]),
             "hd_traversal": (C.c_int, [VP, U32, VP, VP, U64, VP]),
             "hd_device_alloc": (C.c_int, [U64, C.POINTER(VP)]),
@@ -202,12 +206,35 @@ class DevicePool:
         return out
 
     def live_words(self):
+        """(per-bucket word counters, used words of every bucket packed in bucket order)."""
         bw = self.bucket_words()
-        words = self.full_words()
-        wpb = self.cfg.words_per_bucket
-        nz = np.nonzero(bw)[0]
-        parts = [words[int(b) * wpb: int(b) * wpb + int(bw[b])] for b in nz]
-        return bw, (np.concatenate(parts) if parts else np.zeros(0, np.uint32))
+        n = U64(0)
+        check(lib().hd_pool_read_live_words(self.h, None, 0, C.byref(n)))
+        out = np.empty(max(1, n.value), np.uint32)
+        check(lib().hd_pool_read_live_words(self.h, out.ctypes.data, len(out), C.byref(n)))
+        return bw, out[: n.value]
+
+    def write_live_words(self, bucket_words, live):
+        bucket_words = np.ascontiguousarray(bucket_words, np.uint32)
+        live = np.ascontiguousarray(live, np.uint32)
+        check(lib().hd_pool_write_live_words(self.h, bucket_words.ctypes.data, live.ctypes.data, len(live)))
+
+    def state(self):
+        """Everything a CPU pool needs to continue from this state."""
+        bw, live = self.live_words()
+        return dict(bucket_words=bw, live=live, filled=self.filled(), color_nodes=self.color_nodes(),
+                    color_leaves=self.color_leaves(), root=self.root, color_root=self.color_root)
+
+    def timer_begin(self):
+        check(lib().hd_timer_begin(self.h))
+
+    def timer_end(self):
+        ms = C.c_float(0)
+        check(lib().hd_timer_end(self.h, C.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        check(lib().hd_pool_sync(self.h))
 
     def filled(self):
         out = np.zeros(64, np.uint32)

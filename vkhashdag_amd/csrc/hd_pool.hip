@@ -77,6 +77,44 @@ __global__ void k_pyramid(const u32 *src_mn, const u32 *src_mx, u32 *dst_mn, u32
 	dst_mx[i] = max(max(src_mx[a], src_mx[a + 1]), max(src_mx[b], src_mx[b + 1]));
 }
 
+// Used words of every bucket, packed in bucket order (off = exclusive prefix of bucket_words).
+__global__ void k_gather_live(const u32 *__restrict__ words, const u32 *__restrict__ bucket_words,
+                              const u64 *__restrict__ off, u64 nb, u32 wbpb, u32 *__restrict__ out) {
+	u64 b = blockIdx.x;
+	if (b >= nb)
+		return;
+	u32 n = bucket_words[b];
+	const u32 *src = words + (b << wbpb);
+	u32 *dst = out + off[b];
+	for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+		dst[i] = src[i];
+}
+__global__ void k_scatter_live(u32 *__restrict__ words, const u32 *__restrict__ bucket_words,
+                               const u64 *__restrict__ off, u64 nb, u32 wbpb, const u32 *__restrict__ in) {
+	u64 b = blockIdx.x;
+	if (b >= nb)
+		return;
+	u32 n = bucket_words[b];
+	u32 *dst = words + (b << wbpb);
+	const u32 *src = in + off[b];
+	for (u32 i = threadIdx.x; i < n; i += blockDim.x)
+		dst[i] = src[i];
+}
+
+static int live_offsets(Pool &p, u64 **d_off, u64 *total) {
+	std::vector<u32> bw(p.total_buckets);
+	HD_TRY(hipMemcpyAsync(bw.data(), p.bucket_words, p.total_buckets * 4, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	std::vector<u64> off(p.total_buckets + 1, 0);
+	for (u64 b = 0; b < p.total_buckets; ++b)
+		off[b + 1] = off[b] + bw[b];
+	*total = off[p.total_buckets];
+	HD_RET(p.scratch.get<u64>("live_off", p.total_buckets + 1, d_off));
+	HD_TRY(hipMemcpyAsync(*d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+
 } // namespace hd
 
 using namespace hd;
@@ -331,3 +369,43 @@ int hd_device_count(int *n) {
 }
 
 } // extern "C"
+
+// Used words of every bucket packed in bucket order (the dirty-range Flush payload of
+// src/DAGNodePool.cpp:56-85 for a full flush). *n receives the count; out may be null to query it.
+extern "C" int hd_pool_read_live_words(hd_pool *h, uint32_t *out, uint64_t cap, uint64_t *n) {
+	Pool &p = h->p;
+	u64 *off, total;
+	HD_RET(live_offsets(p, &off, &total));
+	*n = total;
+	if (!out)
+		return HD_OK;
+	if (cap < total)
+		return set_error("hd_pool_read_live_words: buffer too small"), HD_ERR_ARG;
+	u32 *tmp;
+	HD_RET(p.scratch.get<u32>("live_tmp", total + 1, &tmp));
+	k_gather_live<<<(u32)p.total_buckets, 256, 0, p.stream>>>(p.words, p.bucket_words, off, p.total_buckets, p.dc.wbpb(),
+	                                                          tmp);
+	HD_TRY(hipGetLastError());
+	HD_TRY(hipMemcpyAsync(out, tmp, total * 4, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+// Inverse: set every bucket's counter and used words (host interop / checkpoint load).
+extern "C" int hd_pool_write_live_words(hd_pool *h, const uint32_t *bucket_words, const uint32_t *live, uint64_t n) {
+	Pool &p = h->p;
+	HD_TRY(hipMemcpyAsync(p.bucket_words, bucket_words, p.total_buckets * 4, hipMemcpyHostToDevice, p.stream));
+	u64 *off, total;
+	HD_RET(live_offsets(p, &off, &total));
+	if (total != n)
+		return set_error("hd_pool_write_live_words: %llu words given, counters sum to %llu", (unsigned long long)n,
+		                 (unsigned long long)total),
+		       HD_ERR_ARG;
+	u32 *tmp;
+	HD_RET(p.scratch.get<u32>("live_tmp", total + 1, &tmp));
+	HD_TRY(hipMemcpyAsync(tmp, live, total * 4, hipMemcpyHostToDevice, p.stream));
+	k_scatter_live<<<(u32)p.total_buckets, 256, 0, p.stream>>>(p.words, p.bucket_words, off, p.total_buckets,
+	                                                           p.dc.wbpb(), tmp);
+	HD_TRY(hipGetLastError());
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}

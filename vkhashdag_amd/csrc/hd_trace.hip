@@ -32,9 +32,11 @@ struct MarchOut {
 };
 
 // DAG_RayMarch (trace.frag:69-270). stack: LDS base for this thread, stride = blockDim.
+// kCount: also count the pool words the march reads (reads += ...), for the algorithmic-bytes pass.
+template <bool kCount>
 __device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_level, float proj_factor,
                                  float proj_bias, float ox, float oy, float oz, float dx, float dy, float dz,
-                                 u32 *stack, u32 stride) {
+                                 u32 *stack, u32 stride, u32 &reads) {
 	MarchOut out;
 	out.hit = false;
 	out.iter = 0;
@@ -81,10 +83,14 @@ __device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_l
 		if (++iter > kMaxIter)
 			return out;
 		if (child_bits == 0u) {
-			if (scale > leaf_scale)
+			if (scale > leaf_scale) {
 				child_bits = W[parent];
-			else if (scale == leaf_scale) { // DAG_GetLeafFirstChildBits (trace.frag:56-67)
+				if (kCount)
+					reads += 1;
+			} else if (scale == leaf_scale) { // DAG_GetLeafFirstChildBits (trace.frag:56-67)
 				u32 l0 = W[parent], l1 = W[parent + 1u];
+				if (kCount)
+					reads += 2;
 				child_bits = ((l0 & 0x000000FFu) ? 0x01u : 0u) | ((l0 & 0x0000FF00u) ? 0x02u : 0u) |
 				             ((l0 & 0x00FF0000u) ? 0x04u : 0u) | ((l0 & 0xFF000000u) ? 0x08u : 0u) |
 				             ((l1 & 0x000000FFu) ? 0x10u : 0u) | ((l1 & 0x0000FF00u) ? 0x20u : 0u) |
@@ -103,6 +109,8 @@ __device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_l
 			if (tc_max < h)
 				stack[(scale - leaf_scale) * stride] = parent;
 			h = tc_max;
+			if (kCount)
+				reads += 1;
 			parent = scale > leaf_scale ? W[parent + 1u + (u32)__builtin_popcount(child_bits & (child_mask - 1u))]
 			                            : (W[parent + (child_shift >> 2u)] >> ((child_shift & 3u) << 3u)) & 0xFFu;
 			idx = 0u;
@@ -207,8 +215,11 @@ __device__ inline RGB rgb565(u32 c) {
 	return {float(c & 0x1Fu) / 31.0f, float((c >> 5) & 0x3Fu) / 63.0f, float((c >> 11) & 0x1Fu) / 31.0f};
 }
 // Color_GetLeafColor (trace.frag:304-349)
-__device__ inline RGB leaf_color(const u32 *__restrict__ L, u32 idx, u32 x, u32 y, u32 z) {
+template <bool kCount>
+__device__ inline RGB leaf_color(const u32 *__restrict__ L, u32 idx, u32 x, u32 y, u32 z, u32 &reads) {
 	u32 macro_cnt = L[idx + 1u], block_cnt = L[idx + 2u];
+	if (kCount)
+		reads += 5; // counts, macro pair, next macro
 	u32 macro_offset = idx + 4u, block_offset = macro_offset + (macro_cnt << 1u);
 	u32 weight_offset = block_offset + (block_cnt << 1u);
 	u32 vox_id = morton3(x, y, z);
@@ -223,6 +234,8 @@ __device__ inline RGB leaf_color(const u32 *__restrict__ L, u32 idx, u32 x, u32 
 		return {0, 0, 0};
 	for (u32 s = 0; s <= 14u && block_cnt != 0u; ++s) {
 		u32 step = block_cnt >> 1u;
+		if (kCount)
+			reads += 1;
 		if ((L[(block_offset + (step << 1u)) | 1u] >> 18u) <= vox_id) {
 			block_cnt -= (step + 1u);
 			block_offset += (step + 1u) << 1u;
@@ -232,6 +245,8 @@ __device__ inline RGB leaf_color(const u32 *__restrict__ L, u32 idx, u32 x, u32 
 	block_offset -= 2u;
 	u32 bx = L[block_offset], by = L[block_offset | 1u];
 	u32 bpw = (by >> 16u) & 0x3u;
+	if (kCount)
+		reads += 2 + (bpw ? 2 : 0);
 	if (bpw == 0u)
 		return unorm8(bx);
 	vox_id -= (by >> 18u);
@@ -247,20 +262,23 @@ __device__ inline RGB leaf_color(const u32 *__restrict__ L, u32 idx, u32 x, u32 
 	return {p.r * (1.0f - a) + q.r * a, p.g * (1.0f - a) + q.g * a, p.b * (1.0f - a) + q.b * a};
 }
 // Color_Fetch (trace.frag:351-364)
+template <bool kCount>
 __device__ inline RGB color_fetch(const u32 *__restrict__ CN, const u32 *__restrict__ CL, u32 root, u32 voxel_level,
-                                  u32 leaf_level, u32 x, u32 y, u32 z) {
+                                  u32 leaf_level, u32 x, u32 y, u32 z, u32 &reads) {
 	u32 ptr = root;
 	for (u32 l = 0; l < leaf_level; ++l) {
 		u32 tag = ptr >> 30u, data = ptr & 0x3FFFFFFFu;
 		if (tag != 0)
 			return unorm8(data);
 		u32 sh = voxel_level - 1u - l;
+		if (kCount)
+			reads += 1;
 		ptr = CN[((u64)ptr << 3u) | ((x >> sh) & 1u) | (((y >> sh) & 1u) << 1u) | (((z >> sh) & 1u) << 2u)];
 	}
 	u32 tag = ptr >> 30u, data = ptr & 0x3FFFFFFFu;
 	if (tag == 2u) {
 		u32 m = (1u << (voxel_level - leaf_level)) - 1u;
-		return leaf_color(CL, data, x & m, y & m, z & m);
+		return leaf_color<kCount>(CL, data, x & m, y & m, z & m, reads);
 	}
 	return unorm8(data);
 }
@@ -272,11 +290,12 @@ struct TraceArgs {
 	const u32 *tile_start; // prefix of tiles per view (n_views + 1)
 	u32 n_views;
 	u32 *rgba, *records;
+	unsigned long long *counters; // [primary rays, shadow rays, march iterations, words read] (may be null)
 };
 
 __device__ inline u32 q8(float x) { return (u32)(fmn(fmx(x, 0.0f), 1.0f) * 255.0f + 0.5f); }
 
-__global__ void __launch_bounds__(kThreads) k_trace(TraceArgs A) {
+template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(TraceArgs A) {
 	__shared__ u32 stack_lds[kStack * kThreads];
 	// view of this tile (binary search over the tile prefix)
 	u32 tile = blockIdx.x, lo = 0, hi = A.n_views;
@@ -302,23 +321,26 @@ __global__ void __launch_bounds__(kThreads) k_trace(TraceArgs A) {
 	      dz = V.look[2] - V.side[2] * cx - V.up[2] * cy;
 	float len = sqrtf(dx * dx + dy * dy + dz * dz);
 	dx = dx / len, dy = dy / len, dz = dz / len;
-	MarchOut h = march(A.words, A.root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx, dy, dz,
-	                   stack, kThreads);
+	u32 reads = 0;
+	MarchOut h = march<kCount>(A.words, A.root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx,
+	                           dy, dz, stack, kThreads, reads);
+	u32 iters = h.iter;
 	float lx = 4.0f, ly = 5.0f, lz = 3.0f;
 	float ll = sqrtf(lx * lx + ly * ly + lz * lz);
 	lx = lx / ll, ly = ly / ll, lz = lz / ll;
 	bool shadow = false;
 	if (h.hit && (V.flags & HD_TRACE_SHADOW)) {
-		MarchOut s = march(A.words, A.root, A.leaf_level, __builtin_inff(), 0.0f, h.hit_pos[0], h.hit_pos[1],
-		                   h.hit_pos[2], lx, ly, lz, stack, kThreads);
+		MarchOut s = march<kCount>(A.words, A.root, A.leaf_level, __builtin_inff(), 0.0f, h.hit_pos[0], h.hit_pos[1],
+		                           h.hit_pos[2], lx, ly, lz, stack, kThreads, reads);
 		shadow = s.hit;
+		iters += s.iter;
 	}
 	float r = 0, g = 0, b = 0;
 	if (h.hit) {
 		float nd = h.norm[0] * lx + h.norm[1] * ly + h.norm[2] * lz;
 		float diffuse = shadow ? 0.5f : fmx(nd, 0.0f) * 0.5f + 0.5f;
-		RGB c = color_fetch(A.cnodes, A.cleaves, A.color_root, A.leaf_level + 1u, A.color_leaf_level, h.vox[0],
-		                    h.vox[1], h.vox[2]);
+		RGB c = color_fetch<kCount>(A.cnodes, A.cleaves, A.color_root, A.leaf_level + 1u, A.color_leaf_level,
+		                            h.vox[0], h.vox[1], h.vox[2], reads);
 		r = diffuse * c.r, g = diffuse * c.g, b = diffuse * c.b;
 	}
 	u64 o = V.out_offset + (u64)py * V.width + px;
@@ -332,6 +354,12 @@ __global__ void __launch_bounds__(kThreads) k_trace(TraceArgs A) {
 		rec[10] = h.iter;
 		rec[11] = shadow ? 1u : 0u;
 	}
+	if (kCount && A.counters) {
+		atomicAdd(&A.counters[0], 1ull);
+		atomicAdd(&A.counters[1], h.hit && (V.flags & HD_TRACE_SHADOW) ? 1ull : 0ull);
+		atomicAdd(&A.counters[2], (unsigned long long)iters);
+		atomicAdd(&A.counters[3], (unsigned long long)reads);
+	}
 }
 
 __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ W, u32 root, u32 leaf_level,
@@ -341,8 +369,9 @@ __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ 
 	u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	MarchOut h = march(W, root, leaf_level, __builtin_inff(), 0.0f, o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i],
-	                   d[3 * i + 1], d[3 * i + 2], stack_lds + threadIdx.x, kThreads);
+	u32 reads = 0;
+	MarchOut h = march<false>(W, root, leaf_level, __builtin_inff(), 0.0f, o[3 * i], o[3 * i + 1], o[3 * i + 2],
+	                          d[3 * i], d[3 * i + 1], d[3 * i + 2], stack_lds + threadIdx.x, kThreads, reads);
 	out[4 * i] = h.hit ? h.pick[0] : 0.0f;
 	out[4 * i + 1] = h.hit ? h.pick[1] : 0.0f;
 	out[4 * i + 2] = h.hit ? h.pick[2] : 0.0f;
@@ -372,8 +401,9 @@ extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd
 	HD_RET(p.scratch.get<u32>("trace_tiles", n_views + 1, &dt));
 	HD_TRY(hipMemcpyAsync(dv, views, n_views * sizeof(hd_view), hipMemcpyHostToDevice, p.stream));
 	HD_TRY(hipMemcpyAsync(dt, tiles.data(), (n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
-	TraceArgs A{p.words, p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, dv, dt, n_views, rgba, records};
-	k_trace<<<tiles[n_views], kThreads, 0, p.stream>>>(A);
+	TraceArgs A{p.words, p.cnodes, p.cleaves, root,   color_root, p.dc.L, p.dc.color_leaf,
+	            dv,      dt,       n_views,   rgba, records, nullptr};
+	k_trace<false><<<tiles[n_views], kThreads, 0, p.stream>>>(A);
 	HD_TRY(hipGetLastError());
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
@@ -381,13 +411,18 @@ extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd
 
 // Asynchronous variant for timing loops and graph capture: views and the tile prefix already on the
 // device (hd_trace_prepare), nothing copied, no synchronisation.
+// counters (device u64[4], zeroed by the caller) selects the instrumented variant that also counts
+// rays, march iterations and the pool words read (the algorithmic bytes of the frame).
 extern "C" int hd_trace_async(hd_pool *h, uint32_t root, uint32_t color_root, const hd_view *d_views,
                               const uint32_t *d_tile_start, uint32_t n_views, uint32_t n_tiles, uint32_t *rgba,
-                              uint32_t *records) {
+                              uint32_t *records, uint64_t *counters) {
 	Pool &p = h->p;
-	TraceArgs A{p.words, p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
-	            n_views, rgba, records};
-	k_trace<<<n_tiles, kThreads, 0, p.stream>>>(A);
+	TraceArgs A{p.words,      p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
+	            n_views,      rgba,     records,   (unsigned long long *)counters};
+	if (counters)
+		k_trace<true><<<n_tiles, kThreads, 0, p.stream>>>(A);
+	else
+		k_trace<false><<<n_tiles, kThreads, 0, p.stream>>>(A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
 }
@@ -406,5 +441,25 @@ extern "C" int hd_traversal(hd_pool *h, uint32_t root, const float *o, const flo
 	k_traversal<<<div_up(n, kThreads), kThreads, 0, p.stream>>>(p.words, root, p.dc.L, o, d, n, out);
 	HD_TRY(hipGetLastError());
 	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+
+// Event timing on the pool stream: hd_timer_begin records a start event, hd_timer_end records an
+// end event, waits for it and returns the elapsed milliseconds between the two.
+namespace {
+hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
+}
+extern "C" int hd_timer_begin(hd_pool *h) {
+	if (!g_ev0) {
+		HD_TRY(hipEventCreate(&g_ev0));
+		HD_TRY(hipEventCreate(&g_ev1));
+	}
+	HD_TRY(hipEventRecord(g_ev0, h->p.stream));
+	return HD_OK;
+}
+extern "C" int hd_timer_end(hd_pool *h, float *ms) {
+	HD_TRY(hipEventRecord(g_ev1, h->p.stream));
+	HD_TRY(hipEventSynchronize(g_ev1));
+	HD_TRY(hipEventElapsedTime(ms, g_ev0, g_ev1));
 	return HD_OK;
 }

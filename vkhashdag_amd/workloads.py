@@ -128,8 +128,9 @@ def _lattice(ix, iy, octave, seed):
 
 def value_noise_heightmap(res: int, octaves: int, seed: int, zlo: int, zhi: int,
                           base_cells: int = 4, lacunarity: float = 2.0, gain: float = 0.5) -> np.ndarray:
-    """res x res heights (uint32, row-major, x fastest) in [zlo, zhi]: fractal value noise with
-    smoothstep-interpolated lattices, base frequency base_cells cells across the grid."""
+    """res x res heights (uint32, row-major [y, x], x fastest) in [zlo, zhi]: fractal value noise
+    (smoothstep-interpolated hashed lattices, base frequency base_cells cells across the grid,
+    each octave lacunarity x the frequency and gain x the amplitude), normalised to [0, 1]."""
     xs = (np.arange(res, dtype=np.float64) + 0.5) / res
     acc = np.zeros((res, res), np.float64)
     amp, freq, norm = 1.0, float(base_cells), 0.0
@@ -138,15 +139,12 @@ def value_noise_heightmap(res: int, octaves: int, seed: int, zlo: int, zhi: int,
         i0 = np.floor(fx).astype(np.int64)
         t = fx - i0
         t = t * t * (3 - 2 * t)
-        X0, Y0 = np.meshgrid(i0, i0, indexing="xy")
-        TX, TY = np.meshgrid(t, t, indexing="xy")
-        v00 = _lattice(X0, Y0, o, seed)
-        v10 = _lattice(X0 + 1, Y0, o, seed)
-        v01 = _lattice(X0, Y0 + 1, o, seed)
-        v11 = _lattice(X0 + 1, Y0 + 1, o, seed)
-        a = v00 + (v10 - v00) * TX
-        b = v01 + (v11 - v01) * TX
-        acc += amp * (a + (b - a) * TY)
+        n = int(i0.max()) + 2
+        idx = np.arange(n, dtype=np.int64)
+        lat = _lattice(idx[None, :], idx[:, None], o, seed)  # [iy, ix]
+        top = lat[np.ix_(i0, i0)] * (1 - t)[None, :] + lat[np.ix_(i0, i0 + 1)] * t[None, :]
+        bot = lat[np.ix_(i0 + 1, i0)] * (1 - t)[None, :] + lat[np.ix_(i0 + 1, i0 + 1)] * t[None, :]
+        acc += amp * (top * (1 - t)[:, None] + bot * t[:, None])
         norm += amp
         amp *= gain
         freq *= lacunarity
