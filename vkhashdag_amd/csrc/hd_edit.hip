@@ -792,12 +792,11 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 	}
 	lv[0].n = 1;
 	u64 entries = 1;
-	// ---- top-down
-	PushEvent *ev;
-	u32 *ev_count, ev_cap = 1u << 16;
-	HD_RET(p.scratch.get<PushEvent>("events", ev_cap, &ev));
-	HD_RET(p.scratch.get<u32>("ev_count", 1, &ev_count));
-	HD_TRY(hipMemsetAsync(ev_count, 0, 4, p.stream));
+	// ---- top-down (push events of level l's children go to a per-level buffer of 8 n_l entries)
+	std::vector<PushEvent *> ev(L, nullptr);
+	u32 *ev_count;
+	HD_RET(p.scratch.get<u32>("ev_count", L, &ev_count));
+	HD_TRY(hipMemsetAsync(ev_count, 0, 4 * L, p.stream));
 	u32 last = 0;
 	for (u32 l = 0; l + 1 < L; ++l) {
 		u32 n = lv[l].n;
@@ -811,11 +810,14 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 			HD_RET(buf(p, "fchild", l, n, &lv[l].first_child));
 			HD_RET(buf(p, "nchild", l, n, &lv[l].nchild));
 		}
+		const bool events = P.stateful && l + 1 > leafL;
+		if (events)
+			HD_RET(buf(p, "events", l, n8, &ev[l]));
 		HD_RET(buf(p, "flag", 0, n8 + 1, &flag));
 		HD_RET(buf(p, "off", 0, n8 + 1, &off));
 		HD_RET(buf(p, "cstate", 0, 2 * n8, &cstate));
 		k_expand<<<div_up(n8 + 1, 256), 256, 0, p.stream>>>(P, l, lv[l].ent, n, lv[l].kids, lv[l].okids, flag, cstate,
-		                                                    ev, ev_count, ev_cap);
+		                                                    ev[l], ev_count + l, events ? (u32)n8 : 0u);
 		HD_TRY(hipGetLastError());
 		HD_RET(excl_scan(p, flag, off, n8 + 1));
 		u32 nn;
@@ -833,11 +835,9 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 	p.last_entries += entries;
 	// ---- colors: dense decode + pushes
 	Color *dense = nullptr;
-	u32 nev = 0;
 	if (P.stateful && lv[leafL].n) {
-		HD_RET(read_u32s(p, ev_count, &nev, 1));
-		if (nev > ev_cap)
-			return set_error("push event overflow"), HD_ERR_CAPACITY;
+		std::vector<u32> nev(L, 0);
+		HD_RET(read_u32s(p, ev_count, nev.data(), L));
 		u64 dense_words = (u64)lv[leafL].n * V;
 		if (dense_words * sizeof(Color) > (48ull << 30))
 			return set_error("color scratch too large (%llu leaves)", (unsigned long long)lv[leafL].n),
@@ -845,8 +845,9 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 		HD_RET(p.scratch.get<Color>("dense", dense_words, &dense));
 		u32 bpj = std::max(1u, std::min(64u, V / 1024u));
 		k_dense_init<<<lv[leafL].n * bpj, 256, 0, p.stream>>>(lv[leafL].ent, lv[leafL].n, V, bpj, p.cleaves, dense);
-		if (nev)
-			k_push<<<nev, 256, 0, p.stream>>>(ev, nev, V, dense);
+		for (u32 l = 0; l < L; ++l)
+			if (nev[l])
+				k_push<<<nev[l], 256, 0, p.stream>>>(ev[l], nev[l], V, dense);
 		HD_TRY(hipGetLastError());
 		p.last_jobs += lv[leafL].n;
 	}
