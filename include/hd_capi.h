@@ -102,6 +102,8 @@ typedef struct hd_stats {
 	uint64_t level_used_words[HD_MAX_LEVELS];
 	uint64_t last_entries, last_upserts, last_jobs; /* work of the last hd_edit */
 	double last_ms;
+	uint32_t last_groups; /* passes of the last batched hd_edit */
+	uint32_t pad_;
 } hd_stats;
 
 typedef struct hd_pool hd_pool;

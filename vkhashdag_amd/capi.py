@@ -34,7 +34,7 @@ class hd_stats(C.Structure):
                 ("color_nodes", C.c_uint64), ("color_leaf_words", C.c_uint64), ("overflow", C.c_uint32),
                 ("node_levels", C.c_uint32), ("level_used_words", C.c_uint64 * HD_MAX_LEVELS),
                 ("last_entries", C.c_uint64), ("last_upserts", C.c_uint64), ("last_jobs", C.c_uint64),
-                ("last_ms", C.c_double)]
+                ("last_ms", C.c_double), ("last_groups", C.c_uint32), ("pad_", C.c_uint32)]
 
 
 class hd_view(C.Structure):

@@ -148,7 +148,7 @@ enum EditType : u32 { kNotAffected = 0, kProceed = 1, kFill = 2, kClear = 3 }; /
 // heightfield column fill). Identical semantics to the CPU oracle's Editor.
 struct DevEditor {
 	hd_edit_desc d;
-	DevHeightmap hm;
+	const DevHeightmap *hm; // device copy of the pool's heightmap view
 	u32 voxel_level;
 
 	__device__ bool is_dig() const { return d.kind == HD_EDIT_SPHERE_DIG || d.kind == HD_EDIT_AABB_DIG; }
@@ -203,7 +203,7 @@ struct DevEditor {
 	}
 	__device__ EditType column_node(u32 level, u32 x, u32 y, u32 z) const {
 		u32 k = voxel_level - level;
-		u32 hmin = hm.fmin(k, x, y), hmax = hm.fmax(k, x, y);
+		u32 hmin = hm->fmin(k, x, y), hmax = hm->fmax(k, x, y);
 		u64 z0 = (u64)z << k, z1 = (u64)(z + 1u) << k;
 		if (z0 >= hmax)
 			return kNotAffected;
@@ -211,12 +211,12 @@ struct DevEditor {
 			return kFill;
 		return kProceed;
 	}
-	__device__ bool column_voxel(u32 x, u32 y, u32 z) const { return z < hm.h(x, y); }
+	__device__ bool column_voxel(u32 x, u32 y, u32 z) const { return z < hm->h(x, y); }
 	__device__ u32 column_rgb8(u32 x, u32 y, u32 z) const {
 		u32 bx = x >> 2, by = y >> 2, bz = z >> 2;
-		if ((u64)bz * 4u + 4u + d.skin <= hm.fmin(2, bx, by))
+		if ((u64)bz * 4u + 4u + d.skin <= hm->fmin(2, bx, by))
 			return d.color.colors & 0xFFFFFFu;
-		return terrain_block_rgb8(d.r2, bx, by, bz, hm.res);
+		return terrain_block_rgb8(d.r2, bx, by, bz, hm->res);
 	}
 	__device__ bool column_uniform(u32 level, u32 x, u32 y, u32 z, u32 &rgb) const {
 		u32 k = voxel_level - level;
@@ -225,7 +225,7 @@ struct DevEditor {
 			return true;
 		}
 		u64 z1 = (u64)(z + 1u) << k;
-		if (z1 + d.skin <= hm.fmin(k, x, y)) {
+		if (z1 + d.skin <= hm->fmin(k, x, y)) {
 			rgb = d.color.colors & 0xFFFFFFu;
 			return true;
 		}

@@ -20,6 +20,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include <chrono>
+#include <cmath>
 
 namespace hd {
 
@@ -27,25 +28,39 @@ struct Entry {
 	u32 x, y, z;  // node coordinate at this level
 	u32 ptr;      // old node pointer (kNull if none)
 	u32 parent;   // index of the parent entry
-	u32 sf;       // bits 0-2 child slot, bit 3 is_final, bit 4 has a color writer
+	u32 sf;       // bits 0-2 child slot, 3 is_final, 4 has a color writer, 6 color-active
 	u32 octree;   // VBREditorWrapper::NodeState::octree_node
 	u32 job;      // color leaf job (entry index at the color leaf level) or kNull
+	u32 loff;     // edit list: the edit index when llen == 1, else offset into the level's list array
+	u32 llen;
 };
-constexpr u32 kSfFinal = 8u, kSfWriter = 16u, kSfNewJob = 32u;
+constexpr u32 kSfFinal = 8u, kSfWriter = 16u, kSfNewJob = 32u, kSfActive = 64u;
 
 struct PushEvent {
 	u32 job, offset, count, colors, wb, pad0, pad1, pad2;
 };
 
+// One pass applies a group of edits whose voxel AABBs are pairwise disjoint (so they commute):
+// a node reached by one edit runs that edit's wrapper logic exactly as NodePoolBase::Edit would;
+// a node reached by several proceeds with its ordered edit list and no color decision.
 struct EditParams {
 	DevConfig c;
-	DevEditor ed;
+	const DevHeightmap *hm;
+	const hd_edit_desc *edits; // the pass's edits (device)
+	u32 n_edits;
 	u32 *words, *bucket_words;
 	const u32 *filled;
 	u32 *cnodes, *cleaves, *meta;
-	u32 stateful;
+	u32 stateful; // some edit of the pass runs through VBREditorWrapper
 	u32 octree_root;
 	u32 leaf_bits; // voxel_level - color_leaf_level
+	__device__ DevEditor editor(u32 k) const {
+		DevEditor e;
+		e.d = edits[k];
+		e.hm = hm;
+		e.voxel_level = c.voxel_level;
+		return e;
+	}
 };
 
 __device__ inline u32 color_child(const u32 *cnodes, u32 p, u32 i) { // DAGColorPool::GetChild (:139-143)
@@ -53,43 +68,95 @@ __device__ inline u32 color_child(const u32 *cnodes, u32 p, u32 i) { // DAGColor
 	return t == kTagNode ? cnodes[(u64)cdata(p) * 8u + i] : (t == kTagColor ? p : kColorNull);
 }
 
+struct Classified {
+	EditType t;
+	u32 octree, sf;
+	bool push;
+	Color col;
+};
+
+// edit_switch + VBREditorWrapper::EditNode (NodePool.hpp:345-360, VBREditor.hpp:37-71) for one
+// edit on the node (cl, cx, cy, cz). octree: the node's state octree (GetChild already applied
+// for cl <= color leaf level); sf: slot | inherited is_final / writer bits.
+__device__ inline Classified classify_single(const EditParams &P, const DevEditor &ed, u32 cl, u32 cx, u32 cy, u32 cz,
+                                            u32 old, u32 octree, u32 sf) {
+	Classified r{kNotAffected, octree, sf, false, Color::none()};
+	if (!ed.stateful()) {
+		r.t = ed.edit_node(cl, cx, cy, cz);
+		return r;
+	}
+	r.sf |= kSfActive;
+	const u32 leafL = P.c.color_leaf;
+	Color col = get_fill(octree);
+	r.t = ed.edit_node_vbr(cl, cx, cy, cz, old != kNull, col);
+	if (!(r.sf & kSfFinal)) {
+		if (cl <= leafL) {
+			if (col.has()) {
+				r.octree = cmake(kTagColor, color_to_rgb8(col));
+				r.sf |= kSfFinal;
+			} else if (r.t == kClear)
+				r.octree = kColorNull;
+			else if (r.t == kProceed && cl == leafL)
+				r.sf |= kSfWriter | kSfNewJob;
+		} else if (r.sf & kSfWriter) {
+			if (col.has()) { // writer->Push(color, 8^(voxel_level - level)); Copy keeps the decoded colors
+				r.push = true;
+				r.col = col;
+				r.sf |= kSfFinal;
+			}
+		}
+	}
+	return r;
+}
+
 // ---------------------------------------------------------------- top-down
-__global__ void k_root(EditParams P, u32 root, Entry *ent0, u32 *out) {
+// out: [0] = 1 if the root proceeds, [1] = root result otherwise, [2] = color root state
+__global__ void k_root(EditParams P, u32 root, Entry *ent0, u32 *list0, u32 *out) {
 	if (threadIdx.x | blockIdx.x)
 		return;
-	u32 octree = P.octree_root, sf = 0;
-	EditType t;
-	if (!P.stateful)
-		t = P.ed.edit_node(0, 0, 0, 0);
-	else {
-		Color col = get_fill(octree);
-		t = P.ed.edit_node_vbr(0, 0, 0, 0, root != kNull, col);
-		if (col.has()) {
-			octree = cmake(kTagColor, color_to_rgb8(col));
-			sf |= kSfFinal;
-		} else if (t == kClear)
-			octree = kColorNull;
-		else if (t == kProceed && P.c.color_leaf == 0)
-			sf |= kSfWriter;
+	u32 cnt = 0, single = 0;
+	bool active = false;
+	if (P.n_edits == 1)
+		cnt = 1;
+	else
+		for (u32 k = 0; k < P.n_edits; ++k) {
+			DevEditor e = P.editor(k);
+			if (e.geom(0, 0, 0, 0) != kNotAffected) {
+				list0[cnt++] = k;
+				single = k;
+				active |= e.stateful();
+			}
+		}
+	Classified c{kNotAffected, P.octree_root, 0u, false, Color::none()};
+	if (cnt == 1)
+		c = classify_single(P, P.editor(single), 0, 0, 0, 0, root, P.octree_root, 0u);
+	else if (cnt > 1) {
+		c.t = kProceed;
+		if (active)
+			c.sf |= kSfActive | (P.c.color_leaf == 0 ? (kSfWriter | kSfNewJob) : 0u);
 	}
-	if (t == kProceed) {
-		ent0[0] = Entry{0, 0, 0, root, kNull, sf, octree, (sf & kSfWriter) ? 0u : kNull};
+	if (c.t == kProceed) {
+		ent0[0] = Entry{0, 0, 0, root, kNull, c.sf & ~kSfNewJob, c.octree, (c.sf & kSfWriter) ? 0u : kNull,
+		                cnt == 1 ? single : 0u, cnt};
 		out[0] = 1;
 	} else {
 		out[0] = 0;
-		out[1] = t == kClear ? kNull : (t == kFill ? P.filled[0] : root);
-		out[2] = octree;
+		out[1] = c.t == kClear ? kNull : (c.t == kFill ? P.filled[0] : root);
+		out[2] = c.octree;
 	}
 }
 
-__global__ void k_expand(EditParams P, u32 l, const Entry *__restrict__ ent, u32 n, u32 *__restrict__ kids,
-                         u32 *__restrict__ okids, u32 *__restrict__ flag, u32 *__restrict__ cstate,
-                         PushEvent *__restrict__ ev, u32 *__restrict__ ev_count, u32 ev_cap) {
+// One lane per (node, child). Writes the child's new pointer when it terminates, its color state,
+// the proceed flag and the length of its filtered edit list (lists of 2+ edits are scanned).
+__global__ void k_expand(EditParams P, u32 l, const Entry *__restrict__ ent, u32 n, const u32 *__restrict__ lists,
+                         u32 *__restrict__ kids, u32 *__restrict__ okids, u32 *__restrict__ flag,
+                         u32 *__restrict__ lcnt, uint4 *__restrict__ cstate, PushEvent *__restrict__ ev,
+                         u32 *__restrict__ ev_count, u32 ev_cap) {
 	u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
 	u64 total = (u64)n * 8u;
 	if (t >= total) {
 		if (t == total)
-			flag[t] = 0;
+			flag[t] = 0, lcnt[t] = 0;
 		return;
 	}
 	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
@@ -100,52 +167,56 @@ __global__ void k_expand(EditParams P, u32 l, const Entry *__restrict__ ent, u32
 		if ((m >> i) & 1u)
 			old = P.words[E.ptr + 1u + (u32)__builtin_popcount(m & ((1u << i) - 1u))];
 	}
-	const u32 cl = l + 1;
+	const u32 cl = l + 1, leafL = P.c.color_leaf;
 	const u32 cx = (E.x << 1) | (i & 1u), cy = (E.y << 1) | ((i >> 1) & 1u), cz = (E.z << 1) | ((i >> 2) & 1u);
-	u32 octree = E.octree, sf = i | (E.sf & (kSfFinal | kSfWriter));
-	EditType tt;
-	if (!P.stateful)
-		tt = P.ed.edit_node(cl, cx, cy, cz);
+	u32 octree = E.octree;
+	if (P.stateful && cl <= leafL)
+		octree = color_child(P.cnodes, E.octree, i);
+	const u32 sf_in = i | (E.sf & (kSfFinal | kSfWriter));
+	Classified c{kNotAffected, octree, sf_in, false, Color::none()};
+	u32 cnt = 1, single = E.loff;
+	if (E.llen == 1)
+		c = classify_single(P, P.editor(E.loff), cl, cx, cy, cz, old, octree, sf_in);
 	else {
-		// VBREditorWrapper::EditNode (VBREditor.hpp:37-71)
-		const u32 leafL = P.c.color_leaf;
-		if (cl <= leafL)
-			octree = color_child(P.cnodes, E.octree, i);
-		Color col = get_fill(octree);
-		tt = P.ed.edit_node_vbr(cl, cx, cy, cz, old != kNull, col);
-		if (!(sf & kSfFinal)) {
-			if (cl <= leafL) {
-				if (col.has()) {
-					octree = cmake(kTagColor, color_to_rgb8(col));
-					sf |= kSfFinal;
-				} else if (tt == kClear)
-					octree = kColorNull;
-				else if (tt == kProceed && cl == leafL)
-					sf |= kSfWriter | kSfNewJob;
-			} else if (sf & kSfWriter) {
-				if (col.has()) { // writer->Push(color, 8^(voxel_level - level)) over this node's Morton range
-					u32 d = cl - leafL, msk = (1u << d) - 1u, sh = 3u * (P.c.voxel_level - cl);
-					u32 k = atomicAdd(ev_count, 1u);
-					if (k < ev_cap)
-						ev[k] = PushEvent{E.job, morton3(cx & msk, cy & msk, cz & msk) << sh, 1u << sh, col.colors, col.wb,
-						                  0, 0, 0};
-					sf |= kSfFinal;
-				}
-				// Copy(count, fill) keeps the decoded source colors: nothing to do.
+		cnt = 0;
+		bool active = false;
+		for (u32 k = 0; k < E.llen; ++k) {
+			u32 idx = lists[E.loff + k];
+			DevEditor ed = P.editor(idx);
+			if (ed.geom(cl, cx, cy, cz) != kNotAffected) {
+				++cnt;
+				single = idx;
+				active |= ed.stateful();
 			}
 		}
+		if (cnt == 1)
+			c = classify_single(P, P.editor(single), cl, cx, cy, cz, old, octree, sf_in);
+		else if (cnt > 1) {
+			c.t = kProceed;
+			if (active)
+				c.sf |= kSfActive | (cl == leafL ? (kSfWriter | kSfNewJob) : 0u);
+		}
 	}
-	kids[t] = tt == kClear ? kNull : (tt == kFill ? P.filled[cl] : old);
+	if (c.push) {
+		u32 d = cl - leafL, msk = (1u << d) - 1u, sh = 3u * (P.c.voxel_level - cl);
+		u32 k = atomicAdd(ev_count, 1u);
+		if (k < ev_cap)
+			ev[k] = PushEvent{E.job, morton3(cx & msk, cy & msk, cz & msk) << sh, 1u << sh, c.col.colors, c.col.wb, 0, 0, 0};
+	}
+	kids[t] = c.t == kClear ? kNull : (c.t == kFill ? P.filled[cl] : old);
 	if (okids)
-		okids[t] = octree;
-	flag[t] = tt == kProceed ? 1u : 0u;
-	cstate[2 * t] = octree;
-	cstate[2 * t + 1] = sf;
+		okids[t] = c.octree;
+	bool proceed = c.t == kProceed;
+	flag[t] = proceed ? 1u : 0u;
+	lcnt[t] = proceed && cnt > 1 ? cnt : 0u;
+	cstate[t] = make_uint4(c.octree, c.sf, single, cnt);
 }
 
-__global__ void k_emit(u32 l, const Entry *__restrict__ ent, u32 n, const u32 *__restrict__ kids,
-                       const u32 *__restrict__ flag, const u32 *__restrict__ off, const u32 *__restrict__ cstate,
-                       Entry *__restrict__ next, u32 *__restrict__ first_child, u32 *__restrict__ nchild) {
+__global__ void k_emit(u32 l, const Entry *__restrict__ ent, u32 n, const u32 *__restrict__ lists,
+                       const EditParams P, const u32 *__restrict__ kids, const u32 *__restrict__ flag,
+                       const u32 *__restrict__ off, const u32 *__restrict__ loffs, const uint4 *__restrict__ cstate,
+                       Entry *__restrict__ next, u32 *__restrict__ next_lists, u32 *__restrict__ first_child,
+                       u32 *__restrict__ nchild) {
 	u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= (u64)n * 8u)
 		return;
@@ -157,10 +228,21 @@ __global__ void k_emit(u32 l, const Entry *__restrict__ ent, u32 n, const u32 *_
 	if (!flag[t])
 		return;
 	Entry E = ent[e];
-	u32 idx = off[t], sf = cstate[2 * t + 1];
+	uint4 cs = cstate[t];
+	u32 idx = off[t], sf = cs.y;
 	u32 job = (sf & kSfNewJob) ? idx : ((sf & kSfWriter) ? E.job : kNull);
-	next[idx] = Entry{(E.x << 1) | (i & 1u), (E.y << 1) | ((i >> 1) & 1u), (E.z << 1) | ((i >> 2) & 1u), kids[t], e,
-	                  sf & 31u, cstate[2 * t], job};
+	const u32 cx = (E.x << 1) | (i & 1u), cy = (E.y << 1) | ((i >> 1) & 1u), cz = (E.z << 1) | ((i >> 2) & 1u);
+	u32 lo = cs.z, ln = cs.w;
+	if (ln > 1) { // the child's filtered list, in the parent's (submission) order
+		lo = loffs[t];
+		u32 w = lo;
+		for (u32 k = 0; k < E.llen; ++k) {
+			u32 ix = lists[E.loff + k];
+			if (P.editor(ix).geom(l + 1, cx, cy, cz) != kNotAffected)
+				next_lists[w++] = ix;
+		}
+	}
+	next[idx] = Entry{cx, cy, cz, kids[t], e, sf & ~kSfNewJob, cs.x, job, lo, ln};
 }
 
 // ---------------------------------------------------------------- colors: dense decode + pushes
@@ -224,8 +306,10 @@ __global__ void k_push(const PushEvent *__restrict__ ev, u32 nev, u32 V, Color *
 
 // ---------------------------------------------------------------- leaves
 // edit_leaf (NodePool.hpp:319-343): one wavefront per leaf, lane i = voxel i (GetLeafCoord order).
-__global__ void k_leaf(EditParams P, const Entry *__restrict__ ent, u32 n, u32 *__restrict__ status,
-                       u32 *__restrict__ pend, u32 *__restrict__ content, Color *__restrict__ dense, u32 V) {
+// A leaf with an edit list folds the voxel through the edits in order.
+__global__ void k_leaf(EditParams P, const Entry *__restrict__ ent, u32 n, const u32 *__restrict__ lists,
+                       u32 *__restrict__ status, u32 *__restrict__ pend, u32 *__restrict__ content,
+                       Color *__restrict__ dense, u32 V) {
 	const u32 lane = threadIdx.x & 63u;
 	const u32 e = (u32)(((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
 	if (e >= n)
@@ -240,19 +324,28 @@ __global__ void k_leaf(EditParams P, const Entry *__restrict__ ent, u32 n, u32 *
 	u32 vx = (E.x << 2) | ((lane >> 2) & 2u) | (lane & 1u);
 	u32 vy = (E.y << 2) | ((lane >> 3) & 2u) | ((lane >> 1) & 1u);
 	u32 vz = (E.z << 2) | ((lane >> 4) & 2u) | ((lane >> 2) & 1u);
-	bool nv;
-	if (!P.stateful)
-		nv = P.ed.edit_voxel(vx, vy, vz, v);
-	else if (E.sf & kSfFinal) {
-		Color dummy = Color::none();
-		nv = P.ed.edit_voxel_vbr(vx, vy, vz, v, dummy);
-	} else { // VBRChunkWriter::Edit (VBRColor.hpp:477-486) on the dense color of this voxel
+	const bool writer = (E.sf & kSfWriter) && !(E.sf & kSfFinal);
+	Color *cp = nullptr;
+	Color col = Color::none();
+	if (writer) { // VBRChunkWriter::Edit (VBRColor.hpp:477-486) on the dense color of this voxel
 		u32 msk = (1u << P.leaf_bits) - 1u;
-		Color *c = dense + (u64)E.job * V + morton3(vx & msk, vy & msk, vz & msk);
-		Color col = *c;
-		nv = P.ed.edit_voxel_vbr(vx, vy, vz, v, col);
-		*c = col;
+		cp = dense + (u64)E.job * V + morton3(vx & msk, vy & msk, vz & msk);
+		col = *cp;
 	}
+	bool nv = v;
+	for (u32 k = 0; k < E.llen; ++k) {
+		DevEditor ed = P.editor(E.llen == 1 ? E.loff : lists[E.loff + k]);
+		if (!ed.stateful())
+			nv = ed.edit_voxel(vx, vy, vz, nv);
+		else if (writer)
+			nv = ed.edit_voxel_vbr(vx, vy, vz, nv, col);
+		else {
+			Color dummy = Color::none();
+			nv = ed.edit_voxel_vbr(vx, vy, vz, nv, dummy);
+		}
+	}
+	if (writer && (E.sf & kSfActive))
+		*cp = col;
 	unsigned long long m = __ballot(nv);
 	if (lane == 0) {
 		u32 n0 = (u32)m, n1 = (u32)(m >> 32);
@@ -597,7 +690,7 @@ __global__ void k_setnode_a(EditParams P, const Entry *__restrict__ ent, u32 n, 
 		return;
 	Entry E = ent[e];
 	u32 r = E.octree, c = 0;
-	if (!(E.sf & kSfFinal)) {
+	if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
 		u32 ch[8];
 		bool all_null = true, same_color = true;
 		for (u32 i = 0; i < 8; ++i) {
@@ -751,38 +844,54 @@ static int upsert_level(Pool &p, const EditParams &P, u32 l, Level &lv, u32 *sta
 	return HD_OK;
 }
 
-static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *root_out, u32 *croot_out) {
+static bool is_stateful(const hd_edit_desc &d) {
+	bool dig = d.kind == HD_EDIT_SPHERE_DIG || d.kind == HD_EDIT_AABB_DIG;
+	bool paint = d.kind == HD_EDIT_SPHERE_PAINT || d.kind == HD_EDIT_AABB_PAINT;
+	return (d.colored || paint) && !dig;
+}
+
+// One level-synchronous pass over a group of edits with pairwise disjoint voxel AABBs (a single
+// edit is the plain path: NodePoolBase::Edit).
+static int edit_pass(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n_edits, u32 *root_out,
+                     u32 *croot_out) {
 	HD_RET(make_filled(p));
 	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
 	EditParams P;
 	P.c = p.dc;
-	P.ed.d = d;
-	P.ed.hm = p.hm;
-	P.ed.voxel_level = p.dc.voxel_level;
+	P.hm = p.d_hm;
+	P.n_edits = n_edits;
 	P.words = p.words;
 	P.bucket_words = p.bucket_words;
 	P.filled = p.d_filled;
 	P.cnodes = p.cnodes;
 	P.cleaves = p.cleaves;
 	P.meta = p.meta;
-	bool dig = d.kind == HD_EDIT_SPHERE_DIG || d.kind == HD_EDIT_AABB_DIG;
-	bool paint = d.kind == HD_EDIT_SPHERE_PAINT || d.kind == HD_EDIT_AABB_PAINT;
-	P.stateful = ((d.colored || paint) && !dig) ? 1u : 0u;
+	P.stateful = 0;
+	for (u32 k = 0; k < n_edits; ++k) {
+		const hd_edit_desc &d = edits[k];
+		if (d.kind > HD_EDIT_COLUMN_FILL)
+			return set_error("unknown edit kind %u", d.kind), HD_ERR_ARG;
+		if (d.kind == HD_EDIT_COLUMN_FILL && !p.hm_mn)
+			return set_error("column fill needs hd_pool_set_heightmap"), HD_ERR_ARG;
+		P.stateful |= is_stateful(d) ? 1u : 0u;
+	}
 	P.octree_root = croot;
 	P.leaf_bits = p.dc.voxel_level - leafL;
-	if (d.kind > HD_EDIT_COLUMN_FILL)
-		return set_error("unknown edit kind %u", d.kind), HD_ERR_ARG;
-	if (d.kind == HD_EDIT_COLUMN_FILL && !p.hm_mn)
-		return set_error("column fill needs hd_pool_set_heightmap"), HD_ERR_ARG;
 	if (P.stateful && P.leaf_bits > 7)
 		return set_error("color leaves deeper than 8^7 voxels are not supported"), HD_ERR_UNSUPPORTED;
 	const u32 V = P.stateful ? 1u << (3u * P.leaf_bits) : 0u;
+	hd_edit_desc *d_edits;
+	HD_RET(p.scratch.get<hd_edit_desc>("pass_edits", n_edits, &d_edits));
+	HD_TRY(hipMemcpyAsync(d_edits, edits, n_edits * sizeof(hd_edit_desc), hipMemcpyHostToDevice, p.stream));
+	P.edits = d_edits;
 
 	std::vector<Level> lv(L);
+	std::vector<u32 *> lists(L, nullptr);
 	u32 *rootbuf;
 	HD_RET(p.scratch.get<u32>("rootbuf", 8, &rootbuf));
 	HD_RET(buf(p, "ent", 0, 1, &lv[0].ent));
-	k_root<<<1, 64, 0, p.stream>>>(P, root, lv[0].ent, rootbuf);
+	HD_RET(buf(p, "lists", 0, n_edits, &lists[0]));
+	k_root<<<1, 64, 0, p.stream>>>(P, root, lv[0].ent, lists[0], rootbuf);
 	u32 rb[3];
 	HD_RET(read_u32s(p, rootbuf, rb, 3));
 	if (rb[0] == 0) {
@@ -797,13 +906,13 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 	u32 *ev_count;
 	HD_RET(p.scratch.get<u32>("ev_count", L, &ev_count));
 	HD_TRY(hipMemsetAsync(ev_count, 0, 4 * L, p.stream));
-	u32 last = 0;
 	for (u32 l = 0; l + 1 < L; ++l) {
 		u32 n = lv[l].n;
 		if (!n)
 			break;
 		u64 n8 = (u64)n * 8u;
-		u32 *flag, *off, *cstate;
+		u32 *flag, *off, *lcnt, *loffs;
+		uint4 *cstate;
 		HD_RET(buf(p, "kids", l, n8, &lv[l].kids));
 		if (P.stateful && l < leafL) {
 			HD_RET(buf(p, "okids", l, n8, &lv[l].okids));
@@ -815,23 +924,32 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 			HD_RET(buf(p, "events", l, n8, &ev[l]));
 		HD_RET(buf(p, "flag", 0, n8 + 1, &flag));
 		HD_RET(buf(p, "off", 0, n8 + 1, &off));
-		HD_RET(buf(p, "cstate", 0, 2 * n8, &cstate));
-		k_expand<<<div_up(n8 + 1, 256), 256, 0, p.stream>>>(P, l, lv[l].ent, n, lv[l].kids, lv[l].okids, flag, cstate,
-		                                                    ev[l], ev_count + l, events ? (u32)n8 : 0u);
+		HD_RET(buf(p, "lcnt", 0, n8 + 1, &lcnt));
+		HD_RET(buf(p, "loffs", 0, n8 + 1, &loffs));
+		HD_RET(buf(p, "cstate", 0, n8, &cstate));
+		k_expand<<<div_up(n8 + 1, 256), 256, 0, p.stream>>>(P, l, lv[l].ent, n, lists[l], lv[l].kids, lv[l].okids,
+		                                                    flag, lcnt, cstate, ev[l], ev_count + l,
+		                                                    events ? (u32)n8 : 0u);
 		HD_TRY(hipGetLastError());
 		HD_RET(excl_scan(p, flag, off, n8 + 1));
-		u32 nn;
-		HD_RET(read_u32s(p, off + n8, &nn, 1));
+		u32 tot[2];
+		if (n_edits > 1) {
+			HD_RET(excl_scan(p, lcnt, loffs, n8 + 1));
+			HD_TRY(hipMemcpyAsync(&tot[1], loffs + n8, 4, hipMemcpyDeviceToHost, p.stream));
+		} else
+			tot[1] = 0;
+		HD_RET(read_u32s(p, off + n8, &tot[0], 1));
+		u32 nn = tot[0];
 		lv[l + 1].n = nn;
 		entries += nn;
 		HD_RET(buf(p, "ent", l + 1, nn, &lv[l + 1].ent));
+		HD_RET(buf(p, "lists", l + 1, tot[1], &lists[l + 1]));
 		if (nn || lv[l].first_child)
-			k_emit<<<div_up(n8, 256), 256, 0, p.stream>>>(l, lv[l].ent, n, lv[l].kids, flag, off, cstate, lv[l + 1].ent,
-			                                              lv[l].first_child, lv[l].nchild);
+			k_emit<<<div_up(n8, 256), 256, 0, p.stream>>>(l, lv[l].ent, n, lists[l], P, lv[l].kids, flag, off, loffs,
+			                                              cstate, lv[l + 1].ent, lists[l + 1], lv[l].first_child,
+			                                              lv[l].nchild);
 		HD_TRY(hipGetLastError());
-		last = l + 1;
 	}
-	(void)last;
 	p.last_entries += entries;
 	// ---- colors: dense decode + pushes
 	Color *dense = nullptr;
@@ -862,7 +980,8 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 		HD_RET(buf(p, "pend", 0, n, &pend));
 		HD_RET(buf(p, "content", 0, (size_t)n * (leaf ? 2 : 9), &content));
 		if (leaf)
-			k_leaf<<<div_up((u64)n * 64u, 256), 256, 0, p.stream>>>(P, lv[l].ent, n, status, pend, content, dense, V);
+			k_leaf<<<div_up((u64)n * 64u, 256), 256, 0, p.stream>>>(P, lv[l].ent, n, lists[l], status, pend, content,
+			                                                         dense, V);
 		else {
 			HD_RET(buf(p, "csize", 0, n, &csize));
 			k_build<<<div_up(n, 256), 256, 0, p.stream>>>(P, lv[l].ent, n, lv[l].kids, status, pend, content, csize);
@@ -934,9 +1053,80 @@ static int edit_one(Pool &p, u32 root, u32 croot, const hd_edit_desc &d, u32 *ro
 	return HD_OK;
 }
 
+// Voxel AABB of an edit, clipped to the grid (inclusive bounds). Column fills cover the grid.
+static void edit_aabb(const hd_edit_desc &d, u32 res, i64 lo[3], i64 hi[3]) {
+	for (int i = 0; i < 3; ++i)
+		lo[i] = 0, hi[i] = (i64)res - 1;
+	if (d.kind <= HD_EDIT_SPHERE_PAINT) {
+		i64 r = (i64)std::sqrt((double)d.r2);
+		while ((r + 1) * (r + 1) <= (i64)d.r2)
+			++r;
+		while (r * r > (i64)d.r2)
+			--r;
+		for (int i = 0; i < 3; ++i)
+			lo[i] = std::max<i64>(lo[i], (i64)d.p0[i] - r), hi[i] = std::min<i64>(hi[i], (i64)d.p0[i] + r);
+	} else if (d.kind <= HD_EDIT_AABB_PAINT) {
+		for (int i = 0; i < 3; ++i)
+			lo[i] = std::max<i64>(lo[i], d.p0[i]), hi[i] = std::min<i64>(hi[i], (i64)d.p1[i] - 1);
+	}
+}
+
+// Level scheduling of a batch: edit k goes to the first group after every group holding an earlier
+// edit whose voxel AABB meets k's. Edits within a group have disjoint voxel sets, so they commute
+// and one pass applies them; groups run in order, so every voxel sees its edits in submission order.
+// A coarse 3-D grid of cells keeps the overlap search local.
+void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std::vector<u32>> &groups) {
+	const u32 cell_bits = res >= 64 ? std::max(4u, (u32)std::log2((double)res) - 5u) : 2u; // ~32^3 cells
+	const u32 cells = std::max(1u, res >> cell_bits);
+	std::vector<std::vector<u32>> grid((size_t)cells * cells * cells);
+	std::vector<i64> box(6 * (size_t)n);
+	std::vector<u32> gid(n, 0);
+	groups.clear();
+	for (u32 k = 0; k < n; ++k) {
+		i64 *b = &box[6 * (size_t)k];
+		edit_aabb(edits[k], res, b, b + 3);
+		u32 g = 0;
+		bool empty = b[0] > b[3] || b[1] > b[4] || b[2] > b[5];
+		i64 c0[3], c1[3];
+		if (!empty) {
+			for (int i = 0; i < 3; ++i)
+				c0[i] = b[i] >> cell_bits, c1[i] = b[i + 3] >> cell_bits;
+			for (i64 z = c0[2]; z <= c1[2]; ++z)
+				for (i64 y = c0[1]; y <= c1[1]; ++y)
+					for (i64 x = c0[0]; x <= c1[0]; ++x)
+						for (u32 j : grid[((size_t)z * cells + y) * cells + x]) {
+							const i64 *o = &box[6 * (size_t)j];
+							if (gid[j] + 1 > g && o[0] <= b[3] && b[0] <= o[3] && o[1] <= b[4] && b[1] <= o[4] &&
+							    o[2] <= b[5] && b[2] <= o[5])
+								g = gid[j] + 1;
+						}
+		}
+		gid[k] = g;
+		if (g >= groups.size())
+			groups.resize(g + 1);
+		groups[g].push_back(k);
+		if (!empty)
+			for (i64 z = c0[2]; z <= c1[2]; ++z)
+				for (i64 y = c0[1]; y <= c1[1]; ++y)
+					for (i64 x = c0[0]; x <= c1[0]; ++x)
+						grid[((size_t)z * cells + y) * cells + x].push_back(k);
+	}
+}
+
 int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
-	(void)p, (void)root, (void)croot, (void)edits, (void)n, (void)root_out, (void)croot_out;
-	return set_error("batched mode not built yet"), HD_ERR_UNSUPPORTED;
+	std::vector<std::vector<u32>> groups;
+	schedule_groups(edits, n, 1u << p.dc.voxel_level, groups);
+	std::vector<hd_edit_desc> g_edits;
+	for (auto &g : groups) {
+		g_edits.clear();
+		for (u32 k : g)
+			g_edits.push_back(edits[k]);
+		HD_RET(edit_pass(p, root, croot, g_edits.data(), (u32)g_edits.size(), &root, &croot));
+	}
+	p.last_groups = (u32)groups.size();
+	*root_out = root;
+	*croot_out = croot;
+	return HD_OK;
 }
 
 } // namespace hd
@@ -955,7 +1145,7 @@ extern "C" int hd_edit(hd_pool *h, uint32_t root, uint32_t color_root, const hd_
 		HD_RET(edit_batched(p, r, c, edits, n, &r, &c));
 	} else {
 		for (u32 i = 0; i < n; ++i)
-			HD_RET(edit_one(p, r, c, edits[i], &r, &c));
+			HD_RET(edit_pass(p, r, c, edits + i, 1, &r, &c));
 	}
 	HD_TRY(hipStreamSynchronize(p.stream));
 	p.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -90,10 +90,12 @@ struct Pool {
 	u64 cnode_cap = 0, cleaf_cap = 0;
 	u32 *hm_mn = nullptr, *hm_mx = nullptr;
 	DevHeightmap hm{};
+	DevHeightmap *d_hm = nullptr; // device copy of hm (editors read it through a pointer)
 	hipStream_t stream = nullptr;
 	Scratch scratch;
 	// statistics of the last edit (host)
 	u64 last_entries = 0, last_upserts = 0, last_jobs = 0;
+	u32 last_groups = 0;
 	double last_ms = 0;
 };
 

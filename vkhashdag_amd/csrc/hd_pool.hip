@@ -181,6 +181,10 @@ int hd_pool_create(const hd_config *cfg, hd_pool **out) {
 		return fail(e);
 	if ((e = hipMalloc(&p.d_filled, HD_MAX_LEVELS * 4)) != hipSuccess)
 		return fail(e);
+	if ((e = hipMalloc(&p.d_hm, sizeof(DevHeightmap))) != hipSuccess)
+		return fail(e);
+	if ((e = hipMemsetAsync(p.d_hm, 0, sizeof(DevHeightmap), p.stream)) != hipSuccess)
+		return fail(e);
 	if ((e = hipMalloc(&p.cnodes, (p.cnode_cap ? p.cnode_cap : 1) * 32)) != hipSuccess)
 		return fail(e);
 	if ((e = hipMalloc(&p.cleaves, (p.cleaf_cap ? p.cleaf_cap : 1) * 4)) != hipSuccess)
@@ -205,7 +209,7 @@ int hd_pool_destroy(hd_pool *h) {
 		(void)hipStreamSynchronize(p.stream);
 	p.scratch.release();
 	for (void *ptr : {(void *)p.words, (void *)p.bucket_words, (void *)p.meta, (void *)p.d_filled, (void *)p.cnodes,
-	                  (void *)p.cleaves, (void *)p.hm_mn, (void *)p.hm_mx})
+	                  (void *)p.cleaves, (void *)p.hm_mn, (void *)p.hm_mx, (void *)p.d_hm})
 		if (ptr)
 			(void)hipFree(ptr);
 	if (p.stream)
@@ -255,6 +259,7 @@ int hd_pool_set_heightmap(hd_pool *h, const uint32_t *heights, uint32_t res) {
 	hm.mn = p.hm_mn;
 	hm.mx = p.hm_mx;
 	p.hm = hm;
+	HD_TRY(hipMemcpyAsync(p.d_hm, &p.hm, sizeof(DevHeightmap), hipMemcpyHostToDevice, p.stream));
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
 }
@@ -323,6 +328,7 @@ int hd_pool_stats(hd_pool *h, hd_stats *s) {
 	s->last_upserts = p.last_upserts;
 	s->last_jobs = p.last_jobs;
 	s->last_ms = p.last_ms;
+	s->last_groups = p.last_groups;
 	return HD_OK;
 }
 int hd_pool_read_color_nodes(hd_pool *h, uint64_t first, uint64_t count, uint32_t *out) {
