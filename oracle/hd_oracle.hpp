@@ -686,6 +686,7 @@ struct NodePool {
 	std::vector<u32> bucket_words;
 	std::vector<std::unique_ptr<u32[]>> pages;
 	std::vector<u32> filled;
+	u64 overflows = 0; // upserts that fell back because the bucket was full
 
 	explicit NodePool(Config c) : cfg(std::move(c)) {
 		if (!Config::validate(cfg))
@@ -779,6 +780,7 @@ struct NodePool {
 			bucket_words[b] = nw;
 			return p;
 		}
+		++overflows;
 		return fallback;
 	}
 	// make_filled_node_pointers (NodePool.hpp:240-262)

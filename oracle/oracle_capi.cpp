@@ -88,6 +88,7 @@ void hdo_gc(void *p, u32 *roots, u32 n) {
 	std::copy(r.begin(), r.end(), roots);
 }
 
+u64 hdo_overflows(void *p) { return static_cast<OPool *>(p)->pool.overflows; }
 u64 hdo_total_words(void *p) { return static_cast<OPool *>(p)->pool.cfg.total_words(); }
 u64 hdo_total_buckets(void *p) { return static_cast<OPool *>(p)->pool.cfg.total_buckets(); }
 void hdo_read_words(void *p, u64 first, u64 count, u32 *out) {

@@ -52,6 +52,7 @@ def olib():
         _sig(L, "hdo_edit_timed", C.c_double, [VP, U32, editp, U32, C.POINTER(U32), C.POINTER(U32)])
         _sig(L, "hdo_gc", None, [VP, u32p, U32])
         _sig(L, "hdo_total_words", U64, [VP])
+        _sig(L, "hdo_overflows", U64, [VP])
         _sig(L, "hdo_total_buckets", U64, [VP])
         _sig(L, "hdo_read_words", None, [VP, U64, U64, u32p])
         _sig(L, "hdo_bucket_words", None, [VP, u32p])
@@ -202,6 +203,9 @@ class OraclePool(_PoolBase):
         out = np.zeros(count, np.uint32)
         olib().hdo_read_words(self.h, first, count, out)
         return out
+
+    def overflows(self):
+        return int(olib().hdo_overflows(self.h))
 
     def bucket_words(self):
         out = np.zeros(self.cfg.total_buckets, np.uint32)

@@ -21,6 +21,9 @@ def _capi():
 
 
 def assert_equivalent(dev, ora, cfg):
+    # a full bucket silently drops an edit in the reference (NodePool.hpp:138-139,195,210); the
+    # batched and sequential layouts differ, so equivalence is only defined without overflow
+    assert ora.overflows() == 0 and dev.stats().overflow == 0, (ora.overflows(), dev.stats().overflow)
     L = cfg.node_levels
     wd = dev.full_words()
     cd = O.canonical_words(wd, L, np.array([dev.root], np.uint32))
@@ -40,7 +43,7 @@ def assert_equivalent(dev, ora, cfg):
 @pytest.mark.parametrize("seed,res,n", [(1, 64, 40), (2, 64, 80), (3, 128, 60), (4, 32, 50)])
 def test_batched_random_edits_equal_sequential(seed, res, n):
     capi = _capi()
-    cfg = small_cfg(res, 8 if res > 64 else 6, 9 if res > 64 else 5, 2, 3)
+    cfg = small_cfg(res, 8, 9 if res > 64 else 6, 2, 3)
     e = random_edits(n, res, seed, True)
     dev, ora = capi.DevicePool(cfg), O.OraclePool(cfg)
     dev.edit(e, capi.MODE_BATCHED)
@@ -71,7 +74,7 @@ def test_batched_terrain_build_inside_batch():
     capi = _capi()
     res = 64
     h = W.value_noise_heightmap(res, 3, 4, 10, 50)
-    cfg = small_cfg(res, 6, 5, 2, 3)
+    cfg = small_cfg(res, 8, 6, 2, 3)
     dev, ora = capi.DevicePool(cfg, heights=h), O.OraclePool(cfg, heights=h)
     e = W.edits(list(random_edits(10, res, 7, True)) + [W.terrain_edit(res, seed=2, skin=3)]
                 + list(W.mixed_batch(30, res, h, 2, 12, seed=9)))

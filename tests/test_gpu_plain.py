@@ -24,6 +24,8 @@ def _capi():
 
 
 def assert_same_pool(dev, ora, where=""):
+    # both report the same bucket-full fallbacks (the reference's silent drop, NodePool.hpp:195,210)
+    assert (dev.stats().overflow & 1) == (1 if ora.overflows() else 0), where
     assert dev.root == ora.root, (where, hex(dev.root), hex(ora.root))
     assert dev.color_root == ora.color_root, (where, hex(dev.color_root), hex(ora.color_root))
     bd, wd = dev.live_words()
