@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--mode", choices=["plain", "batched"], default="plain")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-edits", type=int, default=4096)
+    ap.add_argument("--plain-prefix", type=int, default=512, help="edits of the batch also run plain, for the ratio")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed, not built",
@@ -125,9 +128,29 @@ def main():
         f"({primary} primary + {shadow} shadow, {iters / max(rays, 1):.1f} iterations/ray, "
         f"{words / max(rays, 1):.1f} words/ray)")
 
+    # GC of the edited pool (NodePoolThreadedGC::ThreadedGC)
+    used_before = pool.stats().used_words
+    pool.sync()
+    t0 = time.perf_counter()
+    pool.gc()
+    gc_ms = (time.perf_counter() - t0) * 1e3
+    used_after = pool.stats().used_words
+    log(f"gc: {gc_ms:.1f} ms, used words {used_before} -> {used_after}")
+    pool.close()
+
+    batched = None if args.no_batched else batched_section(args, cfg, heights, res)
+
     cpu = None
     if state0 is not None:
         cpu = cpu_baseline(cfg, heights, state0, batches[0], args.cpu_seconds, res)
+        if batched is not None:
+            b3 = cpu_baseline(cfg, heights, batched.pop("_state0"), batched.pop("_batch"), args.cpu_seconds, res)
+            batched["cpu_reference_voxels_per_s"] = b3["value"]
+            batched["cpu_reference_sample"] = b3["sample"]
+            batched["batched_vs_cpu"] = batched["batched_voxels_per_s"] / b3["value"]
+    if batched is not None:
+        batched.pop("_state0", None)
+        batched.pop("_batch", None)
 
     out = {
         "metric": METRIC,
@@ -168,10 +191,65 @@ def main():
             "algorithmic_bytes_per_launch": bytes_frame,
         },
         "cpu_baseline": cpu,
+        "gc_ms": gc_ms,
+        "gc_used_words": [int(used_before), int(used_after)],
+        "batched": batched,
         "pool": {"used_words": int(st.used_words), "color_nodes": int(st.color_nodes),
                  "color_leaf_words": int(st.color_leaf_words), "overflow": int(st.overflow)},
     }
     print(json.dumps(out))
+
+
+def batched_section(args, cfg, heights, res):
+    """Batched edits on the terrain: 4096 edits of uneven size (40% sphere add, 40% sphere remove,
+    20% box paint, radii / half-extents log-uniform in [2, 256], centres on the surface, seed 2)
+    applied in dependency-ordered passes; the plain path and the batched path also run the same
+    prefix from the same state for the ratio."""
+    from vkhashdag_amd import capi
+    from vkhashdag_amd import workloads as W
+
+    batch = W.mixed_batch(args.batch_edits, res, heights, 2, 256, seed=2)
+    vox = W.edited_voxels(batch, res)
+    prefix = batch[:args.plain_prefix]
+    vox_prefix = W.edited_voxels(prefix, res)
+    out = {"workload": f"{args.batch_edits} mixed edits (40% sphere add, 40% sphere remove, 20% box paint, "
+                       f"r log-uniform 2..256, seed 2) on the same terrain",
+           "edited_voxels": vox}
+
+    def fresh():
+        p = capi.DevicePool(cfg, heights=heights)
+        p.edit(W.terrain_edit(res, seed=1, skin=8))
+        p.sync()
+        return p
+
+    p = fresh()
+    out["_state0"] = None if args.no_cpu else p.state()
+    out["_batch"] = batch
+    t0 = time.perf_counter()
+    p.edit(batch, capi.MODE_BATCHED)
+    tb = time.perf_counter() - t0
+    st = p.stats()
+    out.update(batched_s=tb, batched_voxels_per_s=vox / tb, groups=int(st.last_groups),
+               overflow=int(st.overflow))
+    log(f"batched: {args.batch_edits} edits in {tb * 1e3:.1f} ms ({st.last_groups} passes), "
+        f"{vox / tb / 1e9:.2f} G voxels/s")
+    p.close()
+    # same prefix, plain and batched, from the same state
+    p = fresh()
+    t0 = time.perf_counter()
+    p.edit(prefix, capi.MODE_PLAIN)
+    tp = time.perf_counter() - t0
+    p.close()
+    p = fresh()
+    t0 = time.perf_counter()
+    p.edit(prefix, capi.MODE_BATCHED)
+    tbp = time.perf_counter() - t0
+    p.close()
+    out.update(prefix_edits=len(prefix), prefix_voxels=vox_prefix, plain_prefix_s=tp, batched_prefix_s=tbp,
+               plain_voxels_per_s=vox_prefix / tp, batched_prefix_voxels_per_s=vox_prefix / tbp,
+               batched_vs_plain=tp / tbp)
+    log(f"prefix {len(prefix)} edits: plain {tp * 1e3:.1f} ms, batched {tbp * 1e3:.1f} ms -> {tp / tbp:.1f}x")
+    return out
 
 
 def C_ref(v):

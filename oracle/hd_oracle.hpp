@@ -389,7 +389,7 @@ struct Editor {
 				t = EditType::kProceed;
 			if (uni && (t == EditType::kFill || !has_ptr || final_color == u))
 				final_color = u;
-			else if (!uni && !has_ptr)
+			else if (!uni && !has_ptr && t == EditType::kNotAffected) // untouched empty space: any color
 				final_color = VBRColor::rgb8(d.color.colors);
 			else
 				final_color = {};

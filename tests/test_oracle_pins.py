@@ -390,3 +390,39 @@ def test_edit_and_gc_golden_fixture():
     assert o.root == int(g["gc_root"])
     bw, lw = o.live_words()
     assert np.array_equal(bw, g["gc_bucket_words"]) and np.array_equal(lw, g["gc_live_words"])
+
+
+def _np_block_hash(seed, bx, by, bz):
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    k = np.uint64(seed) ^ (bx.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ (
+        by.astype(np.uint64) * np.uint64(0xC2B2AE3D27D4EB4F)) ^ (bz.astype(np.uint64) * np.uint64(0x165667B19E3779F9))
+    k ^= k >> np.uint64(33)
+    k = (k * np.uint64(0xff51afd7ed558ccd)) & M
+    k ^= k >> np.uint64(33)
+    k = (k * np.uint64(0xc4ceb9fe1a85ec53)) & M
+    k ^= k >> np.uint64(33)
+    return (k & np.uint64(0xFFFFFFFF)).astype(np.int64)
+
+
+def test_terrain_colors_follow_the_block_rule():
+    # Column fill on an empty pool: every occupied voxel decodes to the per-voxel rule --
+    # deep color below the skin, else the block's height ramp + [-8, 8] noise.
+    res, skin, seed, deep = 64, 4, 11, 0x405060
+    h = W.value_noise_heightmap(res, 3, 3, 8, 56)
+    cfg = small_cfg(res, 6, 5, 2, 3)
+    o = O.OraclePool(cfg, heights=h)
+    o.edit(W.terrain_edit(res, seed=seed, deep_color=deep, skin=skin))
+    g = o.voxels().astype(bool)
+    vz, vy, vx = np.nonzero(g)
+    bx, by, bz = vx >> 2, vy >> 2, vz >> 2
+    hmin_b = h.reshape(res // 4, 4, res // 4, 4).min(axis=(1, 3))[by, bx].astype(np.int64)
+    is_deep = bz * 4 + 4 + skin <= hmin_b
+    t = np.minimum((bz * 4 * 255) // (res - 1), 255)
+    base = np.stack([50 + t * 170 // 255, 110 + t * 120 // 255, 40 + t * 200 // 255], 1)
+    hh = _np_block_hash(seed, bx, by, bz)
+    noise = np.stack([((hh >> (8 * c)) & 0xFF) % 17 - 8 for c in range(3)], 1)
+    rgb = np.clip(base + noise, 0, 255)
+    rgb = np.where(is_deep[:, None], np.array([deep & 0xFF, (deep >> 8) & 0xFF, (deep >> 16) & 0xFF]), rgb)
+    got = o.colors(np.stack([vx, vy, vz], 1).astype(np.uint32))
+    assert np.array_equal(got, rgb.astype(np.float32) / np.float32(255.0))
+    assert len(np.unique(rgb, axis=0)) > 20

@@ -270,7 +270,7 @@ struct DevEditor {
 				t = kProceed;
 			if (uni && (t == kFill || !has_ptr || final_color == u))
 				final_color = u;
-			else if (!uni && !has_ptr)
+			else if (!uni && !has_ptr && t == kNotAffected) // untouched empty space: any color
 				final_color = Color::rgb8(d.color.colors);
 			else
 				final_color = Color::none();
