@@ -317,16 +317,17 @@ __host__ __device__ inline u32 morton3(u32 x, u32 y, u32 z) { return morton_part
 // find_node (NodePool.hpp:79-132): scan the bucket's used words node by node; inner-node size
 // comes from the mask word, a zero word is page-tail padding (skip to the next page); leaf
 // buckets hold 2-word leaves only.
-__device__ inline u32 find_in_bucket(const u32 *__restrict__ words, const DevConfig &c, u32 bucket, u32 bw,
-                                     const u32 *node, u32 n, bool leaf) {
+// [from, bw) restricts the scan to the words appended since bucket word `from` (a node boundary).
+__device__ inline u32 find_in_bucket_from(const u32 *__restrict__ words, const DevConfig &c, u32 bucket, u32 from,
+                                          u32 bw, const u32 *node, u32 n, bool leaf) {
 	const u64 base = (u64)bucket << c.wbpb();
 	if (leaf) {
-		for (u32 i = 0; i + 2 <= bw; i += 2)
+		for (u32 i = from; i + 2 <= bw; i += 2)
 			if (words[base + i] == node[0] && words[base + i + 1] == node[1])
 				return (u32)(base + i);
 		return kNull;
 	}
-	u32 i = 0;
+	u32 i = from;
 	while (i < bw) {
 		u32 w0 = words[base + i];
 		u32 sz = inner_words(w0);
@@ -344,6 +345,10 @@ __device__ inline u32 find_in_bucket(const u32 *__restrict__ words, const DevCon
 		i += sz;
 	}
 	return kNull;
+}
+__device__ inline u32 find_in_bucket(const u32 *__restrict__ words, const DevConfig &c, u32 bucket, u32 bw,
+                                     const u32 *node, u32 n, bool leaf) {
+	return find_in_bucket_from(words, c, bucket, 0u, bw, node, n, leaf);
 }
 // append_node (NodePool.hpp:134-157): full -> kNull; a node that would straddle a page zero-fills
 // the page tail and starts the next page. Updates bw.

@@ -21,6 +21,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 
 namespace hd {
 
@@ -762,6 +764,868 @@ __global__ void k_add_meta(u32 *meta, u32 which, const u32 *a, const u32 *b, u32
 	meta[which] += a[0] + (b ? b[ib] : 0u);
 }
 
+// ================================================================ sync-free pass
+// The same pass with no host round trip: level sizes stay in device memory and grids are fixed
+// by host-side upper bounds. Top-down compaction is stable (block totals, then block prefix + LDS
+// scan), so entry indices keep DFS order. Bottom-up, the build kernel finds each new node among
+// the bucket's existing words and queues the misses on a per-bucket list; one walker per bucket
+// then replays the misses in entry (DFS) order with an exact find-or-append over the words this
+// level appended -- the reference's sequential upsert, with no cross-workgroup visibility needed
+// inside a launch. Small levels run in single-workgroup kernels; roots stay in device memory so
+// consecutive passes queue back to back.
+
+constexpr u32 kAT = 256;       // threads per block of the sync-free kernels
+constexpr u32 kGather = 32;    // bucket lists up to this length are sorted in registers
+
+struct Pair {
+	u32 a, b;
+};
+__device__ inline Pair block_excl_scan2(Pair x, Pair *sh, Pair &total) {
+	u32 t = threadIdx.x;
+	sh[t] = x;
+	__syncthreads();
+	for (u32 o = 1; o < kAT; o <<= 1) {
+		Pair y = t >= o ? sh[t - o] : Pair{0, 0};
+		__syncthreads();
+		sh[t].a += y.a;
+		sh[t].b += y.b;
+		__syncthreads();
+	}
+	Pair incl = sh[t];
+	total = sh[kAT - 1];
+	__syncthreads();
+	return Pair{incl.a - x.a, incl.b - x.b};
+}
+
+// per-pass device counters: pm[kPmHdr + l] entries of level l, pm[kPmHdr + L + l] list words
+enum PassMeta : u32 { kPmOverflow = 0, kPmEvents = 1, kPmLeafTotal = 2, kPmHdr = 4 };
+// counters accumulated over the passes of one hd_edit call
+enum AccMeta : u32 { kAccEntries = 0, kAccUpserts = 1, kAccJobs = 2, kAccOverflow = 3, kAccCount = 4 };
+
+// Per-level buffers of the sync-free pass (a device-resident table, indexed by level).
+struct ALevel {
+	Entry *ent;
+	u32 *lists;               // edit lists of multi-edit entries
+	u32 *kids;                // 8 per entry: new pointer of each terminal child
+	u32 *okids;               // 8 per entry: color state of each child (levels above the color leaves)
+	u32 *first_child, *pmask; // index of the first proceeding child at the next level + their slots
+	u32 *status, *pend, *res, *failed, *next, *lpos, *bucket;
+	u32 *content, *csize;
+	u32 *ores, *ccnt, *cbase, *own; // color results, SetNode passes
+	u32 cap, list_cap;
+};
+
+struct PairOut {
+	u32 flag, lcnt;
+	uint4 cs;
+};
+
+// classification of one (node, child) pair: the body of k_expand
+__device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &A, u64 t, PushEvent *__restrict__ ev,
+                                      u32 *__restrict__ pm, u32 ev_cap) {
+	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
+	Entry E = A.ent[e];
+	u32 old = kNull;
+	if (E.ptr != kNull) {
+		u32 m = P.words[E.ptr];
+		if ((m >> i) & 1u)
+			old = P.words[E.ptr + 1u + (u32)__builtin_popcount(m & ((1u << i) - 1u))];
+	}
+	const u32 cl = l + 1, leafL = P.c.color_leaf;
+	const u32 cx = (E.x << 1) | (i & 1u), cy = (E.y << 1) | ((i >> 1) & 1u), cz = (E.z << 1) | ((i >> 2) & 1u);
+	u32 octree = E.octree;
+	if (P.stateful && cl <= leafL)
+		octree = color_child(P.cnodes, E.octree, i);
+	const u32 sf_in = i | (E.sf & (kSfFinal | kSfWriter));
+	Classified c{kNotAffected, octree, sf_in, false, Color::none()};
+	u32 cnt = 1, single = E.loff;
+	if (E.llen == 1)
+		c = classify_single(P, P.editor(E.loff), cl, cx, cy, cz, old, octree, sf_in);
+	else {
+		cnt = 0;
+		bool active = false;
+		for (u32 k = 0; k < E.llen; ++k) {
+			u32 idx = A.lists[E.loff + k];
+			DevEditor ed = P.editor(idx);
+			if (ed.geom(cl, cx, cy, cz) != kNotAffected) {
+				++cnt;
+				single = idx;
+				active |= ed.stateful();
+			}
+		}
+		if (cnt == 1)
+			c = classify_single(P, P.editor(single), cl, cx, cy, cz, old, octree, sf_in);
+		else if (cnt > 1) {
+			c.t = kProceed;
+			if (active)
+				c.sf |= kSfActive | (cl == leafL ? (kSfWriter | kSfNewJob) : 0u);
+		}
+	}
+	if (c.push) {
+		u32 d = cl - leafL, msk = (1u << d) - 1u, sh = 3u * (P.c.voxel_level - cl);
+		u32 k = atomicAdd(&pm[kPmEvents], 1u);
+		if (k < ev_cap)
+			ev[k] = PushEvent{E.job, morton3(cx & msk, cy & msk, cz & msk) << sh, 1u << sh, c.col.colors, c.col.wb, 0, 0, 0};
+		else
+			atomicOr(&pm[kPmOverflow], 1u);
+	}
+	A.kids[t] = c.t == kClear ? kNull : (c.t == kFill ? P.filled[cl] : old);
+	if (A.okids)
+		A.okids[t] = c.octree;
+	bool proceed = c.t == kProceed;
+	return PairOut{proceed ? 1u : 0u, proceed && cnt > 1 ? cnt : 0u, make_uint4(c.octree, c.sf, single, cnt)};
+}
+
+// Writes the child entry of a proceeding pair (k_emit). m8: proceed flags of the pair's node
+// (valid on slot 0), idx / lo: the child's entry index and list offset at level l + 1.
+__device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, const ALevel &N, u32 *__restrict__ pm,
+                                 u64 t, u32 m8, const PairOut &o, u32 idx, u32 lo) {
+	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
+	if (i == 0) {
+		A.first_child[e] = idx;
+		A.pmask[e] = m8;
+	}
+	if (!o.flag)
+		return;
+	Entry E = A.ent[e];
+	u32 sf = o.cs.y;
+	u32 job = (sf & kSfNewJob) ? idx : ((sf & kSfWriter) ? E.job : kNull);
+	const u32 cx = (E.x << 1) | (i & 1u), cy = (E.y << 1) | ((i >> 1) & 1u), cz = (E.z << 1) | ((i >> 2) & 1u);
+	u32 ln = o.cs.w;
+	if (ln > 1) { // the child's filtered list, in the parent's (submission) order
+		if ((u64)lo + ln > N.list_cap) {
+			atomicOr(&pm[kPmOverflow], 1u);
+			ln = 0;
+		} else {
+			u32 w = lo;
+			for (u32 k = 0; k < E.llen; ++k) {
+				u32 ix = A.lists[E.loff + k];
+				if (P.editor(ix).geom(l + 1, cx, cy, cz) != kNotAffected)
+					N.lists[w++] = ix;
+			}
+		}
+	} else
+		lo = o.cs.z;
+	if (idx < N.cap)
+		N.ent[idx] = Entry{cx, cy, cz, A.kids[t], e, sf & ~kSfNewJob, o.cs.x, job, lo, ln};
+	else
+		atomicOr(&pm[kPmOverflow], 1u);
+}
+
+// chunk of the pair range [0, 8n) owned by this block (whole nodes per block)
+__device__ inline void pair_chunk(u32 n, u64 &begin, u64 &end) {
+	u64 P8 = (u64)n * 8u, chunk = (P8 + gridDim.x - 1) / gridDim.x;
+	chunk = (chunk + 7u) & ~7ull;
+	begin = (u64)blockIdx.x * chunk;
+	if (begin > P8)
+		begin = P8;
+	end = begin + chunk < P8 ? begin + chunk : P8;
+}
+
+// Expands the pairs [begin, end) tile by tile and emits the proceeding children, starting at
+// child index base (a block-wide helper; tiles start on node boundaries).
+__device__ inline Pair expand_emit_range(const EditParams &P, u32 l, const ALevel &A, const ALevel &N,
+                                         u32 *__restrict__ pm, PushEvent *ev, u32 ev_cap, u64 begin, u64 end, Pair base,
+                                         const u32 *__restrict__ flag, const u32 *__restrict__ lcnt,
+                                         const uint4 *__restrict__ cstate, Pair *sh, u32 *sflag) {
+	for (u64 t0 = begin; t0 < end; t0 += kAT) {
+		u64 t = t0 + threadIdx.x;
+		bool in = t < end;
+		PairOut o{0, 0, make_uint4(0, 0, 0, 0)};
+		if (in) {
+			if (flag)
+				o = PairOut{flag[t], lcnt[t], cstate[t]};
+			else
+				o = expand_pair(P, l, A, t, ev, pm, ev_cap);
+		}
+		sflag[threadIdx.x] = o.flag;
+		Pair tile;
+		Pair off = block_excl_scan2(Pair{o.flag, o.lcnt}, sh, tile); // synchronizes sflag
+		if (in) {
+			u32 m8 = 0;
+			if ((t & 7u) == 0)
+				for (u32 k = 0; k < 8; ++k)
+					m8 |= sflag[threadIdx.x + k] << k;
+			emit_pair(P, l, A, N, pm, t, m8, o, base.a + off.a, base.b + off.b);
+		}
+		base.a += tile.a;
+		base.b += tile.b;
+		__syncthreads(); // sflag reuse
+	}
+	return base;
+}
+
+__device__ inline void set_level_count(const EditParams &P, u32 l, const ALevel &N, u32 *pm, Pair total) {
+	const u32 L = P.c.L;
+	pm[kPmHdr + l + 1] = min(total.a, N.cap);
+	pm[kPmHdr + L + l + 1] = total.b;
+	if (total.a > N.cap)
+		pm[kPmOverflow] |= 1u;
+}
+
+__device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__restrict__ roots, u32 *__restrict__ pm) {
+	const u32 root = roots[0], croot = roots[1], L = P.c.L;
+	u32 cnt = 0, single = 0;
+	bool active = false;
+	if (P.n_edits == 1)
+		cnt = 1;
+	else
+		for (u32 k = 0; k < P.n_edits; ++k) {
+			DevEditor e = P.editor(k);
+			if (e.geom(0, 0, 0, 0) != kNotAffected) {
+				if (cnt < A0.list_cap)
+					A0.lists[cnt] = k;
+				++cnt;
+				single = k;
+				active |= e.stateful();
+			}
+		}
+	Classified c{kNotAffected, croot, 0u, false, Color::none()};
+	if (cnt == 1)
+		c = classify_single(P, P.editor(single), 0, 0, 0, 0, root, croot, 0u);
+	else if (cnt > 1) {
+		c.t = kProceed;
+		if (active)
+			c.sf |= kSfActive | (P.c.color_leaf == 0 ? (kSfWriter | kSfNewJob) : 0u);
+		if (cnt > A0.list_cap)
+			pm[kPmOverflow] |= 1u;
+	}
+	if (c.t == kProceed) {
+		A0.ent[0] = Entry{0, 0, 0, root, kNull, c.sf & ~kSfNewJob, c.octree, (c.sf & kSfWriter) ? 0u : kNull,
+		                  cnt == 1 ? single : 0u, cnt};
+		pm[kPmHdr] = 1;
+		pm[kPmHdr + L] = cnt > 1 ? cnt : 0u;
+	} else {
+		roots[0] = c.t == kClear ? kNull : (c.t == kFill ? P.filled[0] : root);
+		if (P.stateful)
+			roots[1] = c.octree;
+	}
+}
+
+// Single workgroup: clears the pass counters, classifies the root and expands levels [0, l_end)
+// (every one small enough for one block).
+__global__ void __launch_bounds__(kAT) a_top(EditParams P, const ALevel *__restrict__ lv, u32 *__restrict__ roots,
+                                             u32 *__restrict__ pm, u32 pm_words, u32 l_end, PushEvent *ev, u32 ev_cap) {
+	__shared__ Pair sh[kAT];
+	__shared__ u32 sflag[kAT + 8];
+	for (u32 i = threadIdx.x; i < pm_words; i += kAT)
+		pm[i] = 0;
+	if (threadIdx.x < 8)
+		sflag[kAT + threadIdx.x] = 0;
+	__syncthreads();
+	if (threadIdx.x == 0)
+		root_entry(P, lv[0], roots, pm);
+	__syncthreads();
+	for (u32 l = 0; l < l_end; ++l) {
+		const u32 n = pm[kPmHdr + l];
+		Pair tot = expand_emit_range(P, l, lv[l], lv[l + 1], pm, ev, ev_cap, 0, (u64)n * 8u, Pair{0, 0}, nullptr,
+		                             nullptr, nullptr, sh, sflag);
+		if (threadIdx.x == 0)
+			set_level_count(P, l, lv[l + 1], pm, tot);
+		__syncthreads();
+	}
+}
+
+// Large levels: a_count (classify, block totals) then a_emit (block prefix, stable emit).
+__global__ void __launch_bounds__(kAT) a_count(EditParams P, u32 l, const ALevel *__restrict__ lv, u32 *__restrict__ pm,
+                                               u32 *__restrict__ flag, u32 *__restrict__ lcnt,
+                                               uint4 *__restrict__ cstate, PushEvent *ev, u32 ev_cap,
+                                               Pair *__restrict__ bsum) {
+	__shared__ Pair sh[kAT];
+	const ALevel A = lv[l];
+	u64 begin, end;
+	pair_chunk(pm[kPmHdr + l], begin, end);
+	Pair acc{0, 0};
+	for (u64 t = begin + threadIdx.x; t < end; t += kAT) {
+		PairOut o = expand_pair(P, l, A, t, ev, pm, ev_cap);
+		flag[t] = o.flag;
+		lcnt[t] = o.lcnt;
+		cstate[t] = o.cs;
+		acc.a += o.flag;
+		acc.b += o.lcnt;
+	}
+	Pair tot;
+	block_excl_scan2(acc, sh, tot);
+	if (threadIdx.x == 0)
+		bsum[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(kAT) a_emit(EditParams P, u32 l, const ALevel *__restrict__ lv, u32 *__restrict__ pm,
+                                              const u32 *__restrict__ flag, const u32 *__restrict__ lcnt,
+                                              const uint4 *__restrict__ cstate, const Pair *__restrict__ bsum) {
+	__shared__ Pair sh[kAT];
+	__shared__ u32 sflag[kAT + 8];
+	if (threadIdx.x < 8)
+		sflag[kAT + threadIdx.x] = 0;
+	u64 begin, end;
+	pair_chunk(pm[kPmHdr + l], begin, end);
+	Pair pre{0, 0};
+	for (u32 b = threadIdx.x; b < blockIdx.x; b += kAT)
+		pre.a += bsum[b].a, pre.b += bsum[b].b;
+	Pair base;
+	block_excl_scan2(pre, sh, base);
+	Pair tot = expand_emit_range(P, l, lv[l], lv[l + 1], pm, nullptr, 0, begin, end, base, flag, lcnt, cstate, sh,
+	                             sflag);
+	if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+		set_level_count(P, l, lv[l + 1], pm, tot);
+}
+
+// ---- colors of a sync-free pass: dense decode, pushes
+__global__ void a_dense_init(const ALevel *__restrict__ lv, u32 leafL, const u32 *__restrict__ pm, u32 log2v,
+                             const u32 *__restrict__ cleaves, Color *__restrict__ dense) {
+	const Entry *ent = lv[leafL].ent;
+	u64 total = (u64)pm[kPmHdr + leafL] << log2v;
+	const u32 vm = (1u << log2v) - 1u;
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (u64)gridDim.x * blockDim.x) {
+		u32 e = (u32)(i >> log2v), v = (u32)i & vm;
+		Entry E = ent[e];
+		if (!(E.sf & kSfWriter))
+			continue;
+		dense[i] = ctag(E.octree) == kTagLeaf ? chunk_voxel(cleaves, cdata(E.octree), v) : get_fill(E.octree);
+	}
+}
+// Pushes cover terminal subtrees and leaf edits cover proceeding leaves: disjoint voxel ranges,
+// so this kernel is ordered only after a_dense_init.
+__global__ void a_push(const PushEvent *__restrict__ ev, const u32 *__restrict__ pm, u32 ev_cap, u32 V, u32 jobs,
+                       Color *__restrict__ dense) {
+	u32 nev = min(pm[kPmEvents], ev_cap);
+	for (u32 k = blockIdx.x; k < nev; k += gridDim.x) {
+		PushEvent p = ev[k];
+		if (p.job >= jobs) // a job dropped by a level overflow (reported)
+			continue;
+		Color c{p.colors, p.wb};
+		Color *out = dense + (u64)p.job * V + p.offset;
+		for (u32 v = threadIdx.x; v < p.count; v += blockDim.x)
+			out[v] = c;
+	}
+}
+
+// ---- leaves + bottom-up DAG
+__device__ inline u32 wave_find_leaf(const u32 *__restrict__ W, const DevConfig &c, u32 bucket, u32 bw, u32 n0, u32 n1,
+                                     u32 lane) {
+	const u64 base = (u64)bucket << c.wbpb();
+	for (u32 i0 = 0; i0 < bw; i0 += 128u) {
+		u32 j = i0 + 2u * lane;
+		bool m = j + 2u <= bw && W[base + j] == n0 && W[base + j + 1u] == n1;
+		unsigned long long b = __ballot(m);
+		if (b)
+			return (u32)(base + i0 + 2u * (u32)__builtin_ctzll(b));
+	}
+	return kNull;
+}
+
+// queues a node that is not in its bucket yet: the first arrival (lpos 0) walks the list later
+__device__ inline void queue_miss(const ALevel &A, u32 e, u32 b, u32 *__restrict__ bcnt, u32 *__restrict__ head) {
+	A.lpos[e] = atomicAdd(&bcnt[b], 1u);
+	A.next[e] = atomicExch(&head[b], e);
+}
+
+// edit_leaf (k_leaf) + the find among the bucket's existing leaves: one wavefront per leaf.
+__global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
+                                              Color *__restrict__ dense, u32 V, u32 *__restrict__ bcnt,
+                                              u32 *__restrict__ head) {
+	const u32 lane = threadIdx.x & 63u, level = P.c.L - 1;
+	const ALevel A = lv[level];
+	const u32 n = pm[kPmHdr + level];
+	const u32 waves = gridDim.x * (kAT / 64u);
+	for (u32 e = (u32)(((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6); e < n; e += waves) {
+		Entry E = A.ent[e];
+		u32 w0 = 0, w1 = 0;
+		if (E.ptr != kNull) {
+			w0 = P.words[E.ptr];
+			w1 = P.words[E.ptr + 1u];
+		}
+		bool v = ((lane < 32u ? (w0 >> lane) : (w1 >> (lane - 32u))) & 1u) != 0;
+		u32 vx = (E.x << 2) | ((lane >> 2) & 2u) | (lane & 1u);
+		u32 vy = (E.y << 2) | ((lane >> 3) & 2u) | ((lane >> 1) & 1u);
+		u32 vz = (E.z << 2) | ((lane >> 4) & 2u) | ((lane >> 2) & 1u);
+		const bool writer = (E.sf & kSfWriter) && !(E.sf & kSfFinal) && E.job < lv[P.c.color_leaf].cap;
+		Color *cp = nullptr;
+		Color col = Color::none();
+		if (writer) {
+			u32 msk = (1u << P.leaf_bits) - 1u;
+			cp = dense + (u64)E.job * V + morton3(vx & msk, vy & msk, vz & msk);
+			col = *cp;
+		}
+		bool nv = v;
+		for (u32 k = 0; k < E.llen; ++k) {
+			DevEditor ed = P.editor(E.llen == 1 ? E.loff : A.lists[E.loff + k]);
+			if (!ed.stateful())
+				nv = ed.edit_voxel(vx, vy, vz, nv);
+			else if (writer)
+				nv = ed.edit_voxel_vbr(vx, vy, vz, nv, col);
+			else {
+				Color dummy = Color::none();
+				nv = ed.edit_voxel_vbr(vx, vy, vz, nv, dummy);
+			}
+		}
+		if (writer && (E.sf & kSfActive))
+			*cp = col;
+		unsigned long long m = __ballot(nv);
+		u32 n0 = (u32)m, n1 = (u32)(m >> 32);
+		bool pending = (n0 != w0 || n1 != w1) && m != 0ull; // wavefront-uniform
+		if (!pending) {
+			if (lane == 0) {
+				A.pend[e] = 0;
+				A.status[e] = (n0 != w0 || n1 != w1) ? kNull : E.ptr;
+			}
+			continue;
+		}
+		u32 h = murmur_leaf(n0, n1);
+		u32 b = P.c.level_base[level] + (h & ((1u << P.c.bucket_bits[level]) - 1u));
+		u32 f = wave_find_leaf(P.words, P.c, b, P.bucket_words[b], n0, n1, lane);
+		if (lane == 0) {
+			A.pend[e] = 1;
+			A.content[2 * (u64)e] = n0;
+			A.content[2 * (u64)e + 1] = n1;
+			A.bucket[e] = b;
+			A.res[e] = f;
+			A.failed[e] = 0;
+			if (f == kNull)
+				queue_miss(A, e, b, bcnt, head);
+			else
+				A.lpos[e] = kNull;
+		}
+	}
+}
+
+// Result of a child entry once its level's upserts are done (k_resolve's rule).
+__device__ inline u32 child_result(const ALevel &C, u32 c) {
+	if (c >= C.cap)
+		return kNull; // unreachable with sound bounds (the pass reports an overflow)
+	if (!C.pend[c])
+		return C.status[c];
+	return C.failed[c] ? C.ent[c].ptr : C.res[c];
+}
+
+// edit_node's join (k_build) for entry e of level l, with the children's results resolved from
+// level l + 1, then the find among the bucket's existing nodes.
+__device__ inline void build_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, u32 e,
+                                   u32 *__restrict__ bcnt, u32 *__restrict__ head) {
+	Entry E = A.ent[e];
+	u32 om = 0, k = 1;
+	if (E.ptr != kNull)
+		om = P.words[E.ptr];
+	const u32 pmk = A.pmask[e], fc = A.first_child[e];
+	bool changed = false;
+	u32 nm = 0, nk[8];
+	for (u32 i = 0; i < 8; ++i) {
+		u32 o = ((om >> i) & 1u) ? P.words[E.ptr + k++] : kNull;
+		u32 r = ((pmk >> i) & 1u) ? child_result(C, fc + (u32)__builtin_popcount(pmk & ((1u << i) - 1u)))
+		                          : A.kids[(u64)e * 8u + i];
+		nk[i] = r;
+		changed |= r != o;
+		if (r != kNull)
+			nm |= 1u << i;
+	}
+	if (!changed || nm == 0) {
+		A.pend[e] = 0;
+		A.status[e] = changed ? kNull : E.ptr;
+		return;
+	}
+	u32 c[9];
+	c[0] = nm;
+	k = 1;
+	for (u32 i = 0; i < 8; ++i)
+		if (nk[i] != kNull)
+			c[k++] = nk[i];
+	for (u32 j = 0; j < k; ++j)
+		A.content[(u64)e * 9u + j] = c[j];
+	u32 h = murmur_span(c, k);
+	u32 b = P.c.level_base[l] + (h & ((1u << P.c.bucket_bits[l]) - 1u));
+	u32 f = find_in_bucket(P.words, P.c, b, P.bucket_words[b], c, k, false);
+	A.pend[e] = 1;
+	A.csize[e] = k;
+	A.bucket[e] = b;
+	A.res[e] = f;
+	A.failed[e] = 0;
+	if (f == kNull)
+		queue_miss(A, e, b, bcnt, head);
+	else
+		A.lpos[e] = kNull;
+}
+
+// The walker of a bucket (the entry that arrived first) replays the bucket's misses in entry
+// order: find among the words appended by this walk, else append_node (NodePool.hpp:187-215).
+__device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, u32 e, u32 *__restrict__ bcnt,
+                                  u32 *__restrict__ head, u32 *__restrict__ acc) {
+	if (!A.pend[e] || A.lpos[e] != 0)
+		return;
+	const bool leaf = l == P.c.L - 1;
+	const u32 stride = leaf ? 2u : 9u;
+	const u32 b = A.bucket[e], k = bcnt[b];
+	const u32 bw0 = P.bucket_words[b];
+	u32 bw = bw0, appended = 0;
+	bool any_fail = false;
+	auto upsert = [&](u32 j) {
+		const u32 *node = A.content + (u64)j * stride;
+		u32 nsz = leaf ? 2u : A.csize[j];
+		u32 f = find_in_bucket_from(P.words, P.c, b, bw0, bw, node, nsz, leaf);
+		u32 fail = 0;
+		if (f == kNull) {
+			f = append_to_bucket(P.words, P.c, b, bw, node, nsz);
+			if (f == kNull)
+				fail = 1, any_fail = true;
+			else
+				++appended;
+		}
+		A.res[j] = f;
+		A.failed[j] = fail;
+	};
+	if (k <= kGather) {
+		u32 ord[kGather];
+		u32 g = 0;
+		for (u32 j = head[b]; j != kNull && g < kGather; j = A.next[j])
+			ord[g++] = j;
+		for (u32 x = 1; x < g; ++x) { // insertion sort: entry (DFS) order
+			u32 v = ord[x], y = x;
+			while (y > 0 && ord[y - 1] > v) {
+				ord[y] = ord[y - 1];
+				--y;
+			}
+			ord[y] = v;
+		}
+		for (u32 x = 0; x < g; ++x)
+			upsert(ord[x]);
+	} else {
+		u32 last = 0;
+		for (u32 r = 0; r < k; ++r) {
+			u32 best = kNull;
+			for (u32 j = head[b]; j != kNull; j = A.next[j])
+				if ((r == 0 || j > last) && j < best)
+					best = j;
+			if (best == kNull)
+				break;
+			upsert(best);
+			last = best;
+		}
+	}
+	P.bucket_words[b] = bw;
+	bcnt[b] = 0;
+	head[b] = kNull;
+	if (appended)
+		atomicAdd(&acc[kAccUpserts], appended);
+	if (any_fail)
+		atomicOr(&P.meta[kMetaOverflow], 1u);
+}
+
+__global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel *__restrict__ lv,
+                                               const u32 *__restrict__ pm, u32 *__restrict__ bcnt,
+                                               u32 *__restrict__ head) {
+	const ALevel A = lv[l], C = lv[l + 1];
+	const u32 n = pm[kPmHdr + l];
+	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+		build_entry(P, l, A, C, e, bcnt, head);
+}
+__global__ void __launch_bounds__(kAT) a_walk(EditParams P, u32 l, const ALevel *__restrict__ lv,
+                                              const u32 *__restrict__ pm, u32 *__restrict__ bcnt,
+                                              u32 *__restrict__ head, u32 *__restrict__ acc) {
+	const ALevel A = lv[l];
+	const u32 n = pm[kPmHdr + l];
+	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+		walk_entry(P, l, A, e, bcnt, head, acc);
+}
+
+// ---- colors: encode + SetLeaf (one block per color leaf job, grid-stride)
+__global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const ALevel *__restrict__ lv,
+                                                              const u32 *__restrict__ pm, const Color *__restrict__ dense,
+                                                              u32 V, u32 *__restrict__ info, u32 *__restrict__ app) {
+	__shared__ EncScan sh[kEncThreads];
+	const u32 leafL = P.c.color_leaf;
+	const Entry *ent = lv[leafL].ent;
+	const u32 n = pm[kPmHdr + leafL];
+	for (u32 e = blockIdx.x; e < n; e += gridDim.x) {
+		Entry E = ent[e];
+		if (!(E.sf & kSfWriter)) {
+			if (threadIdx.x == 0)
+				app[e] = 0;
+			continue;
+		}
+		const Color *d = dense + (u64)e * V;
+		u32 per = (V + kEncThreads - 1) / kEncThreads, v0 = threadIdx.x * per, v1 = min(V, v0 + per);
+		EncScan x{0, 0};
+		for (u32 v = v0; v < v1; ++v) {
+			x.heads += is_head(d, v) ? 1u : 0u;
+			x.bits += (d[v].wb >> 8) & 0xFFu;
+		}
+		EncScan tot;
+		block_excl_scan(x, sh, tot);
+		if (threadIdx.x == 0) {
+			u32 M = (V + 0x3FFFu) >> 14, W = (tot.bits + 31u) >> 5;
+			u32 ds = 2u * M + 2u * tot.heads + W + 4u;
+			u32 as = (ds & 1u) ? ds + 1u : ds;
+			u32 inplace = 0;
+			if (!P.c.keep_history && ctag(E.octree) == kTagLeaf) { // DAGColorPool::SetLeaf (:172-204)
+				u32 block = P.cleaves[cdata(E.octree)];
+				if (ds <= block)
+					inplace = 1, as = 0;
+				else
+					as = max(block << 1, as);
+			}
+			info[4 * e] = ds;
+			info[4 * e + 1] = as;
+			info[4 * e + 2] = inplace;
+			info[4 * e + 3] = tot.heads;
+			app[e] = as;
+		}
+	}
+}
+// single block: exclusive prefix of the append sizes in entry (DFS) order
+__global__ void __launch_bounds__(kAT) a_scan_app(u32 leafL, u32 *__restrict__ pm, const u32 *__restrict__ app,
+                                                  u32 *__restrict__ appo) {
+	__shared__ Pair sh[kAT];
+	const u32 n = pm[kPmHdr + leafL];
+	u32 run = 0;
+	for (u32 t0 = 0; t0 < n; t0 += kAT) {
+		u32 t = t0 + threadIdx.x;
+		Pair tile;
+		Pair off = block_excl_scan2(Pair{t < n ? app[t] : 0u, 0}, sh, tile);
+		if (t < n)
+			appo[t] = run + off.a;
+		run += tile.a;
+	}
+	if (threadIdx.x == 0)
+		pm[kPmLeafTotal] = run;
+}
+__global__ void __launch_bounds__(kEncThreads) a_encode_write(EditParams P, const ALevel *__restrict__ lv,
+                                                               const u32 *__restrict__ pm, const Color *__restrict__ dense,
+                                                               u32 V, const u32 *__restrict__ info,
+                                                               const u32 *__restrict__ appo, u64 leaf_cap) {
+	__shared__ EncScan sh[kEncThreads];
+	__shared__ u32 macro_ws[160];
+	const u32 leafL = P.c.color_leaf;
+	const Entry *ent = lv[leafL].ent;
+	u32 *ores = lv[leafL].ores;
+	const u32 n = pm[kPmHdr + leafL];
+	const u32 leaf_base = P.meta[kMetaColorLeaves];
+	for (u32 e = blockIdx.x; e < n; e += gridDim.x) {
+		Entry E = ent[e];
+		u32 result = E.octree;
+		if (E.sf & kSfWriter) {
+			u32 ds = info[4 * e], as = info[4 * e + 1], inplace = info[4 * e + 2], B = info[4 * e + 3];
+			u32 idx = inplace ? cdata(E.octree) : leaf_base + appo[e];
+			if (inplace || (u64)idx + as <= leaf_cap) {
+				u32 *L = P.cleaves;
+				const Color *d = dense + (u64)e * V;
+				u32 M = (V + 0x3FFFu) >> 14;
+				u32 per = (V + kEncThreads - 1) / kEncThreads, v0 = threadIdx.x * per, v1 = min(V, v0 + per);
+				EncScan x{0, 0};
+				for (u32 v = v0; v < v1; ++v) {
+					x.heads += is_head(d, v) ? 1u : 0u;
+					x.bits += (d[v].wb >> 8) & 0xFFu;
+				}
+				EncScan tot;
+				EncScan pre = block_excl_scan(x, sh, tot);
+				u32 W = (tot.bits + 31u) >> 5;
+				u32 mo = idx + 4u, bo = mo + 2u * M, wo = bo + 2u * B;
+				EncScan run = pre;
+				for (u32 v = v0; v < v1; ++v) {
+					bool h = is_head(d, v);
+					if ((v & 0x3FFFu) == 0) {
+						macro_ws[v >> 14] = run.bits;
+						L[mo + 2u * (v >> 14)] = run.heads;
+						L[mo + 2u * (v >> 14) + 1u] = run.bits;
+					}
+					run.heads += h ? 1u : 0u;
+					run.bits += (d[v].wb >> 8) & 0xFFu;
+				}
+				for (u32 w = threadIdx.x; w < W; w += kEncThreads)
+					L[wo + w] = 0u;
+				__syncthreads();
+				run = pre;
+				for (u32 v = v0; v < v1; ++v) {
+					Color c = d[v];
+					u32 bpw = (c.wb >> 8) & 0xFFu;
+					if (is_head(d, v)) {
+						L[bo + 2u * run.heads] = c.colors;
+						L[bo + 2u * run.heads + 1u] = ((v & 0x3FFFu) << 18) | (bpw << 16) | (run.bits - macro_ws[v >> 14]);
+						run.heads += 1u;
+					}
+					if (bpw) {
+						u32 w = c.wb & ((1u << bpw) - 1u), o = run.bits & 31u;
+						atomicOr(&L[wo + (run.bits >> 5)], w << o);
+						if (o + bpw > 32u)
+							atomicOr(&L[wo + (run.bits >> 5) + 1u], w >> (32u - o));
+					}
+					run.bits += bpw;
+				}
+				if (threadIdx.x == 0) {
+					L[idx + 1u] = M;
+					L[idx + 2u] = B;
+					L[idx + 3u] = W;
+					if (!inplace)
+						L[idx] = as;
+				}
+				if (!inplace)
+					for (u32 w = ds + threadIdx.x; w < as; w += kEncThreads)
+						L[idx + w] = 0u;
+				result = inplace ? E.octree : cmake(kTagLeaf, idx);
+			} else if (threadIdx.x == 0)
+				atomicOr(&P.meta[kMetaOverflow], 2u);
+		}
+		if (threadIdx.x == 0)
+			ores[e] = result;
+		__syncthreads(); // macro_ws / sh reuse
+	}
+}
+
+// SetNode (k_setnode_a/b/c) for every level above the color leaves, in one workgroup: pass A
+// bottom-up decisions + subtree append counts, pass B top-down post-order indices, pass C
+// bottom-up writes; then the pool counters and the color root.
+__device__ inline u32 color_child_result(const ALevel &A, const ALevel &C, u32 e, u32 i) {
+	u32 m = A.pmask[e];
+	if ((m >> i) & 1u) {
+		u32 c = A.first_child[e] + (u32)__builtin_popcount(m & ((1u << i) - 1u));
+		return c < C.cap ? C.ores[c] : kColorNull;
+	}
+	return A.okids[(u64)e * 8u + i];
+}
+__device__ inline void setnode_block(const EditParams &P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
+                                     u32 *__restrict__ roots, u64 node_cap) {
+	const u32 leafL = P.c.color_leaf;
+	for (int l = (int)leafL - 1; l >= 0; --l) { // pass A
+		const ALevel A = lv[l], C = lv[l + 1];
+		const u32 n = pm[kPmHdr + l];
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			Entry E = A.ent[e];
+			u32 r = E.octree, c = 0;
+			if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
+				u32 ch[8];
+				bool all_null = true, same_color;
+				for (u32 i = 0; i < 8; ++i) {
+					ch[i] = color_child_result(A, C, e, i);
+					all_null = all_null && ctag(ch[i]) == kTagNull;
+				}
+				same_color = ctag(ch[0]) == kTagColor;
+				for (u32 i = 1; i < 8 && same_color; ++i)
+					same_color = ch[i] == ch[0];
+				if (all_null)
+					r = kColorNull;
+				else if (same_color)
+					r = ch[0];
+				else {
+					bool unchanged = ctag(E.octree) == kTagNode;
+					for (u32 i = 0; i < 8 && unchanged; ++i)
+						unchanged = P.cnodes[(u64)cdata(E.octree) * 8u + i] == ch[i];
+					r = unchanged ? E.octree : kColorNewNode;
+				}
+				if (l + 1 < (int)leafL) {
+					u32 m = A.pmask[e], fc = A.first_child[e];
+					for (u32 k = 0; k < (u32)__builtin_popcount(m); ++k)
+						if (fc + k < C.cap)
+							c += C.ccnt[fc + k];
+				}
+				c += r == kColorNewNode ? 1u : 0u;
+			}
+			A.ccnt[e] = c;
+			A.ores[e] = r;
+		}
+		__syncthreads();
+	}
+	const bool any = leafL > 0 && pm[kPmHdr] > 0;
+	if (threadIdx.x == 0 && any)
+		lv[0].cbase[0] = P.meta[kMetaColorNodes];
+	__syncthreads();
+	for (u32 l = 0; l < leafL; ++l) { // pass B
+		const ALevel A = lv[l], C = lv[l + 1];
+		const u32 n = pm[kPmHdr + l];
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			u32 run = A.cbase[e];
+			if (l + 1 < leafL) {
+				u32 m = A.pmask[e], fc = A.first_child[e];
+				for (u32 k = 0; k < (u32)__builtin_popcount(m); ++k)
+					if (fc + k < C.cap) {
+						C.cbase[fc + k] = run;
+						run += C.ccnt[fc + k];
+					}
+			}
+			A.own[e] = run;
+		}
+		__syncthreads();
+	}
+	for (int l = (int)leafL - 1; l >= 0; --l) { // pass C
+		const ALevel A = lv[l], C = lv[l + 1];
+		const u32 n = pm[kPmHdr + l];
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			if (A.ores[e] != kColorNewNode)
+				continue;
+			u32 idx = A.own[e], r;
+			if ((u64)idx < node_cap) {
+				for (u32 i = 0; i < 8; ++i)
+					P.cnodes[(u64)idx * 8u + i] = color_child_result(A, C, e, i);
+				r = cmake(kTagNode, idx);
+			} else {
+				r = A.ent[e].octree;
+				atomicOr(&P.meta[kMetaOverflow], 2u);
+			}
+			A.ores[e] = r;
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		if (pm[kPmHdr] > 0) {
+			if (leafL > 0)
+				P.meta[kMetaColorNodes] += lv[0].ccnt[0];
+			roots[1] = lv[0].ores[0];
+		}
+		P.meta[kMetaColorLeaves] += pm[kPmLeafTotal];
+	}
+}
+
+// Single workgroup: builds + walks of levels [0, l_from] (small ones) bottom-up, the root result,
+// the pass statistics and, for a colored pass, SetNode.
+__global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
+                                                int l_from, u32 *__restrict__ bcnt, u32 *__restrict__ head,
+                                                u32 *__restrict__ roots, u32 *__restrict__ acc, u64 node_cap) {
+	const u32 L = P.c.L;
+	for (int l = l_from; l >= 0; --l) {
+		const ALevel A = lv[l];
+		const u32 n = pm[kPmHdr + l];
+		if ((u32)l + 1 < L) {
+			const ALevel C = lv[l + 1];
+			for (u32 e = threadIdx.x; e < n; e += blockDim.x)
+				build_entry(P, (u32)l, A, C, e, bcnt, head);
+			__syncthreads();
+		}
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x)
+			walk_entry(P, (u32)l, A, e, bcnt, head, acc);
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		if (pm[kPmHdr] > 0)
+			roots[0] = child_result(lv[0], 0);
+		u32 tot = 0;
+		for (u32 l = 0; l < L; ++l)
+			tot += pm[kPmHdr + l];
+		acc[kAccEntries] += tot;
+		if (P.stateful)
+			acc[kAccJobs] += pm[kPmHdr + P.c.color_leaf];
+		acc[kAccOverflow] |= pm[kPmOverflow];
+	}
+	if (P.stateful) {
+		__syncthreads();
+		setnode_block(P, lv, pm, roots, node_cap);
+	}
+}
+
+__global__ void a_set_roots(u32 *roots, u32 root, u32 croot, u32 *acc) {
+	if (threadIdx.x | blockIdx.x)
+		return;
+	roots[0] = root;
+	roots[1] = croot;
+	for (u32 i = 0; i < kAccCount; ++i)
+		acc[i] = 0;
+}
+__global__ void a_set_roots_keep(u32 *roots, u32 root, u32 croot) {
+	if (threadIdx.x | blockIdx.x)
+		return;
+	roots[0] = root;
+	roots[1] = croot;
+}
+__global__ void a_fill_u32(u32 *p, u64 n, u32 v) {
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x)
+		p[i] = v;
+}
+
 // ---------------------------------------------------------------- host orchestration
 struct Level {
 	Entry *ent = nullptr;
@@ -1113,20 +1977,363 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 	}
 }
 
-int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
-	std::vector<std::vector<u32>> groups;
-	schedule_groups(edits, n, 1u << p.dc.voxel_level, groups);
-	std::vector<hd_edit_desc> g_edits;
-	for (auto &g : groups) {
-		g_edits.clear();
-		for (u32 k : g)
-			g_edits.push_back(edits[k]);
-		HD_RET(edit_pass(p, root, croot, g_edits.data(), (u32)g_edits.size(), &root, &croot));
+// ---------------------------------------------------------------- sync-free pass: host side
+constexpr u32 kTopPairs = 4096;  // a level whose 8 n pairs fit this bound expands inside a_top
+constexpr u32 kSmallLevel = 2048; // levels up to this many entries build + walk inside a_bottom
+constexpr u64 kAsyncBudget = 32ull << 30; // scratch bytes a sync-free pass may use
+
+// Upper bounds on the proceeding nodes of a pass, per level. A node proceeds only where some
+// edit's boundary crosses it: for a sphere the closed node cube [lb, lb + S]^3 meets the sphere
+// surface, so it lies in the shell r +- S sqrt(3) (cubes are interior-disjoint: count <= shell
+// volume / S^3) and within the sphere's box widened by one node on the low side; for a box it
+// overlaps the box without lying inside it. Every proceeding node is crossed by the boundary of
+// one of its edits, so per-edit sums S bound the entries of every level.
+struct PassPlan {
+	u32 first = 0, count = 0;
+	bool stateful = false, async = false;
+	std::vector<u64> cap, list_cap;
+	u64 ev_cap = 0, bytes = 0;
+};
+
+static u64 edit_level_bound(const hd_edit_desc &d, u32 voxel_level, u32 l) {
+	const u32 s = voxel_level - l;
+	const i64 N = (i64)1 << l, S = (i64)1 << s; // nodes per axis, node size
+	if (d.kind <= HD_EDIT_SPHERE_PAINT) {
+		// sphere_node tests the closed cube [lb, lb + S]: a node proceeds only if that cube meets
+		// the sphere surface, so lb <= c + r and lb + S >= c - r (r = floor(sqrt(r2)))
+		i64 r = (i64)std::sqrt((double)d.r2);
+		while ((r + 1) * (r + 1) <= (i64)d.r2)
+			++r;
+		while (r * r > (i64)d.r2)
+			--r;
+		u64 a = 1;
+		for (int i = 0; i < 3; ++i) {
+			i64 lo = (i64)d.p0[i] - r - S, hi = (i64)d.p0[i] + r; // lb in [lo, hi]
+			i64 f = lo <= 0 ? 0 : (lo + S - 1) / S, t = hi < 0 ? -1 : std::min(hi / S, N - 1);
+			a *= t >= f ? (u64)(t - f + 1) : 0u;
+		}
+		double rr = std::sqrt((double)d.r2), tt = (double)S * std::sqrt(3.0);
+		double ro = rr + tt, ri = rr > tt ? rr - tt : 0.0;
+		double shell = 4.0 / 3.0 * M_PI * (ro * ro * ro - ri * ri * ri) / ((double)S * S * S);
+		u64 b = (u64)std::ceil(shell * (1.0 + 1e-9)) + 8u;
+		return std::min(a, b);
 	}
-	p.last_groups = (u32)groups.size();
+	if (d.kind <= HD_EDIT_AABB_PAINT) {
+		// aabb_node: affected iff ub > p0 and lb < p1, filled iff lb >= p0 and ub <= p1 (per axis);
+		// an empty box (p1 <= p0) still proceeds through the nodes that straddle it
+		u64 a = 1, in = 1;
+		for (int i = 0; i < 3; ++i) {
+			i64 p0 = d.p0[i], p1 = d.p1[i];
+			i64 f = p0 / S, t = std::min((p1 + S - 1) / S - 1, N - 1); // ub > p0, lb < p1
+			a *= t >= f ? (u64)(t - f + 1) : 0u;
+			i64 fi = (p0 + S - 1) / S, ti = std::min(p1 / S, N); // [fi, ti) lie inside
+			in *= ti > fi ? (u64)(ti - fi) : 0u;
+		}
+		return a - std::min(a, in);
+	}
+	return (u64)N * N * N;
+}
+
+static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
+	const u32 L = p.dc.L, leafL = p.dc.color_leaf, vl = p.dc.voxel_level;
+	pp.stateful = false;
+	bool column = false;
+	for (u32 k = 0; k < pp.count; ++k) {
+		pp.stateful |= is_stateful(edits[pp.first + k]);
+		column |= edits[pp.first + k].kind == HD_EDIT_COLUMN_FILL;
+	}
+	pp.async = !column && pp.count > 0;
+	if (!pp.async)
+		return;
+	std::vector<u64> S(L, 0);
+	for (u32 k = 0; k < pp.count; ++k)
+		for (u32 l = 1; l < L; ++l)
+			S[l] += edit_level_bound(edits[pp.first + k], vl, l);
+	pp.cap.assign(L, 0);
+	pp.list_cap.assign(L, 0);
+	pp.cap[0] = 1;
+	pp.list_cap[0] = pp.count > 1 ? pp.count : 0;
+	const u64 lim = 1ull << 31;
+	for (u32 l = 1; l < L; ++l) {
+		pp.cap[l] = std::min(std::min(S[l], 8 * pp.cap[l - 1]), lim);
+		// a multi-edit node is crossed by the boundary of all its edits but at most one, which
+		// fills it (two fills would overlap): list words <= 2 S
+		pp.list_cap[l] = pp.count > 1 ? std::min(2 * S[l], lim) : 0;
+	}
+	pp.ev_cap = 0;
+	if (pp.stateful)
+		for (u32 l = leafL; l + 1 < L; ++l)
+			pp.ev_cap += 8 * pp.cap[l];
+	pp.ev_cap = std::min<u64>(pp.ev_cap, lim);
+	// bytes: entries + per-level arrays, expand temporaries, lists, events, dense colors
+	u64 b = 0, maxpairs = 0;
+	for (u32 l = 0; l < L; ++l) {
+		u64 c = pp.cap[l];
+		b += c * (sizeof(Entry) + 4 * 9 /*content*/ + 4 * 10) + 4 * pp.list_cap[l];
+		if (l + 1 < L) {
+			b += c * 8 * 4 * (pp.stateful && l < leafL ? 2 : 1) + c * 8;
+			maxpairs = std::max(maxpairs, 8 * c);
+		}
+	}
+	b += maxpairs * 24 + pp.ev_cap * sizeof(PushEvent);
+	if (pp.stateful)
+		b += pp.cap[leafL] * ((u64)sizeof(Color) << (3 * (vl - leafL))) + pp.cap[leafL] * 24;
+	pp.bytes = b;
+	if (b > kAsyncBudget || (pp.stateful && vl - leafL > 7))
+		pp.async = false;
+}
+
+struct AsyncBufs {
+	std::vector<ALevel> lv;
+	ALevel *d_lv = nullptr;
+	u32 *pm = nullptr, *roots = nullptr, *acc = nullptr, *bcnt = nullptr, *head = nullptr;
+	u32 *flag = nullptr, *lcnt = nullptr, *info = nullptr, *app = nullptr, *appo = nullptr;
+	uint4 *cstate = nullptr;
+	Pair *bsum = nullptr;
+	PushEvent *ev = nullptr;
+	Color *dense = nullptr;
+	hd_edit_desc *edits = nullptr;
+};
+constexpr u32 kMaxEmitBlocks = 1024;
+
+// Buffers for the largest bounds of the call's sync-free passes (allocated before any pass is
+// queued: a growing buffer would be freed under queued work).
+static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::vector<hd_edit_desc> &flat,
+                       AsyncBufs &B) {
+	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
+	std::vector<u64> cap(L, 0), lcap(L, 0);
+	u64 ev = 0, maxpairs = 8;
+	bool stateful = false;
+	for (auto &pp : plans) {
+		if (!pp.async)
+			continue;
+		for (u32 l = 0; l < L; ++l) {
+			cap[l] = std::max(cap[l], pp.cap[l]);
+			lcap[l] = std::max(lcap[l], pp.list_cap[l]);
+			if (l + 1 < L)
+				maxpairs = std::max(maxpairs, 8 * pp.cap[l]);
+		}
+		ev = std::max(ev, pp.ev_cap);
+		stateful |= pp.stateful;
+	}
+	B.lv.assign(L, ALevel{});
+	for (u32 l = 0; l < L; ++l) {
+		ALevel &A = B.lv[l];
+		const u64 c = std::max<u64>(cap[l], 1);
+		A.cap = (u32)cap[l];
+		A.list_cap = (u32)lcap[l];
+		HD_RET(buf(p, "a_ent", l, c, &A.ent));
+		HD_RET(buf(p, "a_lists", l, std::max<u64>(lcap[l], 1), &A.lists));
+		if (l + 1 < L) {
+			HD_RET(buf(p, "a_kids", l, 8 * c, &A.kids));
+			HD_RET(buf(p, "a_fchild", l, c, &A.first_child));
+			HD_RET(buf(p, "a_pmask", l, c, &A.pmask));
+			if (stateful && l < leafL)
+				HD_RET(buf(p, "a_okids", l, 8 * c, &A.okids));
+		}
+		HD_RET(buf(p, "a_status", l, c, &A.status));
+		HD_RET(buf(p, "a_pend", l, c, &A.pend));
+		HD_RET(buf(p, "a_res", l, c, &A.res));
+		HD_RET(buf(p, "a_failed", l, c, &A.failed));
+		HD_RET(buf(p, "a_next", l, c, &A.next));
+		HD_RET(buf(p, "a_lpos", l, c, &A.lpos));
+		HD_RET(buf(p, "a_bucket", l, c, &A.bucket));
+		HD_RET(buf(p, "a_content", l, c * (l + 1 < L ? 9 : 2), &A.content));
+		if (l + 1 < L)
+			HD_RET(buf(p, "a_csize", l, c, &A.csize));
+		if (stateful && l <= leafL) {
+			HD_RET(buf(p, "a_ores", l, c, &A.ores));
+			HD_RET(buf(p, "a_ccnt", l, c, &A.ccnt));
+			HD_RET(buf(p, "a_cbase", l, c, &A.cbase));
+			HD_RET(buf(p, "a_own", l, c, &A.own));
+		}
+	}
+	HD_RET(p.scratch.get<ALevel>("a_levels", L, &B.d_lv));
+	HD_TRY(hipMemcpyAsync(B.d_lv, B.lv.data(), L * sizeof(ALevel), hipMemcpyHostToDevice, p.stream));
+	HD_RET(p.scratch.get<u32>("a_pm", kPmHdr + 2 * L, &B.pm));
+	HD_RET(p.scratch.get<u32>("a_roots", 2, &B.roots));
+	HD_RET(p.scratch.get<u32>("a_acc", kAccCount, &B.acc));
+	HD_RET(p.scratch.get<u32>("a_flag", maxpairs, &B.flag));
+	HD_RET(p.scratch.get<u32>("a_lcnt", maxpairs, &B.lcnt));
+	HD_RET(p.scratch.get<uint4>("a_cstate", maxpairs, &B.cstate));
+	HD_RET(p.scratch.get<Pair>("a_bsum", kMaxEmitBlocks, &B.bsum));
+	HD_RET(p.scratch.get<PushEvent>("a_events", std::max<u64>(ev, 1), &B.ev));
+	if (stateful) {
+		const u64 jobs = std::max<u64>(cap[leafL], 1);
+		HD_RET(p.scratch.get<Color>("a_dense", jobs << (3 * (p.dc.voxel_level - leafL)), &B.dense));
+		HD_RET(p.scratch.get<u32>("a_info", 4 * jobs, &B.info));
+		HD_RET(p.scratch.get<u32>("a_app", jobs, &B.app));
+		HD_RET(p.scratch.get<u32>("a_appo", jobs, &B.appo));
+	}
+	HD_RET(p.scratch.get<hd_edit_desc>("a_edits", std::max<size_t>(flat.size(), 1), &B.edits));
+	HD_TRY(hipMemcpyAsync(B.edits, flat.data(), flat.size() * sizeof(hd_edit_desc), hipMemcpyHostToDevice, p.stream));
+	// per-bucket miss lists: empty between levels (every walker resets its bucket)
+	const char *kb = "a_bcnt", *kh = "a_head";
+	bool fresh = p.scratch.bufs.find(kb) == p.scratch.bufs.end();
+	HD_RET(p.scratch.get<u32>(kb, p.total_buckets, &B.bcnt));
+	HD_RET(p.scratch.get<u32>(kh, p.total_buckets, &B.head));
+	if (fresh) {
+		u32 g = std::min<u64>(div_up(p.total_buckets, 256), 4096);
+		a_fill_u32<<<g, 256, 0, p.stream>>>(B.bcnt, p.total_buckets, 0u);
+		a_fill_u32<<<g, 256, 0, p.stream>>>(B.head, p.total_buckets, kNull);
+		HD_TRY(hipGetLastError());
+	}
+	HD_TRY(hipStreamSynchronize(p.stream)); // the host staging arrays may go out of scope
+	return HD_OK;
+}
+
+static EditParams pass_params(const Pool &p, const hd_edit_desc *d_edits, u32 n, bool stateful) {
+	EditParams P;
+	P.c = p.dc;
+	P.hm = p.d_hm;
+	P.edits = d_edits;
+	P.n_edits = n;
+	P.words = p.words;
+	P.bucket_words = p.bucket_words;
+	P.filled = p.d_filled;
+	P.cnodes = p.cnodes;
+	P.cleaves = p.cleaves;
+	P.meta = p.meta;
+	P.stateful = stateful ? 1u : 0u;
+	P.octree_root = 0;
+	P.leaf_bits = p.dc.voxel_level - p.dc.color_leaf;
+	return P;
+}
+
+static u32 grid_for(u64 items, u32 per_block, u32 max_blocks) {
+	return (u32)std::max<u64>(1, std::min<u64>(div_up(std::max<u64>(items, 1), per_block), max_blocks));
+}
+
+// Queues one sync-free pass (no host synchronization).
+static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B) {
+	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
+	const EditParams P = pass_params(p, B.edits + pp.first, pp.count, pp.stateful);
+	const u32 V = pp.stateful ? 1u << (3u * P.leaf_bits) : 0u;
+	u32 top_end = 0;
+	while (top_end + 1 < L && 8 * pp.cap[top_end] <= kTopPairs)
+		++top_end;
+	a_top<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.roots, B.pm, kPmHdr + 2 * L, top_end, B.ev, (u32)pp.ev_cap);
+	for (u32 l = top_end; l + 1 < L; ++l) {
+		u32 G = grid_for(8 * pp.cap[l], 2048, kMaxEmitBlocks);
+		a_count<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.ev, (u32)pp.ev_cap, B.bsum);
+		a_emit<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.bsum);
+	}
+	const bool colors = pp.stateful && pp.cap[leafL] > 0;
+	if (colors) {
+		a_dense_init<<<grid_for(pp.cap[leafL] * V, 256, 16384), 256, 0, p.stream>>>(B.d_lv, leafL, B.pm,
+		                                                                            3u * P.leaf_bits, p.cleaves, B.dense);
+		if (pp.ev_cap)
+			a_push<<<grid_for(pp.ev_cap, 1, 8192), 256, 0, p.stream>>>(B.ev, B.pm, (u32)pp.ev_cap, V,
+			                                                          B.lv[leafL].cap, B.dense);
+	}
+	a_leaf<<<grid_for(pp.cap[L - 1], kAT / 64, 16384), kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V, B.bcnt, B.head);
+	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
+	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= kSmallLevel)
+		++l_from;
+	for (int l = (int)L - 1; l > l_from; --l) {
+		u32 G = grid_for(pp.cap[l], kAT, 4096);
+		if (l + 1 < (int)L)
+			a_build<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.bcnt, B.head);
+		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.bcnt, B.head, B.acc);
+	}
+	if (colors) {
+		u32 G = grid_for(pp.cap[leafL], 1, 8192);
+		a_encode_size<<<G, kEncThreads, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app);
+		a_scan_app<<<1, kAT, 0, p.stream>>>(leafL, B.pm, B.app, B.appo);
+		a_encode_write<<<G, kEncThreads, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap);
+	}
+	a_bottom<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, l_from, B.bcnt, B.head, B.roots, B.acc, p.cnode_cap);
+	HD_TRY(hipGetLastError());
+	return HD_OK;
+}
+
+static bool force_sync_path() {
+	const char *e = getenv("HD_EDIT_PATH");
+	return e && strcmp(e, "sync") == 0;
+}
+
+// Applies the passes in order: sync-free passes queue back to back with the roots in device
+// memory; a pass the bounds rule out (column fills, scratch over budget) runs the synchronized
+// pipeline in between.
+static int edit_sequence(Pool &p, u32 root, u32 croot, const std::vector<hd_edit_desc> &flat,
+                         std::vector<PassPlan> &plans, u32 *root_out, u32 *croot_out) {
+	HD_RET(make_filled(p));
+	const bool sync_only = force_sync_path();
+	bool any_async = false;
+	for (auto &pp : plans) {
+		plan_pass(p, flat.data(), pp);
+		if (sync_only)
+			pp.async = false;
+		any_async |= pp.async;
+	}
+	if (!any_async) {
+		for (auto &pp : plans)
+			HD_RET(edit_pass(p, root, croot, flat.data() + pp.first, pp.count, &root, &croot));
+		*root_out = root;
+		*croot_out = croot;
+		return HD_OK;
+	}
+	AsyncBufs B;
+	HD_RET(async_setup(p, plans, flat, B));
+	a_set_roots<<<1, 64, 0, p.stream>>>(B.roots, root, croot, B.acc);
+	bool on_device = true;
+	for (auto &pp : plans) {
+		if (pp.async) {
+			if (!on_device)
+				a_set_roots_keep<<<1, 64, 0, p.stream>>>(B.roots, root, croot);
+			on_device = true;
+			HD_RET(queue_pass(p, pp, B));
+		} else {
+			if (on_device) {
+				u32 rr[2];
+				HD_RET(read_u32s(p, B.roots, rr, 2));
+				root = rr[0], croot = rr[1];
+			}
+			on_device = false;
+			HD_RET(edit_pass(p, root, croot, flat.data() + pp.first, pp.count, &root, &croot));
+		}
+	}
+	u32 out[2 + kAccCount];
+	HD_TRY(hipMemcpyAsync(out, B.roots, 8, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipMemcpyAsync(out + 2, B.acc, 4 * kAccCount, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	if (on_device)
+		root = out[0], croot = out[1];
+	p.last_entries += out[2 + kAccEntries];
+	p.last_upserts += out[2 + kAccUpserts];
+	p.last_jobs += out[2 + kAccJobs];
+	if (out[2 + kAccOverflow])
+		return set_error("hd_edit: a level outgrew its bound (internal error)"), HD_ERR_CAPACITY;
 	*root_out = root;
 	*croot_out = croot;
 	return HD_OK;
+}
+
+int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
+	std::vector<std::vector<u32>> groups;
+	schedule_groups(edits, n, 1u << p.dc.voxel_level, groups);
+	std::vector<hd_edit_desc> flat;
+	std::vector<PassPlan> plans;
+	flat.reserve(n);
+	for (auto &g : groups) {
+		PassPlan pp;
+		pp.first = (u32)flat.size();
+		pp.count = (u32)g.size();
+		for (u32 k : g)
+			flat.push_back(edits[k]);
+		plans.push_back(pp);
+	}
+	p.last_groups = (u32)groups.size();
+	return edit_sequence(p, root, croot, flat, plans, root_out, croot_out);
+}
+
+int edit_plain(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
+	std::vector<hd_edit_desc> flat(edits, edits + n);
+	std::vector<PassPlan> plans(n);
+	for (u32 i = 0; i < n; ++i)
+		plans[i].first = i, plans[i].count = 1;
+	p.last_groups = n;
+	return edit_sequence(p, root, croot, flat, plans, root_out, croot_out);
 }
 
 } // namespace hd
@@ -1141,11 +2348,18 @@ extern "C" int hd_edit(hd_pool *h, uint32_t root, uint32_t color_root, const hd_
 	auto t0 = std::chrono::steady_clock::now();
 	p.last_entries = p.last_upserts = p.last_jobs = 0;
 	u32 r = root, c = color_root;
-	if (mode == HD_MODE_BATCHED && n > 1) {
+	for (u32 i = 0; i < n; ++i) {
+		if (edits[i].kind > HD_EDIT_COLUMN_FILL)
+			return set_error("unknown edit kind %u", edits[i].kind), HD_ERR_ARG;
+		if (edits[i].kind == HD_EDIT_COLUMN_FILL && !p.hm_mn)
+			return set_error("column fill needs hd_pool_set_heightmap"), HD_ERR_ARG;
+	}
+	if (n == 0) {
+		p.last_groups = 0;
+	} else if (mode == HD_MODE_BATCHED && n > 1) {
 		HD_RET(edit_batched(p, r, c, edits, n, &r, &c));
 	} else {
-		for (u32 i = 0; i < n; ++i)
-			HD_RET(edit_pass(p, r, c, edits + i, 1, &r, &c));
+		HD_RET(edit_plain(p, r, c, edits, n, &r, &c));
 	}
 	HD_TRY(hipStreamSynchronize(p.stream));
 	p.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
