@@ -809,7 +809,7 @@ struct ALevel {
 	u32 *kids;                // 8 per entry: new pointer of each terminal child
 	u32 *okids;               // 8 per entry: color state of each child (levels above the color leaves)
 	u32 *first_child, *pmask; // index of the first proceeding child at the next level + their slots
-	u32 *status, *pend, *res, *failed, *next, *lpos, *bucket;
+	u32 *status, *pend, *res, *slot, *lpos, *bucket;
 	u32 *content, *csize;
 	u32 *ores, *ccnt, *cbase, *own; // color results, SetNode passes
 	u32 cap, list_cap;
@@ -869,15 +869,19 @@ __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &
 		else
 			atomicOr(&pm[kPmOverflow], 1u);
 	}
-	A.kids[t] = c.t == kClear ? kNull : (c.t == kFill ? P.filled[cl] : old);
+	const u32 kid = c.t == kClear ? kNull : (c.t == kFill ? P.filled[cl] : old);
+	A.kids[t] = kid; // the old child for proceeding and untouched slots
 	if (A.okids)
 		A.okids[t] = c.octree;
 	bool proceed = c.t == kProceed;
-	return PairOut{proceed ? 1u : 0u, proceed && cnt > 1 ? cnt : 0u, make_uint4(c.octree, c.sf, single, cnt)};
+	// bit 0: proceeds, bit 1: a terminal child whose pointer changed
+	u32 flag = (proceed ? 1u : 0u) | (!proceed && kid != old ? 2u : 0u);
+	return PairOut{flag, proceed && cnt > 1 ? cnt : 0u, make_uint4(c.octree, c.sf, single, cnt)};
 }
 
-// Writes the child entry of a proceeding pair (k_emit). m8: proceed flags of the pair's node
-// (valid on slot 0), idx / lo: the child's entry index and list offset at level l + 1.
+// Writes the child entry of a proceeding pair (k_emit). m8: proceed flags of the pair's node in
+// bits 0-7 and its changed terminal children in bits 8-15 (valid on slot 0), idx / lo: the child's
+// entry index and list offset at level l + 1.
 __device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, const ALevel &N, u32 *__restrict__ pm,
                                  u64 t, u32 m8, const PairOut &o, u32 idx, u32 lo) {
 	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
@@ -885,7 +889,7 @@ __device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, co
 		A.first_child[e] = idx;
 		A.pmask[e] = m8;
 	}
-	if (!o.flag)
+	if (!(o.flag & 1u))
 		return;
 	Entry E = A.ent[e];
 	u32 sf = o.cs.y;
@@ -940,12 +944,14 @@ __device__ inline Pair expand_emit_range(const EditParams &P, u32 l, const ALeve
 		}
 		sflag[threadIdx.x] = o.flag;
 		Pair tile;
-		Pair off = block_excl_scan2(Pair{o.flag, o.lcnt}, sh, tile); // synchronizes sflag
+		Pair off = block_excl_scan2(Pair{o.flag & 1u, o.lcnt}, sh, tile); // synchronizes sflag
 		if (in) {
 			u32 m8 = 0;
 			if ((t & 7u) == 0)
-				for (u32 k = 0; k < 8; ++k)
-					m8 |= sflag[threadIdx.x + k] << k;
+				for (u32 k = 0; k < 8; ++k) {
+					u32 f = sflag[threadIdx.x + k];
+					m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k));
+				}
 			emit_pair(P, l, A, N, pm, t, m8, o, base.a + off.a, base.b + off.b);
 		}
 		base.a += tile.a;
@@ -1041,7 +1047,7 @@ __global__ void __launch_bounds__(kAT) a_count(EditParams P, u32 l, const ALevel
 		flag[t] = o.flag;
 		lcnt[t] = o.lcnt;
 		cstate[t] = o.cs;
-		acc.a += o.flag;
+		acc.a += o.flag & 1u;
 		acc.b += o.lcnt;
 	}
 	Pair tot;
@@ -1113,16 +1119,33 @@ __device__ inline u32 wave_find_leaf(const u32 *__restrict__ W, const DevConfig 
 	return kNull;
 }
 
-// queues a node that is not in its bucket yet: the first arrival (lpos 0) walks the list later
-__device__ inline void queue_miss(const ALevel &A, u32 e, u32 b, u32 *__restrict__ bcnt, u32 *__restrict__ head) {
-	A.lpos[e] = atomicAdd(&bcnt[b], 1u);
-	A.next[e] = atomicExch(&head[b], e);
+// Within-level dedup of the new nodes that are not in the pool yet, in a table shared by all
+// levels: slot = (gen << 32) | entry, any other generation is empty, so the table is never
+// cleared. Identical contents share a slot (a group) whose entry is the lowest index (the first
+// in DFS order, the one the reference appends; the others find it). The entry that created the
+// group queues it on its bucket's list; per-group results live beside the slot.
+struct GroupTable {
+	u64 *tab;
+	u32 *gres, *gfail, *gnext;
+	u32 mask;
+};
+
+// Result of a child entry once its level's upserts are done (k_resolve's rule).
+__device__ inline u32 child_result(const ALevel &C, const GroupTable &G, u32 c) {
+	if (c >= C.cap)
+		return kNull; // unreachable with sound bounds (the pass reports an overflow)
+	if (!C.pend[c])
+		return C.status[c];
+	u32 r = C.res[c];
+	if (r != kNull)
+		return r; // found among the bucket's existing nodes
+	u32 s = C.slot[c];
+	return G.gfail[s] ? C.ent[c].ptr : G.gres[s];
 }
 
 // edit_leaf (k_leaf) + the find among the bucket's existing leaves: one wavefront per leaf.
 __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
-                                              Color *__restrict__ dense, u32 V, u32 *__restrict__ bcnt,
-                                              u32 *__restrict__ head) {
+                                              Color *__restrict__ dense, u32 V) {
 	const u32 lane = threadIdx.x & 63u, level = P.c.L - 1;
 	const ALevel A = lv[level];
 	const u32 n = pm[kPmHdr + level];
@@ -1179,76 +1202,118 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 			A.content[2 * (u64)e + 1] = n1;
 			A.bucket[e] = b;
 			A.res[e] = f;
-			A.failed[e] = 0;
-			if (f == kNull)
-				queue_miss(A, e, b, bcnt, head);
-			else
-				A.lpos[e] = kNull;
 		}
 	}
 }
 
-// Result of a child entry once its level's upserts are done (k_resolve's rule).
-__device__ inline u32 child_result(const ALevel &C, u32 c) {
-	if (c >= C.cap)
-		return kNull; // unreachable with sound bounds (the pass reports an overflow)
-	if (!C.pend[c])
-		return C.status[c];
-	return C.failed[c] ? C.ent[c].ptr : C.res[c];
+// edit_node's join (k_build) for entry e of level l, with the children's results resolved from
+// level l + 1, then the find among the bucket's existing nodes.
+// The old child of each proceeding or untouched slot is in kids; pmask bits 8-15 flag the
+// terminal children whose pointer changed.
+struct Built {
+	u32 fin;   // final pointer when known (unchanged / emptied / found)
+	bool need; // new content that is not in the bucket yet
+	u32 k, h, b, c[9];
+};
+template <typename ChildFn>
+__device__ inline Built build_node(const EditParams &P, u32 l, const ALevel &A, u32 e, const Entry &E, ChildFn child) {
+	Built o;
+	const u32 pmw = A.pmask[e], pmk = pmw & 0xFFu, fc = A.first_child[e];
+	const uint4 k0 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e];
+	const uint4 k1 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e + 1];
+	const u32 kv[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+	bool changed = (pmw >> 8) != 0;
+	u32 nm = 0, nk[8];
+	for (u32 i = 0; i < 8; ++i) {
+		u32 r = kv[i];
+		if ((pmk >> i) & 1u) {
+			u32 cr = child(fc + (u32)__builtin_popcount(pmk & ((1u << i) - 1u)));
+			changed |= cr != r;
+			r = cr;
+		}
+		nk[i] = r;
+		if (r != kNull)
+			nm |= 1u << i;
+	}
+	o.need = false;
+	o.k = 0;
+	if (!changed || nm == 0) {
+		o.fin = changed ? kNull : E.ptr;
+		return o;
+	}
+	o.c[0] = nm;
+	u32 k = 1;
+	for (u32 i = 0; i < 8; ++i)
+		if (nk[i] != kNull)
+			o.c[k++] = nk[i];
+	o.k = k;
+	o.h = murmur_span(o.c, k);
+	o.b = P.c.level_base[l] + (o.h & ((1u << P.c.bucket_bits[l]) - 1u));
+	return o;
 }
 
 // edit_node's join (k_build) for entry e of level l, with the children's results resolved from
 // level l + 1, then the find among the bucket's existing nodes.
-__device__ inline void build_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, u32 e,
-                                   u32 *__restrict__ bcnt, u32 *__restrict__ head) {
-	Entry E = A.ent[e];
-	u32 om = 0, k = 1;
-	if (E.ptr != kNull)
-		om = P.words[E.ptr];
-	const u32 pmk = A.pmask[e], fc = A.first_child[e];
-	bool changed = false;
-	u32 nm = 0, nk[8];
-	for (u32 i = 0; i < 8; ++i) {
-		u32 o = ((om >> i) & 1u) ? P.words[E.ptr + k++] : kNull;
-		u32 r = ((pmk >> i) & 1u) ? child_result(C, fc + (u32)__builtin_popcount(pmk & ((1u << i) - 1u)))
-		                          : A.kids[(u64)e * 8u + i];
-		nk[i] = r;
-		changed |= r != o;
-		if (r != kNull)
-			nm |= 1u << i;
-	}
-	if (!changed || nm == 0) {
+__device__ inline void build_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, const GroupTable &G,
+                                   u32 e) {
+	const Entry E = A.ent[e];
+	Built o = build_node(P, l, A, e, E, [&](u32 c) { return child_result(C, G, c); });
+	if (!o.k) {
 		A.pend[e] = 0;
-		A.status[e] = changed ? kNull : E.ptr;
+		A.status[e] = o.fin;
 		return;
 	}
-	u32 c[9];
-	c[0] = nm;
-	k = 1;
-	for (u32 i = 0; i < 8; ++i)
-		if (nk[i] != kNull)
-			c[k++] = nk[i];
-	for (u32 j = 0; j < k; ++j)
-		A.content[(u64)e * 9u + j] = c[j];
-	u32 h = murmur_span(c, k);
-	u32 b = P.c.level_base[l] + (h & ((1u << P.c.bucket_bits[l]) - 1u));
-	u32 f = find_in_bucket(P.words, P.c, b, P.bucket_words[b], c, k, false);
+	for (u32 j = 0; j < o.k; ++j)
+		A.content[(u64)e * 9u + j] = o.c[j];
 	A.pend[e] = 1;
-	A.csize[e] = k;
-	A.bucket[e] = b;
-	A.res[e] = f;
-	A.failed[e] = 0;
-	if (f == kNull)
-		queue_miss(A, e, b, bcnt, head);
-	else
-		A.lpos[e] = kNull;
+	A.csize[e] = o.k;
+	A.bucket[e] = o.b;
+	A.res[e] = find_in_bucket(P.words, P.c, o.b, P.bucket_words[o.b], o.c, o.k, false);
 }
 
-// The walker of a bucket (the entry that arrived first) replays the bucket's misses in entry
-// order: find among the words appended by this walk, else append_node (NodePool.hpp:187-215).
-__device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, u32 e, u32 *__restrict__ bcnt,
-                                  u32 *__restrict__ head, u32 *__restrict__ acc) {
-	if (!A.pend[e] || A.lpos[e] != 0)
+// Joins entry e (pending, not in the pool) to its content's group; the creator queues the group.
+__device__ inline void dedup_entry(const ALevel &A, const GroupTable &G, u32 gen, bool leaf, u32 e,
+                                   u32 *__restrict__ bcnt, u32 *__restrict__ head) {
+	A.lpos[e] = kNull;
+	if (!A.pend[e] || A.res[e] != kNull)
+		return;
+	const u32 stride = leaf ? 2u : 9u, n = leaf ? 2u : A.csize[e];
+	const u32 *ce = A.content + (u64)e * stride;
+	u32 h = leaf ? murmur_leaf(ce[0], ce[1]) : murmur_span(ce, n);
+	u32 s = (h ^ (h >> 15)) * 0x2c1b3c6du;
+	s = (s ^ (s >> 12)) & G.mask;
+	const u64 mine = ((u64)gen << 32) | e;
+	for (;;) {
+		u64 cur = __hip_atomic_load(&G.tab[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if ((u32)(cur >> 32) != gen) {
+			if (atomicCAS((unsigned long long *)&G.tab[s], (unsigned long long)cur, (unsigned long long)mine) == cur) {
+				A.slot[e] = s;
+				u32 b = A.bucket[e];
+				A.lpos[e] = atomicAdd(&bcnt[b], 1u);
+				G.gnext[s] = atomicExch(&head[b], s);
+				return;
+			}
+			continue; // the slot changed under us: look again
+		}
+		u32 j = (u32)cur;
+		bool same = leaf || A.csize[j] == n;
+		for (u32 k = 0; k < n && same; ++k)
+			same = A.content[(u64)j * stride + k] == ce[k];
+		if (same) {
+			atomicMin((unsigned long long *)&G.tab[s], (unsigned long long)mine);
+			A.slot[e] = s;
+			return;
+		}
+		s = (s + 1u) & G.mask;
+	}
+}
+
+// The walker of a bucket (the group queued first) replays the bucket's new groups in the order of
+// their first entries (DFS order): find among the words appended by this walk, else append_node
+// (NodePool.hpp:187-215).
+__device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, const GroupTable &G, u32 e,
+                                  u32 *__restrict__ bcnt, u32 *__restrict__ head, u32 *__restrict__ acc) {
+	if (A.lpos[e] != 0)
 		return;
 	const bool leaf = l == P.c.L - 1;
 	const u32 stride = leaf ? 2u : 9u;
@@ -1256,7 +1321,9 @@ __device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, u
 	const u32 bw0 = P.bucket_words[b];
 	u32 bw = bw0, appended = 0;
 	bool any_fail = false;
-	auto upsert = [&](u32 j) {
+	auto rep_of = [&](u32 s) { return (u32)G.tab[s]; };
+	auto upsert = [&](u32 s) {
+		const u32 j = rep_of(s);
 		const u32 *node = A.content + (u64)j * stride;
 		u32 nsz = leaf ? 2u : A.csize[j];
 		u32 f = find_in_bucket_from(P.words, P.c, b, bw0, bw, node, nsz, leaf);
@@ -1268,34 +1335,36 @@ __device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, u
 			else
 				++appended;
 		}
-		A.res[j] = f;
-		A.failed[j] = fail;
+		G.gres[s] = f;
+		G.gfail[s] = fail;
 	};
 	if (k <= kGather) {
-		u32 ord[kGather];
+		u32 ord[kGather], key[kGather];
 		u32 g = 0;
-		for (u32 j = head[b]; j != kNull && g < kGather; j = A.next[j])
-			ord[g++] = j;
-		for (u32 x = 1; x < g; ++x) { // insertion sort: entry (DFS) order
-			u32 v = ord[x], y = x;
-			while (y > 0 && ord[y - 1] > v) {
+		for (u32 s = head[b]; s != kNull && g < kGather; s = G.gnext[s]) {
+			u32 r = rep_of(s), y = g++;
+			while (y > 0 && key[y - 1] > r) { // insertion sort by first entry
+				key[y] = key[y - 1];
 				ord[y] = ord[y - 1];
 				--y;
 			}
-			ord[y] = v;
+			key[y] = r;
+			ord[y] = s;
 		}
 		for (u32 x = 0; x < g; ++x)
 			upsert(ord[x]);
-	} else {
+	} else { // long lists (distinct new nodes of one bucket): repeated minimum search
 		u32 last = 0;
 		for (u32 r = 0; r < k; ++r) {
-			u32 best = kNull;
-			for (u32 j = head[b]; j != kNull; j = A.next[j])
+			u32 best = kNull, bs = kNull;
+			for (u32 s = head[b]; s != kNull; s = G.gnext[s]) {
+				u32 j = rep_of(s);
 				if ((r == 0 || j > last) && j < best)
-					best = j;
-			if (best == kNull)
+					best = j, bs = s;
+			}
+			if (bs == kNull)
 				break;
-			upsert(best);
+			upsert(bs);
 			last = best;
 		}
 	}
@@ -1309,20 +1378,28 @@ __device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, u
 }
 
 __global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel *__restrict__ lv,
-                                               const u32 *__restrict__ pm, u32 *__restrict__ bcnt,
-                                               u32 *__restrict__ head) {
+                                               const u32 *__restrict__ pm, GroupTable G) {
 	const ALevel A = lv[l], C = lv[l + 1];
 	const u32 n = pm[kPmHdr + l];
 	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-		build_entry(P, l, A, C, e, bcnt, head);
+		build_entry(P, l, A, C, G, e);
+}
+__global__ void __launch_bounds__(kAT) a_dedup(EditParams P, u32 l, const ALevel *__restrict__ lv,
+                                               const u32 *__restrict__ pm, GroupTable G, u32 gen,
+                                               u32 *__restrict__ bcnt, u32 *__restrict__ head) {
+	const ALevel A = lv[l];
+	const u32 n = pm[kPmHdr + l];
+	const bool leaf = l == P.c.L - 1;
+	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+		dedup_entry(A, G, gen, leaf, e, bcnt, head);
 }
 __global__ void __launch_bounds__(kAT) a_walk(EditParams P, u32 l, const ALevel *__restrict__ lv,
-                                              const u32 *__restrict__ pm, u32 *__restrict__ bcnt,
+                                              const u32 *__restrict__ pm, GroupTable G, u32 *__restrict__ bcnt,
                                               u32 *__restrict__ head, u32 *__restrict__ acc) {
 	const ALevel A = lv[l];
 	const u32 n = pm[kPmHdr + l];
 	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-		walk_entry(P, l, A, e, bcnt, head, acc);
+		walk_entry(P, l, A, G, e, bcnt, head, acc);
 }
 
 // ---- colors: encode + SetLeaf (one block per color leaf job, grid-stride)
@@ -1472,7 +1549,7 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_write(EditParams P, cons
 // bottom-up decisions + subtree append counts, pass B top-down post-order indices, pass C
 // bottom-up writes; then the pool counters and the color root.
 __device__ inline u32 color_child_result(const ALevel &A, const ALevel &C, u32 e, u32 i) {
-	u32 m = A.pmask[e];
+	u32 m = A.pmask[e] & 0xFFu;
 	if ((m >> i) & 1u) {
 		u32 c = A.first_child[e] + (u32)__builtin_popcount(m & ((1u << i) - 1u));
 		return c < C.cap ? C.ores[c] : kColorNull;
@@ -1509,7 +1586,7 @@ __device__ inline void setnode_block(const EditParams &P, const ALevel *__restri
 					r = unchanged ? E.octree : kColorNewNode;
 				}
 				if (l + 1 < (int)leafL) {
-					u32 m = A.pmask[e], fc = A.first_child[e];
+					u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
 					for (u32 k = 0; k < (u32)__builtin_popcount(m); ++k)
 						if (fc + k < C.cap)
 							c += C.ccnt[fc + k];
@@ -1531,7 +1608,7 @@ __device__ inline void setnode_block(const EditParams &P, const ALevel *__restri
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
 			u32 run = A.cbase[e];
 			if (l + 1 < leafL) {
-				u32 m = A.pmask[e], fc = A.first_child[e];
+				u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
 				for (u32 k = 0; k < (u32)__builtin_popcount(m); ++k)
 					if (fc + k < C.cap) {
 						C.cbase[fc + k] = run;
@@ -1571,28 +1648,329 @@ __device__ inline void setnode_block(const EditParams &P, const ALevel *__restri
 	}
 }
 
-// Single workgroup: builds + walks of levels [0, l_from] (small ones) bottom-up, the root result,
-// the pass statistics and, for a colored pass, SetNode.
+// ---- small levels in one workgroup, with the level bookkeeping in LDS
+constexpr u32 kSmallLevel = 1024; // entries of a level handled inside a_bottom
+constexpr u32 kSTab = 2048;       // LDS dedup / bucket-map slots (load factor <= 1/2)
+constexpr u32 kColorLds = 4096;   // SetNode entries (all levels above the color leaves) kept in LDS
+
+struct BottomLds {
+	u32 res[2][kSmallLevel]; // final pointers of the current level / its child level
+	u32 content[kSmallLevel][9];
+	u32 csz[kSmallLevel], hash[kSmallLevel], bucket[kSmallLevel], bw[kSmallLevel], slot[kSmallLevel];
+	u32 ptr[kSmallLevel];
+	u32 glist[kSmallLevel], gnext[kSmallLevel], gmap[kSmallLevel]; // groups; bucket-map slot of each
+	u32 tab[kSTab], gres[kSTab];                                   // dedup table; result per slot
+	u32 bkey[kSTab], bmin[kSTab], bhead[kSTab], bn[kSTab];         // bucket map: walker election
+	unsigned char gfail[kSTab];
+	u32 gcount;
+};
+struct ColorLds {
+	u32 r[kColorLds], cnt[kColorLds], base[kColorLds];
+	u32 off[HD_MAX_LEVELS + 1];
+	u32 fits;
+};
+union ALds {
+	BottomLds b;
+	ColorLds c;
+};
+
+__device__ inline bool lds_same(const BottomLds &S, u32 a, u32 b) {
+	if (S.csz[a] != S.csz[b])
+		return false;
+	for (u32 k = 0; k < S.csz[a]; ++k)
+		if (S.content[a][k] != S.content[b][k])
+			return false;
+	return true;
+}
+
+// Walk of a small level: groups of one bucket elect the one with the lowest first entry, which
+// replays them in that order (find among this walk's appends, else append_node).
+__device__ inline void small_walk(const EditParams &P, u32 l, BottomLds &S, u32 *__restrict__ acc) {
+	const bool leaf = l == P.c.L - 1;
+	const u32 ng = S.gcount;
+	for (u32 g = threadIdx.x; g < ng; g += blockDim.x) {
+		const u32 rep = S.tab[S.glist[g]], b = S.bucket[rep];
+		u32 m = (b * 0x9E3779B1u >> 7) & (kSTab - 1u);
+		for (;;) {
+			u32 cur = atomicCAS(&S.bkey[m], kNull, b);
+			if (cur == kNull || cur == b)
+				break;
+			m = (m + 1u) & (kSTab - 1u);
+		}
+		atomicMin(&S.bmin[m], rep);
+		atomicAdd(&S.bn[m], 1u);
+		S.gnext[g] = atomicExch(&S.bhead[m], g);
+		S.gmap[g] = m;
+	}
+	__syncthreads();
+	for (u32 g = threadIdx.x; g < ng; g += blockDim.x) {
+		const u32 rep = S.tab[S.glist[g]], b = S.bucket[rep], m = S.gmap[g];
+		if (S.bmin[m] != rep)
+			continue;
+		const u32 bw0 = S.bw[rep], cnt = S.bn[m];
+		u32 bw = bw0, appended = 0;
+		bool any_fail = false;
+		auto upsert = [&](u32 g2) {
+			const u32 s2 = S.glist[g2], j = S.tab[s2];
+			const u32 *node = S.content[j];
+			u32 nsz = S.csz[j];
+			u32 f = bw > bw0 ? find_in_bucket_from(P.words, P.c, b, bw0, bw, node, nsz, leaf) : kNull;
+			unsigned char fail = 0;
+			if (f == kNull) {
+				f = append_to_bucket(P.words, P.c, b, bw, node, nsz);
+				if (f == kNull)
+					fail = 1, any_fail = true;
+				else
+					++appended;
+			}
+			S.gres[s2] = f;
+			S.gfail[s2] = fail;
+		};
+		if (cnt <= kGather) {
+			u32 ord[kGather], key[kGather], n = 0;
+			for (u32 x = S.bhead[m]; x != kNull && n < kGather; x = S.gnext[x]) {
+				u32 r2 = S.tab[S.glist[x]], y = n++;
+				while (y > 0 && key[y - 1] > r2) {
+					key[y] = key[y - 1];
+					ord[y] = ord[y - 1];
+					--y;
+				}
+				key[y] = r2;
+				ord[y] = x;
+			}
+			for (u32 x = 0; x < n; ++x)
+				upsert(ord[x]);
+		} else {
+			u32 last = 0;
+			for (u32 k = 0; k < cnt; ++k) {
+				u32 best = kNull, bg = kNull;
+				for (u32 x = S.bhead[m]; x != kNull; x = S.gnext[x]) {
+					u32 r2 = S.tab[S.glist[x]];
+					if ((k == 0 || r2 > last) && r2 < best)
+						best = r2, bg = x;
+				}
+				upsert(bg);
+				last = best;
+			}
+		}
+		P.bucket_words[b] = bw;
+		if (appended)
+			atomicAdd(&acc[kAccUpserts], appended);
+		if (any_fail)
+			atomicOr(&P.meta[kMetaOverflow], 1u);
+	}
+}
+
+// SetNode with its decisions and post-order indices in LDS: pass A bottom-up (decision + subtree
+// append counts), then one top-down pass that hands out the indices and writes every appended
+// node with its children's final pointers (a new child's index is its base + count - 1).
+__device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
+                                   u32 *__restrict__ roots, u64 node_cap, ColorLds &S) {
+	const u32 leafL = P.c.color_leaf;
+	for (int l = (int)leafL - 1; l >= 0; --l) { // pass A
+		const ALevel A = lv[l], C = lv[l + 1];
+		const u32 n = pm[kPmHdr + l], o = S.off[l], oc = S.off[l + 1];
+		const bool cl = (u32)l + 1 == leafL;
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			Entry E = A.ent[e];
+			u32 r = E.octree, c = 0;
+			if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
+				const u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
+				u32 ch[8];
+				bool all_null = true, same_color;
+				for (u32 i = 0; i < 8; ++i) {
+					if ((m >> i) & 1u) {
+						u32 ci = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
+						if (ci >= C.cap)
+							ch[i] = kColorNull;
+						else if (cl)
+							ch[i] = C.ores[ci];
+						else {
+							ch[i] = S.r[oc + ci];
+							c += S.cnt[oc + ci];
+						}
+					} else
+						ch[i] = A.okids[(u64)e * 8u + i];
+					all_null = all_null && ctag(ch[i]) == kTagNull;
+				}
+				same_color = ctag(ch[0]) == kTagColor;
+				for (u32 i = 1; i < 8 && same_color; ++i)
+					same_color = ch[i] == ch[0];
+				if (all_null)
+					r = kColorNull;
+				else if (same_color)
+					r = ch[0];
+				else {
+					bool unchanged = ctag(E.octree) == kTagNode;
+					for (u32 i = 0; i < 8 && unchanged; ++i)
+						unchanged = P.cnodes[(u64)cdata(E.octree) * 8u + i] == ch[i];
+					r = unchanged ? E.octree : kColorNewNode;
+				}
+				c += r == kColorNewNode ? 1u : 0u;
+			}
+			S.r[o + e] = r;
+			S.cnt[o + e] = c;
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0 && pm[kPmHdr] > 0)
+		S.base[S.off[0]] = P.meta[kMetaColorNodes];
+	__syncthreads();
+	for (u32 l = 0; l < leafL; ++l) { // top-down: indices + writes
+		const ALevel A = lv[l], C = lv[l + 1];
+		const u32 n = pm[kPmHdr + l], o = S.off[l], oc = S.off[l + 1];
+		const bool cl = l + 1 == leafL;
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			u32 run = S.base[o + e], r = S.r[o + e];
+			const u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
+			u32 ch[8];
+			for (u32 i = 0; i < 8; ++i) {
+				if ((m >> i) & 1u) {
+					u32 ci = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
+					if (ci >= C.cap)
+						ch[i] = kColorNull;
+					else if (cl)
+						ch[i] = C.ores[ci];
+					else {
+						u32 cr = S.r[oc + ci], cc = S.cnt[oc + ci];
+						S.base[oc + ci] = run;
+						run += cc;
+						if (cr == kColorNewNode) {
+							u32 idx = run - 1u;
+							cr = (u64)idx < node_cap ? cmake(kTagNode, idx) : C.ent[ci].octree;
+						}
+						ch[i] = cr;
+					}
+				} else
+					ch[i] = A.okids[(u64)e * 8u + i];
+			}
+			if (r == kColorNewNode) {
+				if ((u64)run < node_cap) {
+					for (u32 i = 0; i < 8; ++i)
+						P.cnodes[(u64)run * 8u + i] = ch[i];
+					r = cmake(kTagNode, run);
+				} else {
+					r = A.ent[e].octree;
+					atomicOr(&P.meta[kMetaOverflow], 2u);
+				}
+			}
+			if (l == 0 && e == 0) {
+				roots[1] = r;
+				P.meta[kMetaColorNodes] += S.cnt[o];
+			}
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		if (leafL == 0 && pm[kPmHdr] > 0)
+			roots[1] = lv[0].ores[0];
+		P.meta[kMetaColorLeaves] += pm[kPmLeafTotal];
+	}
+}
+
+// Single workgroup: the levels [0, l_from] (each at most kSmallLevel entries) bottom-up -- build,
+// dedup and walk with the bookkeeping in LDS -- the root result, the pass statistics and, for a
+// colored pass, SetNode.
 __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
-                                                int l_from, u32 *__restrict__ bcnt, u32 *__restrict__ head,
-                                                u32 *__restrict__ roots, u32 *__restrict__ acc, u64 node_cap) {
+                                                int l_from, GroupTable G, u32 *__restrict__ roots,
+                                                u32 *__restrict__ acc, u64 node_cap) {
+	__shared__ ALds U;
+	BottomLds &S = U.b;
 	const u32 L = P.c.L;
 	for (int l = l_from; l >= 0; --l) {
 		const ALevel A = lv[l];
-		const u32 n = pm[kPmHdr + l];
-		if ((u32)l + 1 < L) {
-			const ALevel C = lv[l + 1];
-			for (u32 e = threadIdx.x; e < n; e += blockDim.x)
-				build_entry(P, (u32)l, A, C, e, bcnt, head);
-			__syncthreads();
+		const u32 n = min(pm[kPmHdr + l], kSmallLevel);
+		const bool leaf = (u32)l + 1 == L;
+		u32 *out = S.res[l & 1];
+		const u32 *cres = S.res[(l + 1) & 1];
+		for (u32 i = threadIdx.x; i < kSTab; i += blockDim.x) {
+			S.tab[i] = kNull;
+			S.bkey[i] = kNull;
+			S.bmin[i] = kNull;
+			S.bhead[i] = kNull;
+			S.bn[i] = 0;
 		}
-		for (u32 e = threadIdx.x; e < n; e += blockDim.x)
-			walk_entry(P, (u32)l, A, e, bcnt, head, acc);
+		if (threadIdx.x == 0)
+			S.gcount = 0;
+		// build (or take a_leaf's result) + the find among the bucket's existing nodes
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			u32 fin = kNull, optr = kNull;
+			bool need = false;
+			if (leaf) {
+				if (!A.pend[e])
+					fin = A.status[e];
+				else if ((fin = A.res[e]) == kNull) {
+					need = true;
+					u32 c0 = A.content[2 * (u64)e], c1 = A.content[2 * (u64)e + 1], b = A.bucket[e];
+					S.content[e][0] = c0;
+					S.content[e][1] = c1;
+					S.csz[e] = 2;
+					S.hash[e] = murmur_leaf(c0, c1);
+					S.bucket[e] = b;
+					S.bw[e] = P.bucket_words[b];
+					optr = A.ent[e].ptr;
+				}
+			} else {
+				const ALevel C = lv[l + 1];
+				const bool csmall = l < l_from;
+				const Entry E = A.ent[e];
+				Built o = build_node(P, (u32)l, A, e, E, [&](u32 c) {
+					return csmall ? (c < kSmallLevel ? cres[c] : kNull) : child_result(C, G, c);
+				});
+				fin = o.fin;
+				if (o.k) {
+					const u32 bw = P.bucket_words[o.b];
+					fin = find_in_bucket(P.words, P.c, o.b, bw, o.c, o.k, false);
+					if (fin == kNull) {
+						need = true;
+						for (u32 j = 0; j < o.k; ++j)
+							S.content[e][j] = o.c[j];
+						S.csz[e] = o.k;
+						S.hash[e] = o.h;
+						S.bucket[e] = o.b;
+						S.bw[e] = bw;
+						optr = E.ptr;
+					}
+				}
+			}
+			out[e] = fin;
+			S.slot[e] = need ? 0u : kNull;
+			S.ptr[e] = optr;
+		}
+		__syncthreads();
+		// dedup in LDS: identical contents join the slot of their lowest entry
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			if (S.slot[e] == kNull)
+				continue;
+			const u32 h = S.hash[e];
+			u32 s = ((h ^ (h >> 15)) * 0x2c1b3c6du) & (kSTab - 1u);
+			for (;;) {
+				u32 cur = atomicCAS(&S.tab[s], kNull, e);
+				if (cur == kNull) {
+					S.glist[atomicAdd(&S.gcount, 1u)] = s;
+					break;
+				}
+				if (S.hash[cur] == h && lds_same(S, cur, e)) {
+					atomicMin(&S.tab[s], e);
+					break;
+				}
+				s = (s + 1u) & (kSTab - 1u);
+			}
+			S.slot[e] = s;
+		}
+		__syncthreads();
+		small_walk(P, (u32)l, S, acc);
+		__syncthreads();
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			u32 s = S.slot[e];
+			if (s != kNull)
+				out[e] = S.gfail[s] ? S.ptr[e] : S.gres[s];
+		}
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) {
 		if (pm[kPmHdr] > 0)
-			roots[0] = child_result(lv[0], 0);
+			roots[0] = l_from >= 0 ? S.res[0][0] : child_result(lv[0], G, 0);
 		u32 tot = 0;
 		for (u32 l = 0; l < L; ++l)
 			tot += pm[kPmHdr + l];
@@ -1601,10 +1979,23 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 			acc[kAccJobs] += pm[kPmHdr + P.c.color_leaf];
 		acc[kAccOverflow] |= pm[kPmOverflow];
 	}
-	if (P.stateful) {
-		__syncthreads();
-		setnode_block(P, lv, pm, roots, node_cap);
+	if (!P.stateful)
+		return;
+	__syncthreads(); // the LDS now holds SetNode's state
+	ColorLds &Cs = U.c;
+	if (threadIdx.x == 0) {
+		u32 o = 0;
+		for (u32 l = 0; l <= P.c.color_leaf; ++l) {
+			Cs.off[l] = o;
+			o += l < P.c.color_leaf ? pm[kPmHdr + l] : 0u;
+		}
+		Cs.fits = o <= kColorLds ? 1u : 0u;
 	}
+	__syncthreads();
+	if (Cs.fits)
+		setnode_lds(P, lv, pm, roots, node_cap, Cs);
+	else
+		setnode_block(P, lv, pm, roots, node_cap);
 }
 
 __global__ void a_set_roots(u32 *roots, u32 root, u32 croot, u32 *acc) {
@@ -1979,7 +2370,6 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 
 // ---------------------------------------------------------------- sync-free pass: host side
 constexpr u32 kTopPairs = 4096;  // a level whose 8 n pairs fit this bound expands inside a_top
-constexpr u32 kSmallLevel = 2048; // levels up to this many entries build + walk inside a_bottom
 constexpr u64 kAsyncBudget = 32ull << 30; // scratch bytes a sync-free pass may use
 
 // Upper bounds on the proceeding nodes of a pass, per level. A node proceeds only where some
@@ -2093,6 +2483,8 @@ struct AsyncBufs {
 	PushEvent *ev = nullptr;
 	Color *dense = nullptr;
 	hd_edit_desc *edits = nullptr;
+	GroupTable G{};
+	u64 tab_slots = 0;
 };
 constexpr u32 kMaxEmitBlocks = 1024;
 
@@ -2134,8 +2526,7 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		HD_RET(buf(p, "a_status", l, c, &A.status));
 		HD_RET(buf(p, "a_pend", l, c, &A.pend));
 		HD_RET(buf(p, "a_res", l, c, &A.res));
-		HD_RET(buf(p, "a_failed", l, c, &A.failed));
-		HD_RET(buf(p, "a_next", l, c, &A.next));
+		HD_RET(buf(p, "a_slot", l, c, &A.slot));
 		HD_RET(buf(p, "a_lpos", l, c, &A.lpos));
 		HD_RET(buf(p, "a_bucket", l, c, &A.bucket));
 		HD_RET(buf(p, "a_content", l, c * (l + 1 < L ? 9 : 2), &A.content));
@@ -2167,7 +2558,33 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 	}
 	HD_RET(p.scratch.get<hd_edit_desc>("a_edits", std::max<size_t>(flat.size(), 1), &B.edits));
 	HD_TRY(hipMemcpyAsync(B.edits, flat.data(), flat.size() * sizeof(hd_edit_desc), hipMemcpyHostToDevice, p.stream));
-	// per-bucket miss lists: empty between levels (every walker resets its bucket)
+	// group table: 2x the largest level; generations make clearing unnecessary except on growth
+	u64 maxcap = 1;
+	for (u32 l = 0; l < L; ++l)
+		maxcap = std::max(maxcap, cap[l]);
+	u64 T = 1024;
+	while (T < 2 * maxcap)
+		T <<= 1;
+	{
+		DevBuf &tb = p.scratch.bufs["a_tab"];
+		void *before = tb.p;
+		HD_RET(tb.ensure(T * 8));
+		B.G.tab = tb.as<u64>();
+		u64 slots = tb.bytes / 8; // ensure() may round up: use every slot it gave
+		while (T * 2 <= slots)
+			T <<= 1;
+		B.tab_slots = slots;
+		if (tb.p != before) {
+			a_fill_u32<<<std::min<u64>(div_up(2 * slots, 256), 4096), 256, 0, p.stream>>>((u32 *)tb.p, 2 * slots, 0u);
+			HD_TRY(hipGetLastError());
+			p.edit_gen = 0;
+		}
+	}
+	B.G.mask = (u32)(T - 1);
+	HD_RET(p.scratch.get<u32>("a_gres", T, &B.G.gres));
+	HD_RET(p.scratch.get<u32>("a_gfail", T, &B.G.gfail));
+	HD_RET(p.scratch.get<u32>("a_gnext", T, &B.G.gnext));
+	// per-bucket group lists: empty between levels (every walker resets its bucket)
 	const char *kb = "a_bcnt", *kh = "a_head";
 	bool fresh = p.scratch.bufs.find(kb) == p.scratch.bufs.end();
 	HD_RET(p.scratch.get<u32>(kb, p.total_buckets, &B.bcnt));
@@ -2226,15 +2643,22 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B) {
 			a_push<<<grid_for(pp.ev_cap, 1, 8192), 256, 0, p.stream>>>(B.ev, B.pm, (u32)pp.ev_cap, V,
 			                                                          B.lv[leafL].cap, B.dense);
 	}
-	a_leaf<<<grid_for(pp.cap[L - 1], kAT / 64, 16384), kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V, B.bcnt, B.head);
+	a_leaf<<<grid_for(pp.cap[L - 1], kAT / 64, 16384), kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V);
+	if (p.edit_gen > 0xFFFFFFFFu - 2 * L) { // generations exhausted: empty the table, restart
+		a_fill_u32<<<4096, 256, 0, p.stream>>>((u32 *)B.G.tab, 2 * B.tab_slots, 0u);
+		p.edit_gen = 0;
+	}
+	const u32 gen0 = p.edit_gen + 1;
+	p.edit_gen += L;
 	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
 	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= kSmallLevel)
 		++l_from;
 	for (int l = (int)L - 1; l > l_from; --l) {
 		u32 G = grid_for(pp.cap[l], kAT, 4096);
 		if (l + 1 < (int)L)
-			a_build<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.bcnt, B.head);
-		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.bcnt, B.head, B.acc);
+			a_build<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G);
+		a_dedup<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, gen0 + (u32)l, B.bcnt, B.head);
+		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.acc);
 	}
 	if (colors) {
 		u32 G = grid_for(pp.cap[leafL], 1, 8192);
@@ -2242,7 +2666,7 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B) {
 		a_scan_app<<<1, kAT, 0, p.stream>>>(leafL, B.pm, B.app, B.appo);
 		a_encode_write<<<G, kEncThreads, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap);
 	}
-	a_bottom<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, l_from, B.bcnt, B.head, B.roots, B.acc, p.cnode_cap);
+	a_bottom<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, l_from, B.G, B.roots, B.acc, p.cnode_cap);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
 }

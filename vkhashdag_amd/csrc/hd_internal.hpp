@@ -97,6 +97,7 @@ struct Pool {
 	u64 last_entries = 0, last_upserts = 0, last_jobs = 0;
 	u32 last_groups = 0;
 	double last_ms = 0;
+	u32 edit_gen = 0; // generation of the sync-free pass's group table (hd_edit.hip)
 };
 
 int make_filled(Pool &p);               // hd_pool.hip
