@@ -350,6 +350,64 @@ __device__ inline u32 find_in_bucket(const u32 *__restrict__ words, const DevCon
                                      const u32 *node, u32 n, bool leaf) {
 	return find_in_bucket_from(words, c, bucket, 0u, bw, node, n, leaf);
 }
+// find_in_bucket with the bucket's first 16 words fetched together with its word count, so a
+// bucket of up to 16 words costs one memory round trip. bw_out: the bucket's word count.
+__device__ inline u32 find_in_bucket_win(const u32 *__restrict__ words, const u32 *__restrict__ bucket_words,
+                                         const DevConfig &c, u32 bucket, const u32 *node, u32 n, bool leaf,
+                                         u32 &bw_out) {
+	const u64 base = (u64)bucket << c.wbpb();
+	u32 W[16];
+	if (c.wbpb() >= 4) {
+		const uint4 *q = reinterpret_cast<const uint4 *>(words + base);
+#pragma unroll
+		for (u32 k = 0; k < 4; ++k) {
+			uint4 t = q[k];
+			W[4 * k] = t.x;
+			W[4 * k + 1] = t.y;
+			W[4 * k + 2] = t.z;
+			W[4 * k + 3] = t.w;
+		}
+	} else {
+#pragma unroll
+		for (u32 k = 0; k < 16; ++k)
+			W[k] = words[base + k]; // a bucket this small sits inside the padded pool
+	}
+	const u32 bw = bucket_words[bucket];
+	bw_out = bw;
+	auto get = [&](u32 i) -> u32 {
+		if (i >= 16u)
+			return words[base + i];
+		u32 v = 0;
+#pragma unroll
+		for (u32 k = 0; k < 16; ++k)
+			v = k == i ? W[k] : v;
+		return v;
+	};
+	if (leaf) {
+		for (u32 i = 0; i + 2 <= bw; i += 2)
+			if (get(i) == node[0] && get(i + 1) == node[1])
+				return (u32)(base + i);
+		return kNull;
+	}
+	u32 i = 0;
+	while (i < bw) {
+		u32 w0 = get(i);
+		u32 sz = inner_words(w0);
+		if (sz == 0) {
+			i = ((i >> c.wbpp) + 1u) << c.wbpp;
+			continue;
+		}
+		if (sz == n && w0 == node[0]) {
+			bool eq = true;
+			for (u32 k = 1; k < n; ++k)
+				eq = eq && get(i + k) == node[k];
+			if (eq)
+				return (u32)(base + i);
+		}
+		i += sz;
+	}
+	return kNull;
+}
 // append_node (NodePool.hpp:134-157): full -> kNull; a node that would straddle a page zero-fills
 // the page tail and starts the next page. Updates bw.
 __device__ inline u32 append_to_bucket(u32 *__restrict__ words, const DevConfig &c, u32 bucket, u32 &bw,

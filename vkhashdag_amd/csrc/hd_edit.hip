@@ -775,6 +775,7 @@ __global__ void k_add_meta(u32 *meta, u32 which, const u32 *a, const u32 *b, u32
 // consecutive passes queue back to back.
 
 constexpr u32 kAT = 256;       // threads per block of the sync-free kernels
+constexpr u32 kTopPairs = 4096; // a level whose 8 n pairs fit this bound expands inside a_top
 constexpr u32 kGather = 32;    // bucket lists up to this length are sorted in registers
 
 struct Pair {
@@ -818,18 +819,28 @@ struct ALevel {
 struct PairOut {
 	u32 flag, lcnt;
 	uint4 cs;
+	u32 kid; // the child's new pointer if terminal, else its old one
 };
 
-// classification of one (node, child) pair: the body of k_expand
-__device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &A, u64 t, PushEvent *__restrict__ ev,
-                                      u32 *__restrict__ pm, u32 ev_cap) {
-	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
-	Entry E = A.ent[e];
+// classification of one (node, child) pair: the body of k_expand. The old node is read whole
+// (9 words, the pool is padded) so its mask and child arrive in one memory round trip.
+__device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel &A, u64 t, const Entry &E,
+                                        PushEvent *__restrict__ ev, u32 *__restrict__ pm, u32 ev_cap) {
+	const u32 i = (u32)(t & 7u);
 	u32 old = kNull;
 	if (E.ptr != kNull) {
-		u32 m = P.words[E.ptr];
-		if ((m >> i) & 1u)
-			old = P.words[E.ptr + 1u + (u32)__builtin_popcount(m & ((1u << i) - 1u))];
+		u32 w[9];
+#pragma unroll
+		for (u32 k = 0; k < 9; ++k)
+			w[k] = P.words[(u64)E.ptr + k];
+		const u32 m = w[0];
+		if ((m >> i) & 1u) {
+			const u32 r = (u32)__builtin_popcount(m & ((1u << i) - 1u));
+#pragma unroll
+			for (u32 k = 0; k < 8; ++k)
+				if (k == r)
+					old = w[1 + k];
+		}
 	}
 	const u32 cl = l + 1, leafL = P.c.color_leaf;
 	const u32 cx = (E.x << 1) | (i & 1u), cy = (E.y << 1) | ((i >> 1) & 1u), cz = (E.z << 1) | ((i >> 2) & 1u);
@@ -876,22 +887,27 @@ __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &
 	bool proceed = c.t == kProceed;
 	// bit 0: proceeds, bit 1: a terminal child whose pointer changed
 	u32 flag = (proceed ? 1u : 0u) | (!proceed && kid != old ? 2u : 0u);
-	return PairOut{flag, proceed && cnt > 1 ? cnt : 0u, make_uint4(c.octree, c.sf, single, cnt)};
+	return PairOut{flag, proceed && cnt > 1 ? cnt : 0u, make_uint4(c.octree, c.sf, single, cnt), kid};
+}
+__device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &A, u64 t, PushEvent *__restrict__ ev,
+                                      u32 *__restrict__ pm, u32 ev_cap) {
+	return expand_pair_e(P, l, A, t, A.ent[(u32)(t >> 3)], ev, pm, ev_cap);
 }
 
 // Writes the child entry of a proceeding pair (k_emit). m8: proceed flags of the pair's node in
 // bits 0-7 and its changed terminal children in bits 8-15 (valid on slot 0), idx / lo: the child's
 // entry index and list offset at level l + 1.
-__device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, const ALevel &N, u32 *__restrict__ pm,
-                                 u64 t, u32 m8, const PairOut &o, u32 idx, u32 lo) {
+__device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A, const ALevel &N,
+                                    u32 *__restrict__ pm, u64 t, const Entry &E, u32 kid, u32 m8, const PairOut &o,
+                                    u32 idx, u32 lo) {
 	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
 	if (i == 0) {
 		A.first_child[e] = idx;
 		A.pmask[e] = m8;
 	}
+	Entry out{};
 	if (!(o.flag & 1u))
-		return;
-	Entry E = A.ent[e];
+		return out;
 	u32 sf = o.cs.y;
 	u32 job = (sf & kSfNewJob) ? idx : ((sf & kSfWriter) ? E.job : kNull);
 	const u32 cx = (E.x << 1) | (i & 1u), cy = (E.y << 1) | ((i >> 1) & 1u), cz = (E.z << 1) | ((i >> 2) & 1u);
@@ -910,10 +926,19 @@ __device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, co
 		}
 	} else
 		lo = o.cs.z;
+	out = Entry{cx, cy, cz, kid, e, sf & ~kSfNewJob, o.cs.x, job, lo, ln};
 	if (idx < N.cap)
-		N.ent[idx] = Entry{cx, cy, cz, A.kids[t], e, sf & ~kSfNewJob, o.cs.x, job, lo, ln};
+		N.ent[idx] = out;
 	else
 		atomicOr(&pm[kPmOverflow], 1u);
+	return out;
+}
+__device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, const ALevel &N, u32 *__restrict__ pm,
+                                 u64 t, u32 m8, const PairOut &o, u32 idx, u32 lo) {
+	if (o.flag & 1u)
+		emit_pair_e(P, l, A, N, pm, t, A.ent[(u32)(t >> 3)], A.kids[t], m8, o, idx, lo);
+	else
+		emit_pair_e(P, l, A, N, pm, t, Entry{}, kNull, m8, o, idx, lo);
 }
 
 // chunk of the pair range [0, 8n) owned by this block (whole nodes per block)
@@ -1009,25 +1034,62 @@ __device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__
 }
 
 // Single workgroup: clears the pass counters, classifies the root and expands levels [0, l_end)
-// (every one small enough for one block).
+// (every one small enough for one block), with the current level's entries and count in LDS.
+constexpr u32 kTopEnt = kTopPairs / 8;
 __global__ void __launch_bounds__(kAT) a_top(EditParams P, const ALevel *__restrict__ lv, u32 *__restrict__ roots,
                                              u32 *__restrict__ pm, u32 pm_words, u32 l_end, PushEvent *ev, u32 ev_cap) {
 	__shared__ Pair sh[kAT];
-	__shared__ u32 sflag[kAT + 8];
+	__shared__ u32 sflag[kAT];
+	__shared__ Entry sent[2][kTopEnt];
+	__shared__ u32 scount;
 	for (u32 i = threadIdx.x; i < pm_words; i += kAT)
 		pm[i] = 0;
-	if (threadIdx.x < 8)
-		sflag[kAT + threadIdx.x] = 0;
 	__syncthreads();
-	if (threadIdx.x == 0)
+	if (threadIdx.x == 0) {
 		root_entry(P, lv[0], roots, pm);
+		scount = pm[kPmHdr];
+		if (scount)
+			sent[0][0] = lv[0].ent[0];
+	}
 	__syncthreads();
 	for (u32 l = 0; l < l_end; ++l) {
-		const u32 n = pm[kPmHdr + l];
-		Pair tot = expand_emit_range(P, l, lv[l], lv[l + 1], pm, ev, ev_cap, 0, (u64)n * 8u, Pair{0, 0}, nullptr,
-		                             nullptr, nullptr, sh, sflag);
-		if (threadIdx.x == 0)
-			set_level_count(P, l, lv[l + 1], pm, tot);
+		const ALevel A = lv[l], N = lv[l + 1];
+		const u32 n = min(scount, kTopEnt);
+		const Entry *cur = sent[l & 1];
+		Entry *nxt = sent[(l + 1) & 1];
+		Pair base{0, 0};
+		for (u64 t0 = 0; t0 < (u64)n * 8u; t0 += kAT) {
+			const u64 t = t0 + threadIdx.x;
+			const bool in = t < (u64)n * 8u;
+			PairOut o{0, 0, make_uint4(0, 0, 0, 0), kNull};
+			Entry E{};
+			if (in) {
+				E = cur[t >> 3];
+				o = expand_pair_e(P, l, A, t, E, ev, pm, ev_cap);
+			}
+			sflag[threadIdx.x] = o.flag;
+			Pair tile;
+			Pair off = block_excl_scan2(Pair{o.flag & 1u, o.lcnt}, sh, tile); // synchronizes sflag
+			if (in) {
+				u32 m8 = 0;
+				if ((t & 7u) == 0)
+					for (u32 k = 0; k < 8; ++k) {
+						u32 f = sflag[threadIdx.x + k];
+						m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k));
+					}
+				const u32 idx = base.a + off.a;
+				Entry ne = emit_pair_e(P, l, A, N, pm, t, E, o.kid, m8, o, idx, base.b + off.b);
+				if ((o.flag & 1u) && idx < kTopEnt)
+					nxt[idx] = ne;
+			}
+			base.a += tile.a;
+			base.b += tile.b;
+			__syncthreads(); // sflag reuse
+		}
+		if (threadIdx.x == 0) {
+			set_level_count(P, l, N, pm, base);
+			scount = min(base.a, N.cap);
+		}
 		__syncthreads();
 	}
 }
@@ -1106,12 +1168,24 @@ __global__ void a_push(const PushEvent *__restrict__ ev, const u32 *__restrict__
 }
 
 // ---- leaves + bottom-up DAG
-__device__ inline u32 wave_find_leaf(const u32 *__restrict__ W, const DevConfig &c, u32 bucket, u32 bw, u32 n0, u32 n1,
-                                     u32 lane) {
+// The first 128 words are fetched together with the word count (the pool is padded past its end).
+__device__ inline u32 wave_find_leaf(const u32 *__restrict__ W, const u32 *__restrict__ bucket_words,
+                                     const DevConfig &c, u32 bucket, u32 n0, u32 n1, u32 lane) {
 	const u64 base = (u64)bucket << c.wbpb();
+	const u32 wpb = c.words_per_bucket();
+	u32 x0 = 0, x1 = 0;
+	if (2u * lane + 2u <= wpb) {
+		x0 = W[base + 2u * lane];
+		x1 = W[base + 2u * lane + 1u];
+	}
+	const u32 bw = bucket_words[bucket];
 	for (u32 i0 = 0; i0 < bw; i0 += 128u) {
 		u32 j = i0 + 2u * lane;
-		bool m = j + 2u <= bw && W[base + j] == n0 && W[base + j + 1u] == n1;
+		if (i0) {
+			x0 = j + 2u <= bw ? W[base + j] : 0u;
+			x1 = j + 2u <= bw ? W[base + j + 1u] : 0u;
+		}
+		bool m = j + 2u <= bw && x0 == n0 && x1 == n1;
 		unsigned long long b = __ballot(m);
 		if (b)
 			return (u32)(base + i0 + 2u * (u32)__builtin_ctzll(b));
@@ -1195,7 +1269,7 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		}
 		u32 h = murmur_leaf(n0, n1);
 		u32 b = P.c.level_base[level] + (h & ((1u << P.c.bucket_bits[level]) - 1u));
-		u32 f = wave_find_leaf(P.words, P.c, b, P.bucket_words[b], n0, n1, lane);
+		u32 f = wave_find_leaf(P.words, P.bucket_words, P.c, b, n0, n1, lane);
 		if (lane == 0) {
 			A.pend[e] = 1;
 			A.content[2 * (u64)e] = n0;
@@ -1268,7 +1342,8 @@ __device__ inline void build_entry(const EditParams &P, u32 l, const ALevel &A, 
 	A.pend[e] = 1;
 	A.csize[e] = o.k;
 	A.bucket[e] = o.b;
-	A.res[e] = find_in_bucket(P.words, P.c, o.b, P.bucket_words[o.b], o.c, o.k, false);
+	u32 bw;
+	A.res[e] = find_in_bucket_win(P.words, P.bucket_words, P.c, o.b, o.c, o.k, false, bw);
 }
 
 // Joins entry e (pending, not in the pool) to its content's group; the creator queues the group.
@@ -1403,6 +1478,57 @@ __global__ void __launch_bounds__(kAT) a_walk(EditParams P, u32 l, const ALevel 
 }
 
 // ---- colors: encode + SetLeaf (one block per color leaf job, grid-stride)
+// Register path of the encoder for chunks of up to kEncPer * kEncThreads voxels: each thread
+// loads its contiguous voxels with wide loads once (one memory round trip instead of one per voxel).
+constexpr u32 kEncPer = 16;
+struct EncRegs {
+	u32 c[kEncPer], w[kEncPer]; // colors and weight words of voxels v0 .. v0 + n - 1
+	u32 v0, n, heads;           // heads: bit k set if voxel v0 + k starts a header
+	EncScan x;
+};
+__device__ inline void enc_load(const Color *__restrict__ d, u32 V, EncRegs &R) {
+	const u32 per = (V + kEncThreads - 1) / kEncThreads;
+	R.v0 = threadIdx.x * per;
+	const u32 v1 = min(V, R.v0 + per);
+	R.n = R.v0 < v1 ? v1 - R.v0 : 0u;
+	Color prev{0, 0};
+	if (R.n && R.v0)
+		prev = d[R.v0 - 1];
+	if (R.n == kEncPer) {
+		const uint4 *q = reinterpret_cast<const uint4 *>(d + R.v0);
+#pragma unroll
+		for (u32 k = 0; k < kEncPer / 2; ++k) {
+			uint4 t = q[k];
+			R.c[2 * k] = t.x;
+			R.w[2 * k] = t.y;
+			R.c[2 * k + 1] = t.z;
+			R.w[2 * k + 1] = t.w;
+		}
+	} else {
+#pragma unroll
+		for (u32 k = 0; k < kEncPer; ++k)
+			if (k < R.n) {
+				Color t = d[R.v0 + k];
+				R.c[k] = t.colors;
+				R.w[k] = t.wb;
+			}
+	}
+	R.heads = 0;
+	R.x = EncScan{0, 0};
+	u32 pc = prev.colors, pb = (prev.wb >> 8) & 0xFFu;
+#pragma unroll
+	for (u32 k = 0; k < kEncPer; ++k)
+		if (k < R.n) {
+			const u32 v = R.v0 + k, b = (R.w[k] >> 8) & 0xFFu;
+			if (v == 0 || (v & 0x3FFFu) == 0 || R.c[k] != pc || b != pb)
+				R.heads |= 1u << k;
+			R.x.bits += b;
+			pc = R.c[k];
+			pb = b;
+		}
+	R.x.heads = (u32)__builtin_popcount(R.heads);
+}
+
 __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const ALevel *__restrict__ lv,
                                                               const u32 *__restrict__ pm, const Color *__restrict__ dense,
                                                               u32 V, u32 *__restrict__ info, u32 *__restrict__ app) {
@@ -1418,11 +1544,17 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const
 			continue;
 		}
 		const Color *d = dense + (u64)e * V;
-		u32 per = (V + kEncThreads - 1) / kEncThreads, v0 = threadIdx.x * per, v1 = min(V, v0 + per);
 		EncScan x{0, 0};
-		for (u32 v = v0; v < v1; ++v) {
-			x.heads += is_head(d, v) ? 1u : 0u;
-			x.bits += (d[v].wb >> 8) & 0xFFu;
+		if (V <= kEncPer * kEncThreads) {
+			EncRegs R;
+			enc_load(d, V, R);
+			x = R.x;
+		} else {
+			u32 per = (V + kEncThreads - 1) / kEncThreads, v0 = threadIdx.x * per, v1 = min(V, v0 + per);
+			for (u32 v = v0; v < v1; ++v) {
+				x.heads += is_head(d, v) ? 1u : 0u;
+				x.bits += (d[v].wb >> 8) & 0xFFu;
+			}
 		}
 		EncScan tot;
 		block_excl_scan(x, sh, tot);
@@ -1480,7 +1612,48 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_write(EditParams P, cons
 		if (E.sf & kSfWriter) {
 			u32 ds = info[4 * e], as = info[4 * e + 1], inplace = info[4 * e + 2], B = info[4 * e + 3];
 			u32 idx = inplace ? cdata(E.octree) : leaf_base + appo[e];
-			if (inplace || (u64)idx + as <= leaf_cap) {
+			if ((inplace || (u64)idx + as <= leaf_cap) && V <= kEncPer * kEncThreads) {
+				u32 *L = P.cleaves;
+				EncRegs R;
+				enc_load(dense + (u64)e * V, V, R);
+				EncScan tot;
+				EncScan run = block_excl_scan(R.x, sh, tot);
+				const u32 W = (tot.bits + 31u) >> 5;
+				const u32 mo = idx + 4u, bo = mo + 2u, wo = bo + 2u * B; // one macro block (V <= 2^14)
+				for (u32 w = threadIdx.x; w < W; w += kEncThreads)
+					L[wo + w] = 0u;
+				if (threadIdx.x == 0) {
+					L[mo] = 0u;
+					L[mo + 1u] = 0u;
+					L[idx + 1u] = 1u;
+					L[idx + 2u] = B;
+					L[idx + 3u] = W;
+					if (!inplace)
+						L[idx] = as;
+				}
+				__syncthreads();
+#pragma unroll
+				for (u32 k = 0; k < kEncPer; ++k)
+					if (k < R.n) {
+						const u32 v = R.v0 + k, bpw = (R.w[k] >> 8) & 0xFFu;
+						if ((R.heads >> k) & 1u) {
+							L[bo + 2u * run.heads] = R.c[k];
+							L[bo + 2u * run.heads + 1u] = (v << 18) | (bpw << 16) | run.bits;
+							run.heads += 1u;
+						}
+						if (bpw) {
+							u32 w = R.w[k] & ((1u << bpw) - 1u), o = run.bits & 31u;
+							atomicOr(&L[wo + (run.bits >> 5)], w << o);
+							if (o + bpw > 32u)
+								atomicOr(&L[wo + (run.bits >> 5) + 1u], w >> (32u - o));
+						}
+						run.bits += bpw;
+					}
+				if (!inplace)
+					for (u32 w = ds + threadIdx.x; w < as; w += kEncThreads)
+						L[idx + w] = 0u;
+				result = inplace ? E.octree : cmake(kTagLeaf, idx);
+			} else if (inplace || (u64)idx + as <= leaf_cap) {
 				u32 *L = P.cleaves;
 				const Color *d = dense + (u64)e * V;
 				u32 M = (V + 0x3FFFu) >> 14;
@@ -1919,8 +2092,8 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 				});
 				fin = o.fin;
 				if (o.k) {
-					const u32 bw = P.bucket_words[o.b];
-					fin = find_in_bucket(P.words, P.c, o.b, bw, o.c, o.k, false);
+					u32 bw;
+					fin = find_in_bucket_win(P.words, P.bucket_words, P.c, o.b, o.c, o.k, false, bw);
 					if (fin == kNull) {
 						need = true;
 						for (u32 j = 0; j < o.k; ++j)
@@ -2369,7 +2542,6 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 }
 
 // ---------------------------------------------------------------- sync-free pass: host side
-constexpr u32 kTopPairs = 4096;  // a level whose 8 n pairs fit this bound expands inside a_top
 constexpr u64 kAsyncBudget = 32ull << 30; // scratch bytes a sync-free pass may use
 
 // Upper bounds on the proceeding nodes of a pass, per level. A node proceeds only where some

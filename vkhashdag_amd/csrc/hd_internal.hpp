@@ -77,6 +77,10 @@ enum Meta : u32 {
 	kMetaCount = 8
 };
 
+// Zero words allocated past the pool so a node can be read whole (up to 9 words) without a
+// bounds test, wherever in the pool it starts.
+constexpr u64 kWordsPad = 16;
+
 struct Pool {
 	hd_config cfg{};
 	DevConfig dc{};

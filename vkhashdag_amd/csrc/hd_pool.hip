@@ -173,7 +173,7 @@ int hd_pool_create(const hd_config *cfg, hd_pool **out) {
 	hipError_t e;
 	if ((e = hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking)) != hipSuccess)
 		return fail(e);
-	if ((e = hipMalloc(&p.words, words * 4)) != hipSuccess)
+	if ((e = hipMalloc(&p.words, (words + kWordsPad) * 4)) != hipSuccess)
 		return fail(e);
 	if ((e = hipMalloc(&p.bucket_words, buckets * 4)) != hipSuccess)
 		return fail(e);
@@ -189,7 +189,7 @@ int hd_pool_create(const hd_config *cfg, hd_pool **out) {
 		return fail(e);
 	if ((e = hipMalloc(&p.cleaves, (p.cleaf_cap ? p.cleaf_cap : 1) * 4)) != hipSuccess)
 		return fail(e);
-	if ((e = hipMemsetAsync(p.words, 0, words * 4, p.stream)) != hipSuccess)
+	if ((e = hipMemsetAsync(p.words, 0, (words + kWordsPad) * 4, p.stream)) != hipSuccess)
 		return fail(e);
 	if ((e = hipMemsetAsync(p.bucket_words, 0, buckets * 4, p.stream)) != hipSuccess)
 		return fail(e);
