@@ -538,21 +538,31 @@ __device__ inline bool is_head(const Color *d, u32 v) {
 	return a.colors != b.colors || ((a.wb >> 8) & 0xFFu) != ((b.wb >> 8) & 0xFFu);
 }
 
+// Block-wide exclusive scan (shuffle scans per wavefront + one exchange of wavefront totals).
 __device__ inline EncScan block_excl_scan(EncScan x, EncScan *sh, EncScan &total) {
-	u32 t = threadIdx.x;
-	sh[t] = x;
-	__syncthreads();
-	for (u32 o = 1; o < kEncThreads; o <<= 1) {
-		EncScan y = t >= o ? sh[t - o] : EncScan{0, 0};
-		__syncthreads();
-		sh[t].heads += y.heads;
-		sh[t].bits += y.bits;
-		__syncthreads();
+	const u32 lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	EncScan v = x;
+#pragma unroll
+	for (u32 o = 1; o < 64; o <<= 1) {
+		u32 a = __shfl_up(v.heads, o), b = __shfl_up(v.bits, o);
+		if (lane >= o)
+			v.heads += a, v.bits += b;
 	}
-	EncScan incl = sh[t];
-	total = sh[kEncThreads - 1];
+	if (lane == 63)
+		sh[w] = v;
 	__syncthreads();
-	return EncScan{incl.heads - x.heads, incl.bits - x.bits};
+	EncScan pre{0, 0}, tot{0, 0};
+#pragma unroll
+	for (u32 k = 0; k < kEncThreads / 64; ++k) {
+		EncScan s = sh[k];
+		if (k < w)
+			pre.heads += s.heads, pre.bits += s.bits;
+		tot.heads += s.heads;
+		tot.bits += s.bits;
+	}
+	__syncthreads();
+	total = tot;
+	return EncScan{pre.heads + v.heads - x.heads, pre.bits + v.bits - x.bits};
 }
 
 // Size pass: one block per color-leaf entry with a writer. info[e] = {data_size, append_size, inplace, B}
@@ -781,21 +791,32 @@ constexpr u32 kGather = 32;    // bucket lists up to this length are sorted in r
 struct Pair {
 	u32 a, b;
 };
+// Block-wide exclusive scan of kAT pairs: shuffle scans inside each wavefront, one LDS exchange
+// of the wavefront totals (two barriers). sh needs kAT / 64 entries.
 __device__ inline Pair block_excl_scan2(Pair x, Pair *sh, Pair &total) {
-	u32 t = threadIdx.x;
-	sh[t] = x;
-	__syncthreads();
-	for (u32 o = 1; o < kAT; o <<= 1) {
-		Pair y = t >= o ? sh[t - o] : Pair{0, 0};
-		__syncthreads();
-		sh[t].a += y.a;
-		sh[t].b += y.b;
-		__syncthreads();
+	const u32 lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	Pair v = x;
+#pragma unroll
+	for (u32 o = 1; o < 64; o <<= 1) {
+		u32 a = __shfl_up(v.a, o), b = __shfl_up(v.b, o);
+		if (lane >= o)
+			v.a += a, v.b += b;
 	}
-	Pair incl = sh[t];
-	total = sh[kAT - 1];
+	if (lane == 63)
+		sh[w] = v;
 	__syncthreads();
-	return Pair{incl.a - x.a, incl.b - x.b};
+	Pair pre{0, 0}, tot{0, 0};
+#pragma unroll
+	for (u32 k = 0; k < kAT / 64; ++k) {
+		Pair s = sh[k];
+		if (k < w)
+			pre.a += s.a, pre.b += s.b;
+		tot.a += s.a;
+		tot.b += s.b;
+	}
+	__syncthreads(); // sh reuse
+	total = tot;
+	return Pair{pre.a + v.a - x.a, pre.b + v.b - x.b};
 }
 
 // per-pass device counters: pm[kPmHdr + l] entries of level l, pm[kPmHdr + L + l] list words
@@ -1036,14 +1057,29 @@ __device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__
 // Single workgroup: clears the pass counters, classifies the root and expands levels [0, l_end)
 // (every one small enough for one block), with the current level's entries and count in LDS.
 constexpr u32 kTopEnt = kTopPairs / 8;
-__global__ void __launch_bounds__(kAT) a_top(EditParams P, const ALevel *__restrict__ lv, u32 *__restrict__ roots,
+constexpr u32 kTopEdits = 64; // a pass with at most this many edits reads them from LDS in a_top
+__global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__restrict__ lv, u32 *__restrict__ roots,
                                              u32 *__restrict__ pm, u32 pm_words, u32 l_end, PushEvent *ev, u32 ev_cap) {
 	__shared__ Pair sh[kAT];
 	__shared__ u32 sflag[kAT];
 	__shared__ Entry sent[2][kTopEnt];
 	__shared__ u32 scount;
+	__shared__ hd_edit_desc sedit[kTopEdits];
+	__shared__ u32 sfilled[HD_MAX_LEVELS];
 	for (u32 i = threadIdx.x; i < pm_words; i += kAT)
 		pm[i] = 0;
+	// the pass's edits and the filled chain are read by every classification: keep them in LDS
+	EditParams P = P0;
+	if (P.n_edits <= kTopEdits) {
+		const u32 *src = reinterpret_cast<const u32 *>(P.edits);
+		u32 *dst = reinterpret_cast<u32 *>(sedit);
+		for (u32 i = threadIdx.x; i < P.n_edits * (u32)(sizeof(hd_edit_desc) / 4); i += kAT)
+			dst[i] = src[i];
+		P.edits = sedit;
+	}
+	if (threadIdx.x < P.c.L)
+		sfilled[threadIdx.x] = P.filled[threadIdx.x];
+	P.filled = sfilled;
 	__syncthreads();
 	if (threadIdx.x == 0) {
 		root_entry(P, lv[0], roots, pm);
@@ -1827,13 +1863,15 @@ constexpr u32 kSTab = 2048;       // LDS dedup / bucket-map slots (load factor <
 constexpr u32 kColorLds = 4096;   // SetNode entries (all levels above the color leaves) kept in LDS
 
 struct BottomLds {
-	u32 res[2][kSmallLevel]; // final pointers of the current level / its child level
+	// per entry; [2] = this level / its child level, whose results the build resolves
+	u32 res[2][kSmallLevel];  // final pointer, or kNull while a group result is pending
+	u32 slot[2][kSmallLevel]; // dedup slot of an entry with new content, else kNull
+	u32 ptr[2][kSmallLevel];  // old pointer (kept when the bucket is full)
 	u32 content[kSmallLevel][9];
-	u32 csz[kSmallLevel], hash[kSmallLevel], bucket[kSmallLevel], bw[kSmallLevel], slot[kSmallLevel];
-	u32 ptr[kSmallLevel];
-	u32 glist[kSmallLevel], gnext[kSmallLevel], gmap[kSmallLevel]; // groups; bucket-map slot of each
-	u32 tab[kSTab], gres[kSTab];                                   // dedup table; result per slot
-	u32 bkey[kSTab], bmin[kSTab], bhead[kSTab], bn[kSTab];         // bucket map: walker election
+	u32 csz[kSmallLevel], hash[kSmallLevel], bucket[kSmallLevel], bw[kSmallLevel], bmap[kSmallLevel];
+	u32 glist[kSmallLevel], gnext[kSmallLevel]; // groups (their slots) and the per-bucket group lists
+	u32 tab[kSTab], gres[kSTab];                // dedup table (lowest entry per content); result per slot
+	u32 bkey[kSTab], bmin[kSTab], bhead[kSTab], bn[kSTab]; // bucket map: lowest entry, groups
 	unsigned char gfail[kSTab];
 	u32 gcount;
 };
@@ -1855,83 +1893,68 @@ __device__ inline bool lds_same(const BottomLds &S, u32 a, u32 b) {
 			return false;
 	return true;
 }
+__device__ inline u32 lds_result(const BottomLds &S, u32 p, u32 c) {
+	const u32 s = S.slot[p][c];
+	return s == kNull ? S.res[p][c] : (S.gfail[s] ? S.ptr[p][c] : S.gres[s]);
+}
 
-// Walk of a small level: groups of one bucket elect the one with the lowest first entry, which
-// replays them in that order (find among this walk's appends, else append_node).
-__device__ inline void small_walk(const EditParams &P, u32 l, BottomLds &S, u32 *__restrict__ acc) {
-	const bool leaf = l == P.c.L - 1;
-	const u32 ng = S.gcount;
-	for (u32 g = threadIdx.x; g < ng; g += blockDim.x) {
-		const u32 rep = S.tab[S.glist[g]], b = S.bucket[rep];
-		u32 m = (b * 0x9E3779B1u >> 7) & (kSTab - 1u);
-		for (;;) {
-			u32 cur = atomicCAS(&S.bkey[m], kNull, b);
-			if (cur == kNull || cur == b)
-				break;
-			m = (m + 1u) & (kSTab - 1u);
+// Walk of a small level's bucket by its lowest new entry (the first group's lowest entry): the
+// bucket's groups are replayed in the order of their lowest entries (find among this walk's
+// appends, else append_node).
+__device__ inline void small_walk_bucket(const EditParams &P, bool leaf, BottomLds &S, u32 e, u32 *__restrict__ acc) {
+	const u32 m = S.bmap[e], b = S.bucket[e], bw0 = S.bw[e], cnt = S.bn[m];
+	u32 bw = bw0, appended = 0;
+	bool any_fail = false;
+	auto upsert = [&](u32 g2) {
+		const u32 s2 = S.glist[g2], j = S.tab[s2];
+		const u32 *node = S.content[j];
+		u32 nsz = S.csz[j];
+		u32 f = bw > bw0 ? find_in_bucket_from(P.words, P.c, b, bw0, bw, node, nsz, leaf) : kNull;
+		unsigned char fail = 0;
+		if (f == kNull) {
+			f = append_to_bucket(P.words, P.c, b, bw, node, nsz);
+			if (f == kNull)
+				fail = 1, any_fail = true;
+			else
+				++appended;
 		}
-		atomicMin(&S.bmin[m], rep);
-		atomicAdd(&S.bn[m], 1u);
-		S.gnext[g] = atomicExch(&S.bhead[m], g);
-		S.gmap[g] = m;
-	}
-	__syncthreads();
-	for (u32 g = threadIdx.x; g < ng; g += blockDim.x) {
-		const u32 rep = S.tab[S.glist[g]], b = S.bucket[rep], m = S.gmap[g];
-		if (S.bmin[m] != rep)
-			continue;
-		const u32 bw0 = S.bw[rep], cnt = S.bn[m];
-		u32 bw = bw0, appended = 0;
-		bool any_fail = false;
-		auto upsert = [&](u32 g2) {
-			const u32 s2 = S.glist[g2], j = S.tab[s2];
-			const u32 *node = S.content[j];
-			u32 nsz = S.csz[j];
-			u32 f = bw > bw0 ? find_in_bucket_from(P.words, P.c, b, bw0, bw, node, nsz, leaf) : kNull;
-			unsigned char fail = 0;
-			if (f == kNull) {
-				f = append_to_bucket(P.words, P.c, b, bw, node, nsz);
-				if (f == kNull)
-					fail = 1, any_fail = true;
-				else
-					++appended;
+		S.gres[s2] = f;
+		S.gfail[s2] = fail;
+	};
+	if (cnt == 1)
+		upsert(S.bhead[m]);
+	else if (cnt <= kGather) {
+		u32 ord[kGather], key[kGather], n = 0;
+		for (u32 x = S.bhead[m]; x != kNull && n < kGather; x = S.gnext[x]) {
+			u32 r2 = S.tab[S.glist[x]], y = n++;
+			while (y > 0 && key[y - 1] > r2) {
+				key[y] = key[y - 1];
+				ord[y] = ord[y - 1];
+				--y;
 			}
-			S.gres[s2] = f;
-			S.gfail[s2] = fail;
-		};
-		if (cnt <= kGather) {
-			u32 ord[kGather], key[kGather], n = 0;
-			for (u32 x = S.bhead[m]; x != kNull && n < kGather; x = S.gnext[x]) {
-				u32 r2 = S.tab[S.glist[x]], y = n++;
-				while (y > 0 && key[y - 1] > r2) {
-					key[y] = key[y - 1];
-					ord[y] = ord[y - 1];
-					--y;
-				}
-				key[y] = r2;
-				ord[y] = x;
-			}
-			for (u32 x = 0; x < n; ++x)
-				upsert(ord[x]);
-		} else {
-			u32 last = 0;
-			for (u32 k = 0; k < cnt; ++k) {
-				u32 best = kNull, bg = kNull;
-				for (u32 x = S.bhead[m]; x != kNull; x = S.gnext[x]) {
-					u32 r2 = S.tab[S.glist[x]];
-					if ((k == 0 || r2 > last) && r2 < best)
-						best = r2, bg = x;
-				}
-				upsert(bg);
-				last = best;
-			}
+			key[y] = r2;
+			ord[y] = x;
 		}
-		P.bucket_words[b] = bw;
-		if (appended)
-			atomicAdd(&acc[kAccUpserts], appended);
-		if (any_fail)
-			atomicOr(&P.meta[kMetaOverflow], 1u);
+		for (u32 x = 0; x < n; ++x)
+			upsert(ord[x]);
+	} else {
+		u32 last = 0;
+		for (u32 k = 0; k < cnt; ++k) {
+			u32 best = kNull, bg = kNull;
+			for (u32 x = S.bhead[m]; x != kNull; x = S.gnext[x]) {
+				u32 r2 = S.tab[S.glist[x]];
+				if ((k == 0 || r2 > last) && r2 < best)
+					best = r2, bg = x;
+			}
+			upsert(bg);
+			last = best;
+		}
 	}
+	P.bucket_words[b] = bw;
+	if (appended)
+		atomicAdd(&acc[kAccUpserts], appended);
+	if (any_fail)
+		atomicOr(&P.meta[kMetaOverflow], 1u);
 }
 
 // SetNode with its decisions and post-order indices in LDS: pass A bottom-up (decision + subtree
@@ -2054,8 +2077,7 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 		const ALevel A = lv[l];
 		const u32 n = min(pm[kPmHdr + l], kSmallLevel);
 		const bool leaf = (u32)l + 1 == L;
-		u32 *out = S.res[l & 1];
-		const u32 *cres = S.res[(l + 1) & 1];
+		const u32 cur = l & 1, chl = (l + 1) & 1;
 		for (u32 i = threadIdx.x; i < kSTab; i += blockDim.x) {
 			S.tab[i] = kNull;
 			S.bkey[i] = kNull;
@@ -2088,7 +2110,7 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 				const bool csmall = l < l_from;
 				const Entry E = A.ent[e];
 				Built o = build_node(P, (u32)l, A, e, E, [&](u32 c) {
-					return csmall ? (c < kSmallLevel ? cres[c] : kNull) : child_result(C, G, c);
+					return csmall ? (c < kSmallLevel ? lds_result(S, chl, c) : kNull) : child_result(C, G, c);
 				});
 				fin = o.fin;
 				if (o.k) {
@@ -2106,44 +2128,53 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 					}
 				}
 			}
-			out[e] = fin;
-			S.slot[e] = need ? 0u : kNull;
-			S.ptr[e] = optr;
+			S.res[cur][e] = fin;
+			S.slot[cur][e] = need ? 0u : kNull;
+			S.ptr[cur][e] = optr;
 		}
 		__syncthreads();
-		// dedup in LDS: identical contents join the slot of their lowest entry
+		// dedup (identical contents join the slot of their lowest entry; a slot's creator lists the
+		// group on its bucket) and the bucket's walker election (its lowest new entry)
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			if (S.slot[e] == kNull)
+			if (S.slot[cur][e] == kNull)
 				continue;
-			const u32 h = S.hash[e];
+			const u32 h = S.hash[e], b = S.bucket[e];
+			u32 m = (b * 0x9E3779B1u >> 7) & (kSTab - 1u);
+			for (;;) {
+				u32 k = atomicCAS(&S.bkey[m], kNull, b);
+				if (k == kNull || k == b)
+					break;
+				m = (m + 1u) & (kSTab - 1u);
+			}
+			atomicMin(&S.bmin[m], e);
+			S.bmap[e] = m;
 			u32 s = ((h ^ (h >> 15)) * 0x2c1b3c6du) & (kSTab - 1u);
 			for (;;) {
-				u32 cur = atomicCAS(&S.tab[s], kNull, e);
-				if (cur == kNull) {
-					S.glist[atomicAdd(&S.gcount, 1u)] = s;
+				u32 c = atomicCAS(&S.tab[s], kNull, e);
+				if (c == kNull) {
+					u32 g = atomicAdd(&S.gcount, 1u);
+					S.glist[g] = s;
+					S.gnext[g] = atomicExch(&S.bhead[m], g);
+					atomicAdd(&S.bn[m], 1u);
 					break;
 				}
-				if (S.hash[cur] == h && lds_same(S, cur, e)) {
+				if (S.hash[c] == h && lds_same(S, c, e)) {
 					atomicMin(&S.tab[s], e);
 					break;
 				}
 				s = (s + 1u) & (kSTab - 1u);
 			}
-			S.slot[e] = s;
+			S.slot[cur][e] = s;
 		}
 		__syncthreads();
-		small_walk(P, (u32)l, S, acc);
-		__syncthreads();
-		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			u32 s = S.slot[e];
-			if (s != kNull)
-				out[e] = S.gfail[s] ? S.ptr[e] : S.gres[s];
-		}
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x)
+			if (S.slot[cur][e] != kNull && S.bmin[S.bmap[e]] == e)
+				small_walk_bucket(P, leaf, S, e, acc);
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) {
 		if (pm[kPmHdr] > 0)
-			roots[0] = l_from >= 0 ? S.res[0][0] : child_result(lv[0], G, 0);
+			roots[0] = l_from >= 0 ? lds_result(S, 0, 0) : child_result(lv[0], G, 0);
 		u32 tot = 0;
 		for (u32 l = 0; l < L; ++l)
 			tot += pm[kPmHdr + l];
