@@ -195,6 +195,57 @@ int hd_timer_end(hd_pool *pool, float *ms);
  * arrays o[3n], d[3n]; out: device float[4n] = hit position in [0,1]^3 + hit flag. */
 int hd_traversal(hd_pool *pool, uint32_t root, const float *o, const float *d, uint64_t n, float *out);
 
+/* ---- serving: a request loop over one device-resident DAG (replaces the reference app's loop of
+ * one edit in flight with later edits dropped, src/main.cpp:214-296 and :287). Requests, sorted by
+ * arrival on the server clock (ms from the call), are admitted per tick: edits into
+ * max_edits_per_tick slots while the node pool has at least min_free_pages free pages (else a GC
+ * runs; edits wait if it frees too little), renders up to max_pixels_per_tick pixels. A tick's
+ * edits run as one batched hd_edit while its renders run as one multi-view trace on a second stream
+ * against the DAG version of the tick's start; the next tick admits what arrived meanwhile.
+ * Concurrent rendering needs append-only colors: the pool must keep color history.
+ * HD_SERVE_PLAIN instead serves one request at a time in arrival order (the baseline loop). */
+enum { HD_REQ_EDIT = 0, HD_REQ_RENDER = 1 };
+enum { HD_SERVE_PLAIN = 1 };
+
+typedef struct hd_request {
+	uint32_t kind;     /* HD_REQ_* */
+	uint32_t client;
+	double arrival_ms; /* server clock */
+	hd_edit_desc edit; /* HD_REQ_EDIT */
+	hd_view view;      /* HD_REQ_RENDER (out_offset is assigned by the server) */
+} hd_request;
+
+typedef struct hd_request_result {
+	double admit_ms, done_ms; /* server clock */
+	uint64_t pixel_offset;    /* renders: first pixel of the image in rgba */
+	uint32_t tick;
+	uint32_t root, color_root; /* renders: the version rendered; edits: the roots after their tick */
+	uint32_t pad_;
+} hd_request_result;
+
+typedef struct hd_serve_config {
+	uint32_t max_edits_per_tick;
+	uint32_t flags; /* HD_SERVE_* */
+	uint64_t max_pixels_per_tick;
+	uint64_t min_free_pages;
+} hd_serve_config;
+
+typedef struct hd_serve_stats {
+	uint32_t ticks, edits, renders, gcs, gated_ticks, max_edits_in_tick, max_renders_in_tick, pad_;
+	uint64_t min_free_pages; /* fewest free pages seen at an edit admission */
+	double wall_ms;
+} hd_serve_stats;
+
+/* Serves n requests. rgba: device u32 buffer of rgba_pixels; when it holds every render's pixels each
+ * image stays at its pixel_offset, else images go to a tick scratch buffer and are not kept. The
+ * roots after the last tick are written to root_out / color_root_out. */
+int hd_serve_run(hd_pool *pool, const hd_serve_config *cfg, uint32_t root, uint32_t color_root,
+                 const hd_request *reqs, uint32_t n, uint32_t *rgba, uint64_t rgba_pixels,
+                 hd_request_result *results, uint32_t *root_out, uint32_t *color_root_out,
+                 hd_serve_stats *stats);
+/* Free node-pool pages (pages of 2^word_bits_per_page words; a bucket holds 2^page_bits_per_bucket). */
+int hd_pool_free_pages(hd_pool *pool, uint64_t *free_pages, uint64_t *total_pages);
+
 /* Device memory helpers for hosts without a GPU runtime of their own. */
 int hd_device_alloc(uint64_t bytes, void **out);
 int hd_device_free(void *ptr);

@@ -22,6 +22,21 @@ class HDError(RuntimeError):
     pass
 
 
+SERVE_PLAIN = 1
+
+
+class hd_serve_config(C.Structure):
+    _fields_ = [("max_edits_per_tick", C.c_uint32), ("flags", C.c_uint32), ("max_pixels_per_tick", C.c_uint64),
+                ("min_free_pages", C.c_uint64)]
+
+
+class hd_serve_stats(C.Structure):
+    _fields_ = [("ticks", C.c_uint32), ("edits", C.c_uint32), ("renders", C.c_uint32), ("gcs", C.c_uint32),
+                ("gated_ticks", C.c_uint32), ("max_edits_in_tick", C.c_uint32),
+                ("max_renders_in_tick", C.c_uint32), ("pad_", C.c_uint32), ("min_free_pages", C.c_uint64),
+                ("wall_ms", C.c_double)]
+
+
 class hd_config(C.Structure):
     _fields_ = [("word_bits_per_page", C.c_uint32), ("page_bits_per_bucket", C.c_uint32),
                 ("node_levels", C.c_uint32), ("bucket_bits", C.c_uint32 * HD_MAX_LEVELS),
@@ -87,6 +102,9 @@ def lib():
             "hd_memcpy_h2d": (C.c_int, [VP, VP, U64]),
             "hd_memcpy_d2h": (C.c_int, [VP, VP, U64]),
             "hd_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+            "hd_serve_run": (C.c_int, [VP, C.POINTER(hd_serve_config), U32, U32, VP, U32, VP, U64, VP, P32, P32,
+                                       C.POINTER(hd_serve_stats)]),
+            "hd_pool_free_pages": (C.c_int, [VP, C.POINTER(U64), C.POINTER(U64)]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -287,6 +305,30 @@ class DevicePool:
         return bout.download(np.float32, 4 * n).reshape(n, 4)
 
     Traversal = traversal
+
+    # -- serving: a request loop over this pool (hd_serve_run)
+    def free_pages(self):
+        f, t = U64(0), U64(0)
+        check(lib().hd_pool_free_pages(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def serve(self, requests, max_edits_per_tick=64, max_pixels_per_tick=1 << 24, min_free_pages=0, plain=False,
+              keep_images=False):
+        """Serves a request trace (workloads.serving_trace). Returns (results, stats, rgba buffer or
+        None); results follow workloads.RESULT_DTYPE."""
+        from .workloads import REQUEST_DTYPE, RESULT_DTYPE, REQ_RENDER
+        reqs = np.ascontiguousarray(requests, REQUEST_DTYPE)
+        res = np.zeros(len(reqs), RESULT_DTYPE)
+        cfg = hd_serve_config(max_edits_per_tick, SERVE_PLAIN if plain else 0, max_pixels_per_tick, min_free_pages)
+        px = int(sum(int(r["view"]["width"]) * int(r["view"]["height"]) for r in reqs if r["kind"] == REQ_RENDER))
+        buf = DeviceBuffer(max(1, px) * 4) if keep_images else None
+        st = hd_serve_stats()
+        r, cr = U32(0), U32(0)
+        check(lib().hd_serve_run(self.h, C.byref(cfg), self.root, self.color_root, reqs.ctypes.data, len(reqs),
+                                 buf.ptr if buf else None, px if buf else 0, res.ctypes.data, C.byref(r),
+                                 C.byref(cr), C.byref(st)))
+        self.root, self.color_root = r.value, cr.value
+        return res, st, buf
 
 
 def device_count() -> int:

@@ -378,6 +378,15 @@ __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ 
 	out[4 * i + 3] = h.hit ? 1.0f : 0.0f;
 }
 
+int trace_on_stream(Pool &p, hipStream_t s, u32 root, u32 color_root, const hd_view *d_views, const u32 *d_tiles,
+                    u32 n_views, u32 n_tiles, u32 *rgba) {
+	TraceArgs A{p.words, p.cnodes, p.cleaves, root,    color_root, p.dc.L, p.dc.color_leaf,
+	            d_views, d_tiles,  n_views,   rgba,    nullptr,    nullptr};
+	k_trace<false><<<n_tiles, kThreads, 0, s>>>(A);
+	HD_TRY(hipGetLastError());
+	return HD_OK;
+}
+
 } // namespace hd
 
 using namespace hd;

@@ -288,3 +288,52 @@ def orbit_view(center, radius, seed, width, height, fov=math.pi / 3, res=None, l
 def view_block(pos, look, side, up, proj_factor) -> np.ndarray:
     return np.concatenate([np.asarray(pos, np.float32), look, side, up,
                            np.array([proj_factor], np.float32)]).astype(np.float32)
+
+
+# ------------------------------------------------------------------ serving traces
+# hd_view / hd_request / hd_request_result (include/hd_capi.h)
+VIEW_DTYPE = np.dtype([("pos", "<f4", (3,)), ("look", "<f4", (3,)), ("side", "<f4", (3,)), ("up", "<f4", (3,)),
+                       ("proj_factor", "<f4"), ("width", "<u4"), ("height", "<u4"), ("flags", "<u4"),
+                       ("out_offset", "<u8")])
+assert VIEW_DTYPE.itemsize == 72
+REQUEST_DTYPE = np.dtype([("kind", "<u4"), ("client", "<u4"), ("arrival_ms", "<f8"), ("edit", EDIT_DTYPE),
+                          ("view", VIEW_DTYPE)])
+assert REQUEST_DTYPE.itemsize == 144
+RESULT_DTYPE = np.dtype([("admit_ms", "<f8"), ("done_ms", "<f8"), ("pixel_offset", "<u8"), ("tick", "<u4"),
+                         ("root", "<u4"), ("color_root", "<u4"), ("pad_", "<u4")])
+assert RESULT_DTYPE.itemsize == 40
+REQ_EDIT, REQ_RENDER = 0, 1
+
+
+def view_record(vb, width, height, flags) -> np.ndarray:
+    v = np.zeros((), VIEW_DTYPE)
+    vb = np.asarray(vb, np.float32)
+    v["pos"], v["look"], v["side"], v["up"], v["proj_factor"] = vb[0:3], vb[3:6], vb[6:9], vb[9:12], vb[12]
+    v["width"], v["height"], v["flags"] = width, height, flags
+    return v
+
+
+def serving_trace(n, res, heights, clients, rate_per_s, seed, rmin=2, rmax=64, edit_frac=0.7,
+                  px_min=(256, 256), px_max=(1920, 1080), shadow=True):
+    """Poisson arrivals at an aggregate rate from `clients` clients: edit_frac edits (40% sphere
+    add, 40% sphere remove, 20% box paint; radii / half-extents log-uniform in [rmin, rmax], centred
+    on the terrain surface) and renders at uniform-random resolutions in [px_min, px_max] from
+    random orbit cameras (1 shadow ray per hit). Sorted by arrival (ms)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1000.0 / rate_per_s, size=n)
+    t = np.cumsum(gaps)
+    out = np.zeros(n, REQUEST_DTYPE)
+    for i in range(n):
+        out[i]["arrival_ms"] = t[i]
+        out[i]["client"] = int(rng.integers(0, clients))
+        if rng.uniform() < edit_frac:
+            out[i]["kind"] = REQ_EDIT
+            out[i]["edit"] = mixed_batch(1, res, heights, rmin, rmax, int(rng.integers(1 << 31)))[0]
+        else:
+            out[i]["kind"] = REQ_RENDER
+            w = int(rng.integers(px_min[0], px_max[0] + 1))
+            h = int(rng.integers(px_min[1], px_max[1] + 1))
+            zc = float(np.median(heights)) if heights is not None else res / 2
+            vb = orbit_view((res / 2, res / 2, zc), res * 0.8, int(rng.integers(1 << 31)), w, h, res=res)
+            out[i]["view"] = view_record(vb, w, h, 1 if shadow else 0)
+    return out
