@@ -47,6 +47,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-edits", type=int, default=4096)
     ap.add_argument("--plain-prefix", type=int, default=512, help="edits of the batch also run plain, for the ratio")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=3000)
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="aggregate request arrivals per second")
+    ap.add_argument("--serve-overload", type=float, default=10.0, help="rate multiple of the overload run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed, not built",
@@ -139,6 +143,7 @@ def main():
     pool.close()
 
     batched = None if args.no_batched else batched_section(args, cfg, heights, res)
+    serving = None if args.no_serve else serving_section(args, cfg, heights, res)
 
     cpu = None
     if state0 is not None:
@@ -194,6 +199,7 @@ def main():
         "gc_ms": gc_ms,
         "gc_used_words": [int(used_before), int(used_after)],
         "batched": batched,
+        "serving": serving,
         "pool": {"used_words": int(st.used_words), "color_nodes": int(st.color_nodes),
                  "color_leaf_words": int(st.color_leaf_words), "overflow": int(st.overflow)},
     }
@@ -249,6 +255,62 @@ def batched_section(args, cfg, heights, res):
                plain_voxels_per_s=vox_prefix / tp, batched_prefix_voxels_per_s=vox_prefix / tbp,
                batched_vs_plain=tp / tbp)
     log(f"prefix {len(prefix)} edits: plain {tp * 1e3:.1f} ms, batched {tbp * 1e3:.1f} ms -> {tp / tbp:.1f}x")
+    return out
+
+
+def serving_section(args, cfg, heights, res):
+    """64 clients, Poisson arrivals at an aggregate rate; 70% edits (40% sphere add, 40% sphere
+    remove, 20% box paint, radii log-uniform in [2, 64] -- the [2, 512] range of a 32768^3 scene
+    scaled to this grid), 30% renders at uniform-random resolutions 256x256..1920x1080 from random
+    orbit cameras with 1 shadow ray per hit. The continuous-batching loop and the
+    one-request-at-a-time loop each serve the same trace from the same terrain."""
+    from dataclasses import replace
+
+    from vkhashdag_amd import capi
+    from vkhashdag_amd import workloads as W
+
+    scfg = replace(cfg, keep_history=True)  # renders overlap edits: colors must be append-only
+    out = {}
+    for run, rate in (("offered", args.serve_rate), ("overload", args.serve_rate * args.serve_overload)):
+        reqs = W.serving_trace(args.serve_requests, res, heights, clients=64, rate_per_s=rate, seed=4,
+                               rmin=2, rmax=64)
+        out[run] = serve_run(args, scfg, heights, res, reqs, rate)
+    return out
+
+
+def serve_run(args, scfg, heights, res, reqs, rate):
+    from vkhashdag_amd import capi
+    from vkhashdag_amd import workloads as W
+
+    edits = reqs[reqs["kind"] == W.REQ_EDIT]["edit"]
+    vox = W.edited_voxels(edits, res)
+    rend = reqs[reqs["kind"] == W.REQ_RENDER]["view"]
+    pixels = int(np.sum(rend["width"].astype(np.int64) * rend["height"]))
+    out = {"workload": f"{len(reqs)} requests from 64 clients, Poisson at {rate:.0f}/s aggregate: "
+                       f"{len(edits)} edits (r log-uniform 2..64) and {len(rend)} renders "
+                       f"(256x256..1920x1080, primary + shadow) on the {res}^3 terrain",
+           "edited_voxels": vox, "render_pixels": pixels}
+    for name, plain in (("continuous", False), ("plain", True)):
+        p = capi.DevicePool(scfg, heights=heights)
+        p.edit(W.terrain_edit(res, seed=1, skin=8))
+        p.sync()
+        res_, st, _ = p.serve(reqs, max_edits_per_tick=256, max_pixels_per_tick=1 << 23,
+                              min_free_pages=1 << 14, plain=plain)
+        lat = res_["done_ms"] - reqs["arrival_ms"]
+        le = lat[reqs["kind"] == W.REQ_EDIT]
+        lr = lat[reqs["kind"] == W.REQ_RENDER]
+        wall = st.wall_ms / 1e3
+        out[name] = {
+            "wall_s": wall, "requests_per_s": len(reqs) / wall, "edited_voxels_per_s": vox / wall,
+            "pixels_per_s": pixels / wall, "ticks": int(st.ticks), "gcs": int(st.gcs),
+            "max_edits_in_tick": int(st.max_edits_in_tick), "max_renders_in_tick": int(st.max_renders_in_tick),
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "edit_p50": float(np.percentile(le, 50)), "edit_p99": float(np.percentile(le, 99)),
+                           "render_p50": float(np.percentile(lr, 50)), "render_p99": float(np.percentile(lr, 99))},
+            "overflow": int(p.stats().overflow)}
+        log(f"serve {name} @ {rate:.0f}/s: {len(reqs)} requests in {wall:.2f}s, {st.ticks} ticks, latency p50 "
+            f"{out[name]['latency_ms']['p50']:.2f} ms p99 {out[name]['latency_ms']['p99']:.2f} ms")
+        p.close()
     return out
 
 
