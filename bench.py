@@ -48,6 +48,11 @@ def main():
     ap.add_argument("--batch-edits", type=int, default=4096)
     ap.add_argument("--plain-prefix", type=int, default=512, help="edits of the batch also run plain, for the ratio")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--trace-only", action="store_true",
+                    help="terrain + one edit batch, then --frames timed frames and a copy of --copy-mib MiB "
+                         "(the byte-count calibration of the HBM counters); for rocprofv3 runs")
+    ap.add_argument("--frames", type=int, default=3)
+    ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=3000)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="aggregate request arrivals per second")
     ap.add_argument("--serve-overload", type=float, default=10.0, help="rate multiple of the overload run")
@@ -99,6 +104,9 @@ def main():
         capi.check(capi.lib().hd_trace_async(pool.h, pool.root, pool.color_root, dview.ptr, dtiles.ptr, 1,
                                              int(tiles[1]), rgba.ptr, None, counters.ptr if count else None))
         return pool.timer_end()
+
+    if args.trace_only:
+        return trace_only(args, pool, batches[0], render, npx)
 
     edit_ms, trace_ms, step_ms = [], [], []
     for s in range(args.warmup + args.steps):
@@ -192,7 +200,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(args),
+            "traffic_source": TRAFFIC_PROFILE if profiled_traffic(args) else None,
             "algorithmic_bytes_per_launch": bytes_frame,
         },
         "cpu_baseline": cpu,
@@ -258,6 +267,19 @@ def batched_section(args, cfg, heights, res):
     return out
 
 
+def trace_only(args, pool, batch, render, npx):
+    """Profiling driver: the edited terrain, --frames frames of k_trace, then one device copy of
+    --copy-mib MiB (a kernel of known traffic: reads and writes exactly that many bytes)."""
+    from vkhashdag_amd import capi
+    pool.edit(batch)
+    ms = [render() for _ in range(args.frames)]
+    nbytes = args.copy_mib << 20
+    src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
+    capi.check(capi.lib().hd_device_copy(dst.ptr, src.ptr, nbytes, pool.h))
+    pool.sync()
+    print(json.dumps({"frames": args.frames, "trace_ms": ms, "pixels": npx, "copy_bytes": nbytes}))
+
+
 def serving_section(args, cfg, heights, res):
     """64 clients, Poisson arrivals at an aggregate rate; 70% edits (40% sphere add, 40% sphere
     remove, 20% box paint, radii log-uniform in [2, 64] -- the [2, 512] range of a 32768^3 scene
@@ -312,6 +334,23 @@ def serve_run(args, scfg, heights, res, reqs, rate):
             f"{out[name]['latency_ms']['p50']:.2f} ms p99 {out[name]['latency_ms']['p99']:.2f} ms")
         p.close()
     return out
+
+
+TRAFFIC_PROFILE = "profiles/k_trace_4096_terrain_1080p.md"
+
+
+def profiled_traffic(args):
+    """HBM bytes per k_trace launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled
+    by a copy of known size); only for the workload they were taken on."""
+    if (args.res, args.width, args.height) != (4096, 1920, 1080):
+        return None
+    try:
+        for line in open(os.path.join(ROOT, TRAFFIC_PROFILE)):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def C_ref(v):

@@ -102,6 +102,7 @@ def lib():
             "hd_memcpy_h2d": (C.c_int, [VP, VP, U64]),
             "hd_memcpy_d2h": (C.c_int, [VP, VP, U64]),
             "hd_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+            "hd_device_copy": (C.c_int, [VP, VP, U64, VP]),
             "hd_serve_run": (C.c_int, [VP, C.POINTER(hd_serve_config), U32, U32, VP, U32, VP, U64, VP, P32, P32,
                                        C.POINTER(hd_serve_stats)]),
             "hd_pool_free_pages": (C.c_int, [VP, C.POINTER(U64), C.POINTER(U64)]),

@@ -3,6 +3,8 @@
 // paged color storage of src/DAGColorPool.hpp:51-64 with flat HBM arrays.
 #include "hd_internal.hpp"
 
+#include <algorithm>
+
 #include <cstdarg>
 #include <mutex>
 
@@ -375,6 +377,28 @@ int hd_device_count(int *n) {
 }
 
 } // extern "C"
+
+namespace hd {
+// Plain 16-byte-vector copy: a kernel whose HBM traffic is known (bytes read = bytes written = n).
+__global__ void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, u64 n16) {
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (u64)gridDim.x * blockDim.x)
+		dst[i] = src[i];
+}
+} // namespace hd
+
+extern "C" int hd_device_copy(void *dst, const void *src, uint64_t bytes, hd_pool *pool) {
+	if (!dst || !src || (bytes & 15u) || ((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u))
+		return set_error("hd_device_copy: null or unaligned argument (16-byte multiples)"), HD_ERR_ARG;
+	hipStream_t s = pool ? pool->p.stream : nullptr;
+	const u64 n16 = bytes >> 4;
+	if (n16) {
+		u32 g = (u32)std::min<u64>(div_up(n16, 256), 8192);
+		k_copy16<<<g, 256, 0, s>>>((uint4 *)dst, (const uint4 *)src, n16);
+		HD_TRY(hipGetLastError());
+	}
+	HD_TRY(hipStreamSynchronize(s));
+	return HD_OK;
+}
 
 // Used words of every bucket packed in bucket order (the dirty-range Flush payload of
 // src/DAGNodePool.cpp:56-85 for a full flush). *n receives the count; out may be null to query it.
