@@ -195,6 +195,37 @@ int hd_timer_end(hd_pool *pool, float *ms);
  * arrays o[3n], d[3n]; out: device float[4n] = hit position in [0,1]^3 + hit flag. */
 int hd_traversal(hd_pool *pool, uint32_t root, const float *o, const float *d, uint64_t n, float *out);
 
+/* ---- frames captured as hipGraphs: up to max_passes passes of pairwise-disjoint edits (max_edits
+ * each, every edit within max_radius voxels: sphere radius or box half-extent) followed by one
+ * multi-view trace (n_views views up to max_width x max_height, pixels packed view after view) of
+ * the roots the passes leave on the device. The sequence is captured once at creation; each frame
+ * sets its edits and views (hd_frame_set) and replays the graph (hd_frame_launch). Edits are grouped
+ * as HD_MODE_BATCHED groups them, so a frame's DAG equals hd_edit(..., HD_MODE_BATCHED) word for
+ * word, and one edit per frame equals HD_MODE_PLAIN. The node-pool GC stays outside the graph:
+ * after hd_gc, hand the remapped roots back with hd_frame_set_roots. */
+enum { HD_FRAME_COLORS = 1 }; /* colored (VBR) edits allowed: the color passes are captured */
+
+typedef struct hd_frame_config {
+	uint32_t max_edits, max_passes, max_radius;
+	uint32_t n_views, max_width, max_height;
+	uint32_t flags; /* HD_FRAME_* */
+	uint32_t pad_;
+} hd_frame_config;
+
+typedef struct hd_frame hd_frame;
+int hd_frame_create(hd_pool *pool, const hd_frame_config *cfg, uint32_t root, uint32_t color_root,
+                    hd_frame **out);
+/* Edits of the next frame (grouped into passes; errors when they need more passes, more edits per
+ * pass or a larger radius than the frame was built for) and its views (out_offset is assigned). */
+int hd_frame_set(hd_frame *frame, const hd_edit_desc *edits, uint32_t n, const hd_view *views,
+                 uint32_t n_views);
+int hd_frame_set_roots(hd_frame *frame, uint32_t root, uint32_t color_root);
+int hd_frame_launch(hd_frame *frame); /* asynchronous, on the pool's stream */
+/* Waits for the launched frames; writes the roots after the last one. */
+int hd_frame_wait(hd_frame *frame, uint32_t *root_out, uint32_t *color_root_out);
+uint32_t *hd_frame_rgba(hd_frame *frame); /* device RGBA8 pixels of the last frame */
+int hd_frame_destroy(hd_frame *frame);
+
 /* ---- serving: a request loop over one device-resident DAG (replaces the reference app's loop of
  * one edit in flight with later edits dropped, src/main.cpp:214-296 and :287). Requests, sorted by
  * arrival on the server clock (ms from the call), are admitted per tick: edits into

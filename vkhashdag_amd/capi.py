@@ -30,6 +30,15 @@ class hd_serve_config(C.Structure):
                 ("min_free_pages", C.c_uint64)]
 
 
+FRAME_COLORS = 1
+
+
+class hd_frame_config(C.Structure):
+    _fields_ = [("max_edits", C.c_uint32), ("max_passes", C.c_uint32), ("max_radius", C.c_uint32),
+                ("n_views", C.c_uint32), ("max_width", C.c_uint32), ("max_height", C.c_uint32),
+                ("flags", C.c_uint32), ("pad_", C.c_uint32)]
+
+
 class hd_serve_stats(C.Structure):
     _fields_ = [("ticks", C.c_uint32), ("edits", C.c_uint32), ("renders", C.c_uint32), ("gcs", C.c_uint32),
                 ("gated_ticks", C.c_uint32), ("max_edits_in_tick", C.c_uint32),
@@ -103,6 +112,13 @@ def lib():
             "hd_memcpy_d2h": (C.c_int, [VP, VP, U64]),
             "hd_device_count": (C.c_int, [C.POINTER(C.c_int)]),
             "hd_device_copy": (C.c_int, [VP, VP, U64, VP]),
+            "hd_frame_create": (C.c_int, [VP, C.POINTER(hd_frame_config), U32, U32, C.POINTER(VP)]),
+            "hd_frame_set": (C.c_int, [VP, VP, U32, VP, U32]),
+            "hd_frame_set_roots": (C.c_int, [VP, U32, U32]),
+            "hd_frame_launch": (C.c_int, [VP]),
+            "hd_frame_wait": (C.c_int, [VP, P32, P32]),
+            "hd_frame_rgba": (VP, [VP]),
+            "hd_frame_destroy": (C.c_int, [VP]),
             "hd_serve_run": (C.c_int, [VP, C.POINTER(hd_serve_config), U32, U32, VP, U32, VP, U64, VP, P32, P32,
                                        C.POINTER(hd_serve_stats)]),
             "hd_pool_free_pages": (C.c_int, [VP, C.POINTER(U64), C.POINTER(U64)]),
@@ -330,6 +346,60 @@ class DevicePool:
                                  C.byref(cr), C.byref(st)))
         self.root, self.color_root = r.value, cr.value
         return res, st, buf
+
+
+class Frame:
+    """An edit -> render frame captured as a hipGraph (hd_frame_*): set edits + views, launch, wait."""
+
+    def __init__(self, pool, max_edits, max_passes, max_radius, n_views, max_width, max_height, colors=True):
+        self.pool = pool
+        cfg = hd_frame_config(max_edits, max_passes, max_radius, n_views, max_width, max_height,
+                              FRAME_COLORS if colors else 0, 0)
+        h = VP()
+        check(lib().hd_frame_create(pool.h, C.byref(cfg), pool.root, pool.color_root, C.byref(h)))
+        self.h = h.value
+        self.views = []
+
+    def set(self, edits, views):
+        """edits: EDIT_DTYPE records; views: list of (13-float view block, width, height, flags)."""
+        e = np.ascontiguousarray(np.atleast_1d(edits).astype(EDIT_DTYPE)) if len(edits) else np.zeros(0, EDIT_DTYPE)
+        vs = (hd_view * max(1, len(views)))()
+        for i, (vb, w, hh, fl) in enumerate(views):
+            vs[i] = make_view(vb, w, hh, fl)
+        self.views = [(w, hh) for (_, w, hh, _) in views]
+        check(lib().hd_frame_set(self.h, e.ctypes.data if len(e) else None, len(e), vs, len(views)))
+
+    def launch(self):
+        check(lib().hd_frame_launch(self.h))
+
+    def wait(self):
+        r, cr = U32(0), U32(0)
+        check(lib().hd_frame_wait(self.h, C.byref(r), C.byref(cr)))
+        self.pool.root, self.pool.color_root = r.value, cr.value
+        return r.value, cr.value
+
+    def set_roots(self, root, color_root):
+        check(lib().hd_frame_set_roots(self.h, root, color_root))
+
+    def images(self):
+        """RGBA8 pixels of the last frame, one array per view."""
+        n = sum(w * h for w, h in self.views)
+        out = np.empty(max(1, n), np.uint32)
+        ptr = lib().hd_frame_rgba(self.h)
+        check(lib().hd_memcpy_d2h(out.ctypes.data, ptr, out.nbytes))
+        res, off = [], 0
+        for w, h in self.views:
+            res.append(out[off:off + w * h])
+            off += w * h
+        return res
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hd_frame_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def device_count() -> int:

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 namespace hd {
 
@@ -820,7 +821,7 @@ __device__ inline Pair block_excl_scan2(Pair x, Pair *sh, Pair &total) {
 }
 
 // per-pass device counters: pm[kPmHdr + l] entries of level l, pm[kPmHdr + L + l] list words
-enum PassMeta : u32 { kPmOverflow = 0, kPmEvents = 1, kPmLeafTotal = 2, kPmHdr = 4 };
+enum PassMeta : u32 { kPmOverflow = 0, kPmEvents = 1, kPmLeafTotal = 2, kPmGen = 3, kPmHdr = 4 };
 // counters accumulated over the passes of one hd_edit call
 enum AccMeta : u32 { kAccEntries = 0, kAccUpserts = 1, kAccJobs = 2, kAccOverflow = 3, kAccCount = 4 };
 
@@ -1059,7 +1060,8 @@ __device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__
 constexpr u32 kTopEnt = kTopPairs / 8;
 constexpr u32 kTopEdits = 64; // a pass with at most this many edits reads them from LDS in a_top
 __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__restrict__ lv, u32 *__restrict__ roots,
-                                             u32 *__restrict__ pm, u32 pm_words, u32 l_end, PushEvent *ev, u32 ev_cap) {
+                                             u32 *__restrict__ pm, u32 pm_words, u32 l_end, PushEvent *ev, u32 ev_cap,
+                                             u32 *__restrict__ gen_ctr) {
 	__shared__ Pair sh[kAT];
 	__shared__ u32 sflag[kAT];
 	__shared__ Entry sent[2][kTopEnt];
@@ -1082,6 +1084,7 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 	P.filled = sfilled;
 	__syncthreads();
 	if (threadIdx.x == 0) {
+		pm[kPmGen] = atomicAdd(gen_ctr, P.c.L) + 1u; // this pass's dedup generations: pm[kPmGen] + level
 		root_entry(P, lv[0], roots, pm);
 		scount = pm[kPmHdr];
 		if (scount)
@@ -1496,10 +1499,10 @@ __global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel
 		build_entry(P, l, A, C, G, e);
 }
 __global__ void __launch_bounds__(kAT) a_dedup(EditParams P, u32 l, const ALevel *__restrict__ lv,
-                                               const u32 *__restrict__ pm, GroupTable G, u32 gen,
-                                               u32 *__restrict__ bcnt, u32 *__restrict__ head) {
+                                               const u32 *__restrict__ pm, GroupTable G, u32 *__restrict__ bcnt,
+                                               u32 *__restrict__ head) {
 	const ALevel A = lv[l];
-	const u32 n = pm[kPmHdr + l];
+	const u32 n = pm[kPmHdr + l], gen = pm[kPmGen] + l;
 	const bool leaf = l == P.c.L - 1;
 	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
 		dedup_entry(A, G, gen, leaf, e, bcnt, head);
@@ -2688,13 +2691,16 @@ struct AsyncBufs {
 	hd_edit_desc *edits = nullptr;
 	GroupTable G{};
 	u64 tab_slots = 0;
+	u32 *gen_ctr = nullptr;    // device: generations handed out so far (a_top takes L per pass)
+	u32 *gen_mirror = nullptr; // host mirror of *gen_ctr
 };
 constexpr u32 kMaxEmitBlocks = 1024;
 
 // Buffers for the largest bounds of the call's sync-free passes (allocated before any pass is
 // queued: a growing buffer would be freed under queued work).
 static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::vector<hd_edit_desc> &flat,
-                       AsyncBufs &B) {
+                       AsyncBufs &B, const std::string &pre = "a_", u32 *gen_mirror = nullptr) {
+	B.gen_mirror = gen_mirror ? gen_mirror : &p.edit_gen;
 	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
 	std::vector<u64> cap(L, 0), lcap(L, 0);
 	u64 ev = 0, maxpairs = 8;
@@ -2717,49 +2723,49 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		const u64 c = std::max<u64>(cap[l], 1);
 		A.cap = (u32)cap[l];
 		A.list_cap = (u32)lcap[l];
-		HD_RET(buf(p, "a_ent", l, c, &A.ent));
-		HD_RET(buf(p, "a_lists", l, std::max<u64>(lcap[l], 1), &A.lists));
+		HD_RET(buf(p, (pre + "ent").c_str(), l, c, &A.ent));
+		HD_RET(buf(p, (pre + "lists").c_str(), l, std::max<u64>(lcap[l], 1), &A.lists));
 		if (l + 1 < L) {
-			HD_RET(buf(p, "a_kids", l, 8 * c, &A.kids));
-			HD_RET(buf(p, "a_fchild", l, c, &A.first_child));
-			HD_RET(buf(p, "a_pmask", l, c, &A.pmask));
+			HD_RET(buf(p, (pre + "kids").c_str(), l, 8 * c, &A.kids));
+			HD_RET(buf(p, (pre + "fchild").c_str(), l, c, &A.first_child));
+			HD_RET(buf(p, (pre + "pmask").c_str(), l, c, &A.pmask));
 			if (stateful && l < leafL)
-				HD_RET(buf(p, "a_okids", l, 8 * c, &A.okids));
+				HD_RET(buf(p, (pre + "okids").c_str(), l, 8 * c, &A.okids));
 		}
-		HD_RET(buf(p, "a_status", l, c, &A.status));
-		HD_RET(buf(p, "a_pend", l, c, &A.pend));
-		HD_RET(buf(p, "a_res", l, c, &A.res));
-		HD_RET(buf(p, "a_slot", l, c, &A.slot));
-		HD_RET(buf(p, "a_lpos", l, c, &A.lpos));
-		HD_RET(buf(p, "a_bucket", l, c, &A.bucket));
-		HD_RET(buf(p, "a_content", l, c * (l + 1 < L ? 9 : 2), &A.content));
+		HD_RET(buf(p, (pre + "status").c_str(), l, c, &A.status));
+		HD_RET(buf(p, (pre + "pend").c_str(), l, c, &A.pend));
+		HD_RET(buf(p, (pre + "res").c_str(), l, c, &A.res));
+		HD_RET(buf(p, (pre + "slot").c_str(), l, c, &A.slot));
+		HD_RET(buf(p, (pre + "lpos").c_str(), l, c, &A.lpos));
+		HD_RET(buf(p, (pre + "bucket").c_str(), l, c, &A.bucket));
+		HD_RET(buf(p, (pre + "content").c_str(), l, c * (l + 1 < L ? 9 : 2), &A.content));
 		if (l + 1 < L)
-			HD_RET(buf(p, "a_csize", l, c, &A.csize));
+			HD_RET(buf(p, (pre + "csize").c_str(), l, c, &A.csize));
 		if (stateful && l <= leafL) {
-			HD_RET(buf(p, "a_ores", l, c, &A.ores));
-			HD_RET(buf(p, "a_ccnt", l, c, &A.ccnt));
-			HD_RET(buf(p, "a_cbase", l, c, &A.cbase));
-			HD_RET(buf(p, "a_own", l, c, &A.own));
+			HD_RET(buf(p, (pre + "ores").c_str(), l, c, &A.ores));
+			HD_RET(buf(p, (pre + "ccnt").c_str(), l, c, &A.ccnt));
+			HD_RET(buf(p, (pre + "cbase").c_str(), l, c, &A.cbase));
+			HD_RET(buf(p, (pre + "own").c_str(), l, c, &A.own));
 		}
 	}
-	HD_RET(p.scratch.get<ALevel>("a_levels", L, &B.d_lv));
+	HD_RET(p.scratch.get<ALevel>(pre + "levels", L, &B.d_lv));
 	HD_TRY(hipMemcpyAsync(B.d_lv, B.lv.data(), L * sizeof(ALevel), hipMemcpyHostToDevice, p.stream));
-	HD_RET(p.scratch.get<u32>("a_pm", kPmHdr + 2 * L, &B.pm));
-	HD_RET(p.scratch.get<u32>("a_roots", 2, &B.roots));
-	HD_RET(p.scratch.get<u32>("a_acc", kAccCount, &B.acc));
-	HD_RET(p.scratch.get<u32>("a_flag", maxpairs, &B.flag));
-	HD_RET(p.scratch.get<u32>("a_lcnt", maxpairs, &B.lcnt));
-	HD_RET(p.scratch.get<uint4>("a_cstate", maxpairs, &B.cstate));
-	HD_RET(p.scratch.get<Pair>("a_bsum", kMaxEmitBlocks, &B.bsum));
-	HD_RET(p.scratch.get<PushEvent>("a_events", std::max<u64>(ev, 1), &B.ev));
+	HD_RET(p.scratch.get<u32>(pre + "pm", kPmHdr + 2 * L, &B.pm));
+	HD_RET(p.scratch.get<u32>(pre + "roots", 2, &B.roots));
+	HD_RET(p.scratch.get<u32>(pre + "acc", kAccCount, &B.acc));
+	HD_RET(p.scratch.get<u32>(pre + "flag", maxpairs, &B.flag));
+	HD_RET(p.scratch.get<u32>(pre + "lcnt", maxpairs, &B.lcnt));
+	HD_RET(p.scratch.get<uint4>(pre + "cstate", maxpairs, &B.cstate));
+	HD_RET(p.scratch.get<Pair>(pre + "bsum", kMaxEmitBlocks, &B.bsum));
+	HD_RET(p.scratch.get<PushEvent>(pre + "events", std::max<u64>(ev, 1), &B.ev));
 	if (stateful) {
 		const u64 jobs = std::max<u64>(cap[leafL], 1);
-		HD_RET(p.scratch.get<Color>("a_dense", jobs << (3 * (p.dc.voxel_level - leafL)), &B.dense));
-		HD_RET(p.scratch.get<u32>("a_info", 4 * jobs, &B.info));
-		HD_RET(p.scratch.get<u32>("a_app", jobs, &B.app));
-		HD_RET(p.scratch.get<u32>("a_appo", jobs, &B.appo));
+		HD_RET(p.scratch.get<Color>(pre + "dense", jobs << (3 * (p.dc.voxel_level - leafL)), &B.dense));
+		HD_RET(p.scratch.get<u32>(pre + "info", 4 * jobs, &B.info));
+		HD_RET(p.scratch.get<u32>(pre + "app", jobs, &B.app));
+		HD_RET(p.scratch.get<u32>(pre + "appo", jobs, &B.appo));
 	}
-	HD_RET(p.scratch.get<hd_edit_desc>("a_edits", std::max<size_t>(flat.size(), 1), &B.edits));
+	HD_RET(p.scratch.get<hd_edit_desc>(pre + "edits", std::max<size_t>(flat.size(), 1), &B.edits));
 	HD_TRY(hipMemcpyAsync(B.edits, flat.data(), flat.size() * sizeof(hd_edit_desc), hipMemcpyHostToDevice, p.stream));
 	// group table: 2x the largest level; generations make clearing unnecessary except on growth
 	u64 maxcap = 1;
@@ -2769,7 +2775,7 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 	while (T < 2 * maxcap)
 		T <<= 1;
 	{
-		DevBuf &tb = p.scratch.bufs["a_tab"];
+		DevBuf &tb = p.scratch.bufs[pre + "tab"];
 		void *before = tb.p;
 		HD_RET(tb.ensure(T * 8));
 		B.G.tab = tb.as<u64>();
@@ -2777,16 +2783,18 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		while (T * 2 <= slots)
 			T <<= 1;
 		B.tab_slots = slots;
-		if (tb.p != before) {
+		HD_RET(p.scratch.get<u32>(pre + "gen", 1, &B.gen_ctr));
+		if (tb.p != before) { // a new table: empty, generations restart
 			a_fill_u32<<<std::min<u64>(div_up(2 * slots, 256), 4096), 256, 0, p.stream>>>((u32 *)tb.p, 2 * slots, 0u);
+			a_fill_u32<<<1, 64, 0, p.stream>>>(B.gen_ctr, 1, 0u);
 			HD_TRY(hipGetLastError());
-			p.edit_gen = 0;
+			*B.gen_mirror = 0;
 		}
 	}
 	B.G.mask = (u32)(T - 1);
-	HD_RET(p.scratch.get<u32>("a_gres", T, &B.G.gres));
-	HD_RET(p.scratch.get<u32>("a_gfail", T, &B.G.gfail));
-	HD_RET(p.scratch.get<u32>("a_gnext", T, &B.G.gnext));
+	HD_RET(p.scratch.get<u32>(pre + "gres", T, &B.G.gres));
+	HD_RET(p.scratch.get<u32>(pre + "gfail", T, &B.G.gfail));
+	HD_RET(p.scratch.get<u32>(pre + "gnext", T, &B.G.gnext));
 	// per-bucket group lists: empty between levels (every walker resets its bucket)
 	const char *kb = "a_bcnt", *kh = "a_head";
 	bool fresh = p.scratch.bufs.find(kb) == p.scratch.bufs.end();
@@ -2799,6 +2807,20 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		HD_TRY(hipGetLastError());
 	}
 	HD_TRY(hipStreamSynchronize(p.stream)); // the host staging arrays may go out of scope
+	return HD_OK;
+}
+
+// The device hands out L generations per pass (a_top); the host mirrors the count so that before
+// the 32-bit counter wraps the table is emptied and the counter restarted (queued, not captured).
+static int gen_reserve(Pool &p, AsyncBufs &B, u32 passes) {
+	const u64 need = (u64)passes * p.dc.L;
+	if ((u64)*B.gen_mirror + need > 0xFFFFFFF0ull) {
+		a_fill_u32<<<4096, 256, 0, p.stream>>>((u32 *)B.G.tab, 2 * B.tab_slots, 0u);
+		a_fill_u32<<<1, 64, 0, p.stream>>>(B.gen_ctr, 1, 0u);
+		HD_TRY(hipGetLastError());
+		*B.gen_mirror = 0;
+	}
+	*B.gen_mirror += (u32)need;
 	return HD_OK;
 }
 
@@ -2825,14 +2847,17 @@ static u32 grid_for(u64 items, u32 per_block, u32 max_blocks) {
 }
 
 // Queues one sync-free pass (no host synchronization).
-static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B) {
+static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = true) {
 	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
 	const EditParams P = pass_params(p, B.edits + pp.first, pp.count, pp.stateful);
 	const u32 V = pp.stateful ? 1u << (3u * P.leaf_bits) : 0u;
 	u32 top_end = 0;
 	while (top_end + 1 < L && 8 * pp.cap[top_end] <= kTopPairs)
 		++top_end;
-	a_top<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.roots, B.pm, kPmHdr + 2 * L, top_end, B.ev, (u32)pp.ev_cap);
+	if (reserve)
+		HD_RET(gen_reserve(p, B, 1));
+	a_top<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.roots, B.pm, kPmHdr + 2 * L, top_end, B.ev, (u32)pp.ev_cap,
+	                               B.gen_ctr);
 	for (u32 l = top_end; l + 1 < L; ++l) {
 		u32 G = grid_for(8 * pp.cap[l], 2048, kMaxEmitBlocks);
 		a_count<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.ev, (u32)pp.ev_cap, B.bsum);
@@ -2847,12 +2872,6 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B) {
 			                                                          B.lv[leafL].cap, B.dense);
 	}
 	a_leaf<<<grid_for(pp.cap[L - 1], kAT / 64, 16384), kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V);
-	if (p.edit_gen > 0xFFFFFFFFu - 2 * L) { // generations exhausted: empty the table, restart
-		a_fill_u32<<<4096, 256, 0, p.stream>>>((u32 *)B.G.tab, 2 * B.tab_slots, 0u);
-		p.edit_gen = 0;
-	}
-	const u32 gen0 = p.edit_gen + 1;
-	p.edit_gen += L;
 	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
 	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= kSmallLevel)
 		++l_from;
@@ -2860,7 +2879,7 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B) {
 		u32 G = grid_for(pp.cap[l], kAT, 4096);
 		if (l + 1 < (int)L)
 			a_build<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G);
-		a_dedup<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, gen0 + (u32)l, B.bcnt, B.head);
+		a_dedup<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head);
 		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.acc);
 	}
 	if (colors) {
@@ -2963,6 +2982,223 @@ int edit_plain(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u
 	return edit_sequence(p, root, croot, flat, plans, root_out, croot_out);
 }
 
+
+// ---------------------------------------------------------------- frames captured as hipGraphs
+// A frame = up to max_passes sync-free passes (groups of pairwise-disjoint edits, max_edits each,
+// every edit within the radius budget) followed by one multi-view trace of the new roots. The whole
+// sequence is captured once; each frame then writes its edits (padded with an empty box, which no
+// node classification can touch), views and tile prefix into the frame's device buffers and
+// replays the graph. Roots stay on the device between frames.
+struct FrameState {
+	Pool *p = nullptr;
+	hd_frame_config cfg{};
+	AsyncBufs B;
+	std::vector<PassPlan> plans;
+	std::vector<u64> budget; // per-level node bound of one edit
+	std::string pre;
+	u32 gen_mirror = 0;
+	// staging for the asynchronous uploads: two slots, each reused only after its copies completed
+	hd_edit_desc *h_edits[2] = {nullptr, nullptr};
+	hd_view *h_views[2] = {nullptr, nullptr};
+	u32 *h_tiles[2] = {nullptr, nullptr};
+	hipEvent_t copied[2] = {nullptr, nullptr};
+	u32 slot = 0;
+	u32 *h_roots = nullptr;
+	hd_view *d_views = nullptr;
+	u32 *d_tiles = nullptr, *d_rgba = nullptr;
+	u32 max_tiles = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	~FrameState() {
+		if (exec)
+			(void)hipGraphExecDestroy(exec);
+		if (graph)
+			(void)hipGraphDestroy(graph);
+		for (int k = 0; k < 2; ++k) {
+			for (void *h : {(void *)h_edits[k], (void *)h_views[k], (void *)h_tiles[k]})
+				if (h)
+					(void)hipHostFree(h);
+			if (copied[k])
+				(void)hipEventDestroy(copied[k]);
+		}
+		if (h_roots)
+			(void)hipHostFree(h_roots);
+		if (p) { // the frame's own buffers (its graph holds their addresses)
+			for (auto it = p->scratch.bufs.begin(); it != p->scratch.bufs.end();)
+				if (it->first.compare(0, pre.size(), pre) == 0) {
+					it->second.release();
+					it = p->scratch.bufs.erase(it);
+				} else
+					++it;
+		}
+	}
+};
+
+// Largest edit_level_bound of a sphere of radius <= R or a box of half-extent <= R, over every
+// position: a span of n voxels meets at most ceil((n - 1) / S) + 1 nodes of size S and holds at
+// least floor(n / S) - 1 of them whole; the sphere's shell bound does not depend on the position.
+static u64 budget_level_bound(u32 R, u32 voxel_level, u32 l) {
+	const u32 s = voxel_level - l;
+	const u64 S = 1ull << s, N = 1ull << l;
+	auto meet = [&](u64 n) { return std::min<u64>(N, (n + S - 2) / S + 1); };
+	auto whole = [&](u64 n) { return n / S >= 1 ? std::min<u64>(N, n / S - 1) : 0ull; };
+	const u64 nb = 2ull * R + 1; // box: [c - R, c + R]
+	u64 a = meet(nb), in = whole(nb);
+	const u64 box = a * a * a - in * in * in;
+	const u64 ns = 2ull * R + S + 1; // sphere: lb in [c - R - S, c + R]
+	const u64 as = meet(ns);
+	double rr = (double)R, tt = (double)S * std::sqrt(3.0);
+	double ro = rr + tt, ri = rr > tt ? rr - tt : 0.0;
+	double shell = 4.0 / 3.0 * M_PI * (ro * ro * ro - ri * ri * ri) / ((double)S * S * S);
+	const u64 sph = std::min<u64>(as * as * as, (u64)std::ceil(shell * (1.0 + 1e-9)) + 8u);
+	return std::max(box, sph);
+}
+
+static hd_edit_desc noop_edit() { // AABB [0,0)^3: every node has lb >= p1, so nothing is affected
+	hd_edit_desc d{};
+	d.kind = HD_EDIT_AABB_DIG;
+	return d;
+}
+
+static int frame_create(Pool &p, const hd_frame_config &cfg, u32 root, u32 croot, FrameState &F) {
+	static u32 frame_ids = 0;
+	const u32 L = p.dc.L;
+	if (!cfg.max_edits || !cfg.max_passes || !cfg.n_views || !cfg.max_width || !cfg.max_height)
+		return set_error("hd_frame_create: empty budget"), HD_ERR_ARG;
+	HD_RET(make_filled(p));
+	F.p = &p;
+	F.cfg = cfg;
+	F.pre = "f" + std::to_string(++frame_ids) + "_";
+	// per-edit budget: what a sphere of radius <= R or a box of half-extent <= R can reach at any
+	// position (worst alignment): frame_set holds every edit to it
+	F.budget.assign(L, 0);
+	for (u32 l = 1; l < L; ++l)
+		F.budget[l] = budget_level_bound(cfg.max_radius, p.dc.voxel_level, l);
+	// one plan per pass, as plan_pass would bound max_edits budget edits
+	std::vector<hd_edit_desc> flat((size_t)cfg.max_passes * cfg.max_edits, noop_edit());
+	const bool colors = (cfg.flags & HD_FRAME_COLORS) != 0;
+	for (u32 k = 0; k < cfg.max_passes; ++k) {
+		PassPlan pp;
+		pp.first = k * cfg.max_edits;
+		pp.count = cfg.max_edits;
+		pp.stateful = colors;
+		pp.async = true;
+		pp.cap.assign(L, 0);
+		pp.list_cap.assign(L, 0);
+		pp.cap[0] = 1;
+		pp.list_cap[0] = cfg.max_edits > 1 ? cfg.max_edits : 0;
+		const u64 lim = 1ull << 31;
+		for (u32 l = 1; l < L; ++l) {
+			u64 S = F.budget[l] * cfg.max_edits;
+			pp.cap[l] = std::min(std::min(S, 8 * pp.cap[l - 1]), lim);
+			pp.list_cap[l] = cfg.max_edits > 1 ? std::min(2 * S, lim) : 0;
+		}
+		pp.ev_cap = 0;
+		if (colors)
+			for (u32 l = p.dc.color_leaf; l + 1 < L; ++l)
+				pp.ev_cap += 8 * pp.cap[l];
+		pp.ev_cap = std::min<u64>(pp.ev_cap, lim);
+		F.plans.push_back(pp);
+	}
+	if (colors && p.dc.voxel_level - p.dc.color_leaf > 7)
+		return set_error("hd_frame_create: color leaves deeper than 8^7 voxels"), HD_ERR_UNSUPPORTED;
+	HD_RET(async_setup(p, F.plans, flat, F.B, F.pre, &F.gen_mirror));
+	// views, tiles, output
+	const u32 tiles1 = div_up(cfg.max_width, 16) * div_up(cfg.max_height, 16);
+	F.max_tiles = tiles1 * cfg.n_views;
+	HD_RET(p.scratch.get<hd_view>(F.pre + "views", cfg.n_views, &F.d_views));
+	HD_RET(p.scratch.get<u32>(F.pre + "tiles", cfg.n_views + 1, &F.d_tiles));
+	HD_RET(p.scratch.get<u32>(F.pre + "rgba", (u64)cfg.n_views * cfg.max_width * cfg.max_height, &F.d_rgba));
+	for (int k = 0; k < 2; ++k) {
+		HD_TRY(hipHostMalloc((void **)&F.h_edits[k], flat.size() * sizeof(hd_edit_desc), hipHostMallocDefault));
+		HD_TRY(hipHostMalloc((void **)&F.h_views[k], cfg.n_views * sizeof(hd_view), hipHostMallocDefault));
+		HD_TRY(hipHostMalloc((void **)&F.h_tiles[k], (cfg.n_views + 1) * 4, hipHostMallocDefault));
+		HD_TRY(hipEventCreateWithFlags(&F.copied[k], hipEventDisableTiming));
+		HD_TRY(hipEventRecord(F.copied[k], p.stream));
+	}
+	HD_TRY(hipHostMalloc((void **)&F.h_roots, 8 + 4 * kAccCount, hipHostMallocDefault));
+	std::memset(F.h_views[0], 0, cfg.n_views * sizeof(hd_view));
+	std::memset(F.h_tiles[0], 0, (cfg.n_views + 1) * 4);
+	HD_TRY(hipMemcpyAsync(F.d_views, F.h_views[0], cfg.n_views * sizeof(hd_view), hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipMemcpyAsync(F.d_tiles, F.h_tiles[0], (cfg.n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
+	a_set_roots<<<1, 64, 0, p.stream>>>(F.B.roots, root, croot, F.B.acc);
+	HD_TRY(hipGetLastError());
+	HD_TRY(hipStreamSynchronize(p.stream));
+	// capture: the passes, then the trace of the roots they leave on the device
+	HD_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
+	int rc = HD_OK;
+	for (u32 k = 0; k < cfg.max_passes && rc == HD_OK; ++k)
+		rc = queue_pass(p, F.plans[k], F.B, false);
+	if (rc == HD_OK)
+		rc = trace_on_stream(p, p.stream, 0, 0, F.d_views, F.d_tiles, cfg.n_views, F.max_tiles, F.d_rgba, F.B.roots);
+	hipError_t e = hipStreamEndCapture(p.stream, &F.graph);
+	HD_RET(rc);
+	HD_TRY(e);
+	HD_TRY(hipGraphInstantiate(&F.exec, F.graph, nullptr, nullptr, 0));
+	return HD_OK;
+}
+
+static int frame_set(FrameState &F, const hd_edit_desc *edits, u32 n, const hd_view *views, u32 n_views) {
+	Pool &p = *F.p;
+	const hd_frame_config &cfg = F.cfg;
+	const u32 L = p.dc.L;
+	if (n_views > cfg.n_views)
+		return set_error("hd_frame_set: %u views exceed the frame's %u", n_views, cfg.n_views), HD_ERR_ARG;
+	for (u32 k = 0; k < n; ++k) {
+		const hd_edit_desc &d = edits[k];
+		if (d.kind > HD_EDIT_AABB_PAINT)
+			return set_error("hd_frame_set: edit %u: column fills and unknown kinds are not frame edits", k), HD_ERR_ARG;
+		if (is_stateful(d) && !(cfg.flags & HD_FRAME_COLORS))
+			return set_error("hd_frame_set: edit %u is colored but the frame has no color passes", k), HD_ERR_ARG;
+		for (u32 l = 1; l < L; ++l)
+			if (edit_level_bound(d, p.dc.voxel_level, l) > F.budget[l])
+				return set_error("hd_frame_set: edit %u exceeds the frame's radius budget %u", k, cfg.max_radius),
+				       HD_ERR_CAPACITY;
+	}
+	std::vector<std::vector<u32>> groups;
+	if (n)
+		schedule_groups(edits, n, 1u << p.dc.voxel_level, groups);
+	if (groups.size() > cfg.max_passes)
+		return set_error("hd_frame_set: the edits need %zu passes, the frame has %u", groups.size(), cfg.max_passes),
+		       HD_ERR_CAPACITY;
+	const u32 k = F.slot;
+	F.slot ^= 1u;
+	HD_TRY(hipEventSynchronize(F.copied[k])); // this slot's previous uploads have been read
+	hd_edit_desc *he = F.h_edits[k];
+	hd_view *hv = F.h_views[k];
+	u32 *ht = F.h_tiles[k];
+	for (u64 i = 0; i < (u64)cfg.max_passes * cfg.max_edits; ++i)
+		he[i] = noop_edit();
+	for (size_t g = 0; g < groups.size(); ++g) {
+		if (groups[g].size() > cfg.max_edits)
+			return set_error("hd_frame_set: a pass needs %zu edits, the frame has %u", groups[g].size(), cfg.max_edits),
+			       HD_ERR_CAPACITY;
+		for (size_t j = 0; j < groups[g].size(); ++j)
+			he[g * cfg.max_edits + j] = edits[groups[g][j]];
+	}
+	u64 off = 0;
+	ht[0] = 0;
+	for (u32 v = 0; v < cfg.n_views; ++v) {
+		hd_view w{};
+		if (v < n_views) {
+			w = views[v];
+			if (!w.width || !w.height || w.width > cfg.max_width || w.height > cfg.max_height)
+				return set_error("hd_frame_set: view %u is empty or larger than %ux%u", v, cfg.max_width,
+				                 cfg.max_height),
+				       HD_ERR_ARG;
+			w.out_offset = off;
+			off += (u64)w.width * w.height;
+		}
+		hv[v] = w;
+		ht[v + 1] = ht[v] + (v < n_views ? div_up(w.width, 16) * div_up(w.height, 16) : 0u);
+	}
+	HD_TRY(hipMemcpyAsync(F.B.edits, he, (u64)cfg.max_passes * cfg.max_edits * sizeof(hd_edit_desc),
+	                      hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipMemcpyAsync(F.d_views, hv, cfg.n_views * sizeof(hd_view), hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipMemcpyAsync(F.d_tiles, ht, (cfg.n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipEventRecord(F.copied[k], p.stream));
+	return HD_OK;
+}
 } // namespace hd
 
 using namespace hd;
@@ -2992,5 +3228,77 @@ extern "C" int hd_edit(hd_pool *h, uint32_t root, uint32_t color_root, const hd_
 	p.last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	*root_out = r;
 	*color_root_out = c;
+	return HD_OK;
+}
+
+// ---- frames (hipGraph): the C-ABI
+struct hd_frame {
+	hd::FrameState F;
+};
+
+extern "C" int hd_frame_create(hd_pool *h, const hd_frame_config *cfg, uint32_t root, uint32_t color_root,
+                               hd_frame **out) {
+	if (!h || !cfg || !out)
+		return set_error("hd_frame_create: null argument"), HD_ERR_ARG;
+	hd_frame *f = new hd_frame;
+	int rc = frame_create(h->p, *cfg, root, color_root, f->F);
+	if (rc != HD_OK) {
+		delete f;
+		return rc;
+	}
+	*out = f;
+	return HD_OK;
+}
+
+extern "C" int hd_frame_set(hd_frame *f, const hd_edit_desc *edits, uint32_t n, const hd_view *views,
+                            uint32_t n_views) {
+	if (!f || (!edits && n) || (!views && n_views))
+		return set_error("hd_frame_set: null argument"), HD_ERR_ARG;
+	return frame_set(f->F, edits, n, views, n_views);
+}
+
+extern "C" int hd_frame_set_roots(hd_frame *f, uint32_t root, uint32_t color_root) {
+	if (!f)
+		return set_error("hd_frame_set_roots: null argument"), HD_ERR_ARG;
+	Pool &p = *f->F.p;
+	a_set_roots_keep<<<1, 64, 0, p.stream>>>(f->F.B.roots, root, color_root);
+	HD_TRY(hipGetLastError());
+	return HD_OK;
+}
+
+extern "C" int hd_frame_launch(hd_frame *f) {
+	if (!f)
+		return set_error("hd_frame_launch: null argument"), HD_ERR_ARG;
+	FrameState &F = f->F;
+	Pool &p = *F.p;
+	HD_RET(gen_reserve(p, F.B, F.cfg.max_passes)); // outside the graph: may queue a table reset
+	HD_TRY(hipGraphLaunch(F.exec, p.stream));
+	return HD_OK;
+}
+
+extern "C" int hd_frame_wait(hd_frame *f, uint32_t *root_out, uint32_t *color_root_out) {
+	if (!f)
+		return set_error("hd_frame_wait: null argument"), HD_ERR_ARG;
+	FrameState &F = f->F;
+	Pool &p = *F.p;
+	HD_TRY(hipMemcpyAsync(F.h_roots, F.B.roots, 8, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipMemcpyAsync(F.h_roots + 2, F.B.acc, 4 * kAccCount, hipMemcpyDeviceToHost, p.stream));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	if (F.h_roots[2 + kAccOverflow])
+		return set_error("hd_frame_wait: a level outgrew its bound (internal error)"), HD_ERR_CAPACITY;
+	if (root_out)
+		*root_out = F.h_roots[0];
+	if (color_root_out)
+		*color_root_out = F.h_roots[1];
+	return HD_OK;
+}
+
+extern "C" uint32_t *hd_frame_rgba(hd_frame *f) { return f ? f->F.d_rgba : nullptr; }
+
+extern "C" int hd_frame_destroy(hd_frame *f) {
+	if (f) {
+		(void)hipStreamSynchronize(f->F.p->stream);
+		delete f;
+	}
 	return HD_OK;
 }

@@ -106,8 +106,9 @@ struct Pool {
 
 int make_filled(Pool &p);               // hd_pool.hip
 // Multi-view trace of (root, color_root) queued on stream s; views and tile prefix on the device.
+// d_roots (optional): read the roots from device memory instead (graph replays).
 int trace_on_stream(Pool &p, hipStream_t s, u32 root, u32 color_root, const hd_view *d_views, const u32 *d_tiles,
-                    u32 n_views, u32 n_tiles, u32 *rgba); // hd_trace.hip
+                    u32 n_views, u32 n_tiles, u32 *rgba, const u32 *d_roots = nullptr); // hd_trace.hip
 int read_meta(Pool &p, u32 *host_meta); // hd_pool.hip
 
 inline u32 div_up(u64 a, u64 b) { return (u32)((a + b - 1) / b); }

@@ -291,14 +291,18 @@ struct TraceArgs {
 	u32 n_views;
 	u32 *rgba, *records;
 	unsigned long long *counters; // [primary rays, shadow rays, march iterations, words read] (may be null)
+	const u32 *roots;             // if set: (root, color root) read from the device (graph replays)
 };
 
 __device__ inline u32 q8(float x) { return (u32)(fmn(fmx(x, 0.0f), 1.0f) * 255.0f + 0.5f); }
 
 template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(TraceArgs A) {
 	__shared__ u32 stack_lds[kStack * kThreads];
-	// view of this tile (binary search over the tile prefix)
+	// view of this tile (binary search over the tile prefix); a fixed grid may run past the last tile
 	u32 tile = blockIdx.x, lo = 0, hi = A.n_views;
+	if (tile >= A.tile_start[A.n_views])
+		return;
+	const u32 root = A.roots ? A.roots[0] : A.root, color_root = A.roots ? A.roots[1] : A.color_root;
 	while (hi - lo > 1) {
 		u32 mid = (lo + hi) >> 1;
 		if (A.tile_start[mid] <= tile)
@@ -322,7 +326,7 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 	float len = sqrtf(dx * dx + dy * dy + dz * dz);
 	dx = dx / len, dy = dy / len, dz = dz / len;
 	u32 reads = 0;
-	MarchOut h = march<kCount>(A.words, A.root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx,
+	MarchOut h = march<kCount>(A.words, root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx,
 	                           dy, dz, stack, kThreads, reads);
 	u32 iters = h.iter;
 	float lx = 4.0f, ly = 5.0f, lz = 3.0f;
@@ -330,7 +334,7 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 	lx = lx / ll, ly = ly / ll, lz = lz / ll;
 	bool shadow = false;
 	if (h.hit && (V.flags & HD_TRACE_SHADOW)) {
-		MarchOut s = march<kCount>(A.words, A.root, A.leaf_level, __builtin_inff(), 0.0f, h.hit_pos[0], h.hit_pos[1],
+		MarchOut s = march<kCount>(A.words, root, A.leaf_level, __builtin_inff(), 0.0f, h.hit_pos[0], h.hit_pos[1],
 		                           h.hit_pos[2], lx, ly, lz, stack, kThreads, reads);
 		shadow = s.hit;
 		iters += s.iter;
@@ -339,7 +343,7 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 	if (h.hit) {
 		float nd = h.norm[0] * lx + h.norm[1] * ly + h.norm[2] * lz;
 		float diffuse = shadow ? 0.5f : fmx(nd, 0.0f) * 0.5f + 0.5f;
-		RGB c = color_fetch<kCount>(A.cnodes, A.cleaves, A.color_root, A.leaf_level + 1u, A.color_leaf_level,
+		RGB c = color_fetch<kCount>(A.cnodes, A.cleaves, color_root, A.leaf_level + 1u, A.color_leaf_level,
 		                            h.vox[0], h.vox[1], h.vox[2], reads);
 		r = diffuse * c.r, g = diffuse * c.g, b = diffuse * c.b;
 	}
@@ -379,9 +383,9 @@ __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ 
 }
 
 int trace_on_stream(Pool &p, hipStream_t s, u32 root, u32 color_root, const hd_view *d_views, const u32 *d_tiles,
-                    u32 n_views, u32 n_tiles, u32 *rgba) {
+                    u32 n_views, u32 n_tiles, u32 *rgba, const u32 *d_roots) {
 	TraceArgs A{p.words, p.cnodes, p.cleaves, root,    color_root, p.dc.L, p.dc.color_leaf,
-	            d_views, d_tiles,  n_views,   rgba,    nullptr,    nullptr};
+	            d_views, d_tiles,  n_views,   rgba,    nullptr,    nullptr, d_roots};
 	k_trace<false><<<n_tiles, kThreads, 0, s>>>(A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
@@ -411,7 +415,7 @@ extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd
 	HD_TRY(hipMemcpyAsync(dv, views, n_views * sizeof(hd_view), hipMemcpyHostToDevice, p.stream));
 	HD_TRY(hipMemcpyAsync(dt, tiles.data(), (n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
 	TraceArgs A{p.words, p.cnodes, p.cleaves, root,   color_root, p.dc.L, p.dc.color_leaf,
-	            dv,      dt,       n_views,   rgba, records, nullptr};
+	            dv,      dt,       n_views,   rgba, records, nullptr, nullptr};
 	k_trace<false><<<tiles[n_views], kThreads, 0, p.stream>>>(A);
 	HD_TRY(hipGetLastError());
 	HD_TRY(hipStreamSynchronize(p.stream));
@@ -427,7 +431,7 @@ extern "C" int hd_trace_async(hd_pool *h, uint32_t root, uint32_t color_root, co
                               uint32_t *records, uint64_t *counters) {
 	Pool &p = h->p;
 	TraceArgs A{p.words,      p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
-	            n_views,      rgba,     records,   (unsigned long long *)counters};
+	            n_views,      rgba,     records,   (unsigned long long *)counters, nullptr};
 	if (counters)
 		k_trace<true><<<n_tiles, kThreads, 0, p.stream>>>(A);
 	else
