@@ -123,6 +123,24 @@ def main():
             step_ms.append((t2 - t0) * 1e3)
         log(f"step {s}: edit {1e3 * (t1 - t0):.1f} ms ({vox[s] / 1e6:.1f} M voxels), trace kernel {kms:.3f} ms")
 
+    # the same frame with the beam prepass (beam.frag): coarse 8x8-block march, then the trace
+    vb_ = capi.make_view(view_block, args.width, args.height, capi.TRACE_SHADOW | capi.TRACE_BEAM)
+    beams = np.zeros(2, np.uint32)
+    capi.check(capi.lib().hd_trace_beam_sizes(C_ref(vb_), 1, beams.ctypes.data))
+    dview_b = capi.DeviceBuffer(72)
+    dview_b.upload(np.frombuffer(bytes(vb_), np.uint8))
+    dbeams = capi.DeviceBuffer(8)
+    dbeams.upload(beams)
+    dbeam = capi.DeviceBuffer(int(beams[1]) * 4)
+    beam_ms = []
+    for _ in range(3):
+        pool.timer_begin()
+        capi.check(capi.lib().hd_trace_beam_async(pool.h, pool.root, pool.color_root, dview_b.ptr, dtiles.ptr,
+                                                  dbeams.ptr, 1, int(tiles[1]), int(beams[1]), dbeam.ptr, rgba.ptr))
+        beam_ms.append(pool.timer_end())
+    beam_ms = float(np.mean(beam_ms[1:]))
+    log(f"trace with beam prepass: {beam_ms:.3f} ms (beam image {int(beams[1])} blocks)")
+
     timed_vox = sum(vox[args.warmup:])
     edit_s = sum(edit_ms) / 1e3
     value = timed_vox / edit_s
@@ -192,6 +210,7 @@ def main():
         "rays_per_s": rays_per_s,
         "mrays_per_s": rays_per_s / 1e6,
         "trace_ms": trace_avg_ms,
+        "trace_beam_ms": beam_ms,
         "rays_per_frame": rays,
         "roofline": {
             "kernel": "k_trace (DAG_RayMarch + shadow + Color_Fetch)",

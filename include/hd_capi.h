@@ -170,7 +170,8 @@ typedef struct hd_view {
 
 enum {
 	HD_TRACE_SHADOW = 1, /* one shadow ray per hit toward normalize(4,5,3) (no LOD) */
-	HD_TRACE_RECORDS = 2 /* also write 48-byte per-pixel hit records */
+	HD_TRACE_RECORDS = 2, /* also write 48-byte per-pixel hit records */
+	HD_TRACE_BEAM = 4     /* beam prepass (beam.frag): primary rays start at the 8x8-block coarse depth */
 };
 
 /* Render views on the device (DAG_RayMarch + Color_Fetch + the diffuse shading of trace.frag:374-402).
@@ -185,6 +186,15 @@ int hd_trace(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *v
 int hd_trace_async(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *d_views,
                    const uint32_t *d_tile_start, uint32_t n_views, uint32_t n_tiles, uint32_t *rgba,
                    uint32_t *records, uint64_t *counters);
+/* Beam prepass + trace, asynchronous (beam.frag:59-223, BeamPass.cpp:9-111, TracePass.cpp:35-55,
+ * trace.frag:384-390): d_beam_start = hd_trace_beam_sizes prefix on the device, d_beam = n_beam
+ * floats. Views with HD_TRACE_BEAM start each primary ray at 0.98 x the smallest of the 2x2 beam
+ * texels around the pixel (+inf: the pixel misses) with that distance as the LOD bias. */
+int hd_trace_beam_async(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *d_views,
+                        const uint32_t *d_tile_start, const uint32_t *d_beam_start, uint32_t n_views,
+                        uint32_t n_tiles, uint32_t n_beam, float *d_beam, uint32_t *rgba);
+/* Host helper: beam_start[0..n_views] prefix of beam texels (ceil(W/8) x ceil(H/8) per beam view). */
+int hd_trace_beam_sizes(const hd_view *views, uint32_t n_views, uint32_t *beam_start);
 /* Host helper: tile_start[0..n_views] prefix of 16x16 tiles per view. */
 int hd_trace_tiles(const hd_view *views, uint32_t n_views, uint32_t *tile_start);
 /* HIP-event timing of the work queued on the pool's stream between the two calls. */

@@ -1181,6 +1181,7 @@ struct Hit {
 	float hit_pos[3] = {0, 0, 0};
 	float norm[3] = {0, 0, 0};
 	u32 iter = 0;
+	float t = 0, scale = 0; // t_min and scale_exp2 where the march stopped (beam.frag's outputs)
 };
 
 // DAG_GetLeafFirstChildBits (trace.frag:56-67)
@@ -1311,6 +1312,8 @@ Hit ray_march(WordFn &&word, u32 root, u32 leaf_level, float proj_factor, float 
 		n[2] = -n[2];
 	const u32 voxel_level = leaf_level + 1u, voxel_scale = S - voxel_level;
 	out.iter = iter;
+	out.t = t_min;
+	out.scale = scale_exp2;
 	if (!(scale < S && t_min <= t_max))
 		return out; // a miss reports zeros (trace.frag leaves them unused)
 	u32 vox_size = 1u << (scale - voxel_scale);
@@ -1421,6 +1424,32 @@ inline RGBf color_fetch(const ColorPool &cp, u32 root, u32 voxel_level, u32 leaf
 		return leaf_color(cp.leaves, data, lp);
 	}
 	return unpack_unorm8(data);
+}
+
+// Beam prepass (beam.frag:59-223, BeamPass.cpp:9-111): one coarse ray per 8x8-pixel block through
+// the block's centre, Camera_GenRay over the ceil(W/8) x ceil(H/8) beam image, LOD with the
+// projection factor of a block (the view's proj_factor / 8); the block's value is
+// max(t - cube size, 0) where the march stopped, +inf on a miss.
+constexpr u32 kBeamBlock = 8;
+template <typename WordFn>
+inline float beam_value(WordFn &word, u32 root, u32 L, float proj, V3 pos, V3 look, V3 side, V3 up, u32 bx, u32 by,
+                        u32 Wb, u32 Hb) {
+	float cx = (float(bx) + 0.5f) / float(Wb), cy = (float(by) + 0.5f) / float(Hb);
+	cx = cx * 2.0f - 1.0f;
+	cy = cy * 2.0f - 1.0f;
+	V3 d = normalize3(V3{look.x - side.x * cx - up.x * cy, look.y - side.y * cx - up.y * cy,
+	                     look.z - side.z * cx - up.z * cy});
+	Hit h = ray_march(word, root, L, proj * (1.0f / float(kBeamBlock)), 0.0f, pos, d);
+	return h.hit ? std::max(h.t - h.scale, 0.0f) : INFINITY;
+}
+// The trace's beam lookup (trace.frag:384-390): a MIN-reduction bilinear fetch, i.e. the smallest of
+// the 2x2 beam texels around the pixel centre (clamped to the edge), scaled by 0.98.
+inline float beam_sample(const float *B, u32 Wb, u32 Hb, float u, float v) {
+	float x = u * float(Wb) - 0.5f, y = v * float(Hb) - 0.5f;
+	int x0 = (int)std::floor(x), y0 = (int)std::floor(y);
+	auto cl = [](int a, u32 n) { return (u32)std::min(std::max(a, 0), (int)n - 1); };
+	u32 xa = cl(x0, Wb), xb = cl(x0 + 1, Wb), ya = cl(y0, Hb), yb = cl(y0 + 1, Hb);
+	return std::min(std::min(B[ya * Wb + xa], B[ya * Wb + xb]), std::min(B[yb * Wb + xa], B[yb * Wb + xb])) * 0.98f;
 }
 
 // Brute-force DDA over a dense grid in double precision (Amanatides & Woo), with the

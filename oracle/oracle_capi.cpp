@@ -268,14 +268,30 @@ void hdo_render_arrays(const u32 *words, u64 nwords, u32 L, u32 root, const u32 
 	    up{view[9], view[10], view[11]};
 	float proj = view[12];
 	const V3 light = normalize3(V3{4.0f, 5.0f, 3.0f});
+	// flags bit 2: the beam prepass (beam.frag) and beam-started primary rays (trace.frag:384-390)
+	const bool beam = (flags & 4u) != 0;
+	const u32 Wb = (W + kBeamBlock - 1) / kBeamBlock, Hb = (H + kBeamBlock - 1) / kBeamBlock;
+	std::vector<float> B;
+	if (beam) {
+		B.resize((size_t)Wb * Hb);
+		for (u32 by = 0; by < Hb; ++by)
+			for (u32 bx = 0; bx < Wb; ++bx)
+				B[(size_t)by * Wb + bx] = beam_value(v, root, L, proj, pos, look, side, up, bx, by, Wb, Hb);
+	}
 	for (u32 py = 0; py < H; ++py)
 		for (u32 px = 0; px < W; ++px) {
-			float cx = (float(px) + 0.5f) / float(W), cy = (float(py) + 0.5f) / float(H);
-			cx = cx * 2.0f - 1.0f;
-			cy = cy * 2.0f - 1.0f;
+			float u = (float(px) + 0.5f) / float(W), w = (float(py) + 0.5f) / float(H);
+			float cx = u * 2.0f - 1.0f, cy = w * 2.0f - 1.0f;
 			V3 d = normalize3(V3{look.x - side.x * cx - up.x * cy, look.y - side.y * cx - up.y * cy,
 			                     look.z - side.z * cx - up.z * cy});
-			Hit h = ray_march(v, root, L, proj, 0.0f, pos, d);
+			Hit h;
+			if (!beam)
+				h = ray_march(v, root, L, proj, 0.0f, pos, d);
+			else {
+				float t0 = beam_sample(B.data(), Wb, Hb, u, w);
+				if (!std::isinf(t0))
+					h = ray_march(v, root, L, proj, t0, V3{pos.x + t0 * d.x, pos.y + t0 * d.y, pos.z + t0 * d.z}, d);
+			}
 			bool shadow = false;
 			if (h.hit && (flags & 1u)) {
 				Hit s = ray_march(v, root, L, INFINITY, 0.0f, V3{h.hit_pos[0], h.hit_pos[1], h.hit_pos[2]}, light);
@@ -338,4 +354,17 @@ void hdo_load_live(void *p, const u32 *bucket_words, const u32 *live, u64 nlive,
 	o->color.leaves.assign(cleaves, cleaves + ncleaves);
 }
 
+} // extern "C"
+
+extern "C" {
+// The beam image of one view (beam.frag), ceil(W/8) x ceil(H/8) floats.
+void hdo_beam_image(const u32 *words, u64 nwords, u32 L, u32 root, const float *view, u32 W, u32 H, float *out) {
+	WordView v{words, nwords};
+	V3 pos{view[0], view[1], view[2]}, look{view[3], view[4], view[5]}, side{view[6], view[7], view[8]},
+	    up{view[9], view[10], view[11]};
+	const u32 Wb = (W + kBeamBlock - 1) / kBeamBlock, Hb = (H + kBeamBlock - 1) / kBeamBlock;
+	for (u32 by = 0; by < Hb; ++by)
+		for (u32 bx = 0; bx < Wb; ++bx)
+			out[(size_t)by * Wb + bx] = beam_value(v, root, L, view[12], pos, look, side, up, bx, by, Wb, Hb);
+}
 } // extern "C"

@@ -77,6 +77,7 @@ def olib():
         _sig(L, "hdo_render_arrays", None,
              [u32p, U64, U32, U32, u32p, U64, u32p, U64, U32, U32, f32p, U32, U32, U32, u32p, VP])
         _sig(L, "hdo_dda", None, [u8p, U32, f32p, f32p, U64, u32p])
+        _sig(L, "hdo_beam_image", None, [u32p, U64, U32, U32, f32p, U32, U32, f32p])
         _sig(L, "hdo_load_live", None, [VP, u32p, u32p, U64, u32p, U32, u32p, U64, u32p, U64])
         _olib = L
     return _olib
@@ -355,17 +356,26 @@ def march_rays(words, L, root, o, d, proj_factor=float("inf"), proj_bias=0.0):
     return rec
 
 
-def render(words, L, root, cnodes, cleaves, color_root, color_leaf_level, view, W, H, shadow=True):
+def render(words, L, root, cnodes, cleaves, color_root, color_leaf_level, view, W, H, shadow=True, beam=False):
     rgba = np.zeros(W * H, np.uint32)
     rec = np.zeros((W * H, 12), np.uint32)
     cn = np.ascontiguousarray(cnodes, np.uint32)
     cl = np.ascontiguousarray(cleaves, np.uint32)
+    flags = (1 if shadow else 0) | (4 if beam else 0)
     olib().hdo_render_arrays(np.ascontiguousarray(words, np.uint32), len(words), L, root,
                              cn if len(cn) else np.zeros(8, np.uint32), len(cn) // 8,
                              cl if len(cl) else np.zeros(1, np.uint32), len(cl), color_root, color_leaf_level,
-                             np.ascontiguousarray(view, np.float32), W, H, 1 if shadow else 0, rgba,
-                             rec.ctypes.data)
+                             np.ascontiguousarray(view, np.float32), W, H, flags, rgba, rec.ctypes.data)
     return rgba, rec
+
+
+def beam_image(words, L, root, view, W, H):
+    """beam.frag's image for a W x H view: ceil(W/8) x ceil(H/8) start distances (+inf = miss)."""
+    wb, hb = (W + 7) // 8, (H + 7) // 8
+    out = np.zeros(wb * hb, np.float32)
+    olib().hdo_beam_image(np.ascontiguousarray(words, np.uint32), len(words), L, root,
+                          np.ascontiguousarray(view, np.float32), W, H, out)
+    return out.reshape(hb, wb)
 
 
 def dda(grid, o, d):

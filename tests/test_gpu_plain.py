@@ -139,3 +139,28 @@ def test_traversal_pick_matches_reference_golden():
     hit = ref[:, 3] > 0
     # NodePoolTraversal::Traversal<float> computed on the device with the same fp32 order
     assert np.array_equal(out[hit, :3], ref[hit, :3])
+
+
+def test_render_beam_bit_exact():
+    """Beam prepass (beam.frag) + beam-started primary rays: RGBA8 and hit records equal the
+    oracle's restatement; LOD off (no block projection factor) is rejected."""
+    capi = _capi()
+    res = 256
+    h = W.value_noise_heightmap(res, 5, 4, 40, 200)
+    cfg = small_cfg(res, 10, 9, 2, 4)
+    dev, ora = capi.DevicePool(cfg, heights=h), O.OraclePool(cfg, heights=h)
+    e = W.edits([W.terrain_edit(res, seed=4, skin=8)] + list(W.terrain_sphere_edits(12, res, h, 3, 30, seed=2)))
+    dev.edit(e)
+    ora.edit(e)
+    words = ora.full_words()
+    for seed, (Wd, Ht) in enumerate([(160, 96), (123, 77)]):
+        view = W.orbit_view((res / 2, res / 2, res * 0.3), res * 0.9, seed, Wd, Ht, res=res, elevation=(0.3, 0.6))
+        rgba_d, rec_d = dev.render(view, Wd, Ht, shadow=True, records=True, beam=True)
+        rgba_o, rec_o = O.render(words, cfg.node_levels, ora.root, ora.color_nodes(), ora.color_leaves(),
+                                 ora.color_root, cfg.color_leaf_level, view, Wd, Ht, shadow=True, beam=True)
+        assert (rec_o[:, 3] >> 31).mean() > 0.3
+        assert np.array_equal(rec_d, rec_o), np.nonzero(np.any(rec_d != rec_o, axis=1))[0][:8]
+        assert np.array_equal(rgba_d, rgba_o)
+    view = W.orbit_view((res / 2, res / 2, res * 0.3), res * 0.9, 0, 32, 32, res=res, lod=False)
+    with pytest.raises(capi.HDError, match="finite proj_factor"):
+        dev.render(view, 32, 32, beam=True)

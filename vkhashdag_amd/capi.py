@@ -15,7 +15,7 @@ from .workloads import EDIT_DTYPE, HD_COLOR_NULL, HD_NULL, PoolConfig
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhashdag_amd.so")
 HD_MAX_LEVELS = 24
 MODE_PLAIN, MODE_BATCHED = 0, 1
-TRACE_SHADOW, TRACE_RECORDS = 1, 2
+TRACE_SHADOW, TRACE_RECORDS, TRACE_BEAM = 1, 2, 4
 
 
 class HDError(RuntimeError):
@@ -105,6 +105,8 @@ def lib():
             "hd_pool_read_live_words": (C.c_int, [VP, VP, U64, C.POINTER(U64)]),
             "hd_pool_write_live_words": (C.c_int, [VP, VP, VP, U64]),
             "hd_trace_tiles": (C.c_int, [VP, U32, VP]),
+            "hd_trace_beam_sizes": (C.c_int, [VP, U32, VP]),
+            "hd_trace_beam_async": (C.c_int, [VP, U32, U32, VP, VP, VP, U32, U32, U32, VP, VP]),
             "hd_traversal": (C.c_int, [VP, U32, VP, VP, U64, VP]),
             "hd_device_alloc": (C.c_int, [U64, C.POINTER(VP)]),
             "hd_device_free": (C.c_int, [VP]),
@@ -297,8 +299,8 @@ class DevicePool:
         return out[:n]
 
     # -- rendering (trace.frag) and picking (NodePoolTraversal)
-    def render(self, vb, width, height, shadow=True, records=False, root=None, color_root=None):
-        flags = (TRACE_SHADOW if shadow else 0) | (TRACE_RECORDS if records else 0)
+    def render(self, vb, width, height, shadow=True, records=False, root=None, color_root=None, beam=False):
+        flags = (TRACE_SHADOW if shadow else 0) | (TRACE_RECORDS if records else 0) | (TRACE_BEAM if beam else 0)
         v = make_view(vb, width, height, flags)
         npx = width * height
         rgba = DeviceBuffer(npx * 4)

@@ -11,6 +11,8 @@
 // [scale][thread] so a wavefront's stack accesses hit 64 consecutive banks.
 #include "hd_internal.hpp"
 
+#include <cmath>
+
 namespace hd {
 
 constexpr u32 kStack = 23;  // STACK_SIZE, fraction bits of float (trace.frag:52-54)
@@ -27,6 +29,7 @@ struct MarchOut {
 	bool hit;
 	u32 vox[3], vox_size, iter;
 	float hit_pos[3], norm[3];
+	float t, scale; // t_min and scale_exp2 where the march stopped (beam.frag's outputs)
 	// NodePoolTraversal result: clamp(o + t_min * d, cube) - 1 (valid when hit)
 	float pick[3];
 };
@@ -41,6 +44,8 @@ __device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_l
 	out.hit = false;
 	out.iter = 0;
 	out.vox_size = 0;
+	out.t = 0.0f;
+	out.scale = 0.0f;
 	for (int i = 0; i < 3; ++i)
 		out.vox[i] = 0, out.hit_pos[i] = 0.0f, out.norm[i] = 0.0f, out.pick[i] = 0.0f;
 	if (root == kNull)
@@ -174,6 +179,8 @@ __device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_l
 	const u32 voxel_level = leaf_level + 1u, voxel_scale = kStack - voxel_level;
 	const bool ok = scale < kStack && t_min <= t_max;
 	out.iter = iter;
+	out.t = t_min;
+	out.scale = scale_exp2;
 	if (!ok) { // a miss reports zeros (trace.frag leaves them unused)
 		for (int i = 0; i < 3; ++i)
 			out.vox[i] = 0, out.hit_pos[i] = 0.0f, out.norm[i] = 0.0f, out.pick[i] = 0.0f;
@@ -292,7 +299,24 @@ struct TraceArgs {
 	u32 *rgba, *records;
 	unsigned long long *counters; // [primary rays, shadow rays, march iterations, words read] (may be null)
 	const u32 *roots;             // if set: (root, color root) read from the device (graph replays)
+	const float *beam;            // beam images of the HD_TRACE_BEAM views (may be null)
+	const u32 *beam_start;        // prefix of beam texels per view (n_views + 1)
 };
+
+// Beam prepass (beam.frag:59-223, BeamPass.cpp:9-111): one coarse ray per 8x8-pixel block through
+// the block's centre on the ceil(W/8) x ceil(H/8) beam image, LOD with a block's projection factor
+// (the view's / 8); the block's value is max(t - cube size, 0) where the march stopped, +inf on a
+// miss. The trace starts each pixel at the smallest of the 2x2 beam texels around it, times 0.98
+// (a MIN-reduction bilinear fetch, trace.frag:384-390), with that distance as the LOD bias.
+constexpr u32 kBeamBlock = 8;
+__device__ inline u32 beam_dim(u32 n) { return (n + kBeamBlock - 1u) / kBeamBlock; }
+__device__ inline float beam_sample(const float *__restrict__ B, u32 Wb, u32 Hb, float u, float v) {
+	float x = u * float(Wb) - 0.5f, y = v * float(Hb) - 0.5f;
+	int x0 = (int)floorf(x), y0 = (int)floorf(y);
+	auto cl = [](int a, u32 n) { return (u32)min(max(a, 0), (int)n - 1); };
+	u32 xa = cl(x0, Wb), xb = cl(x0 + 1, Wb), ya = cl(y0, Hb), yb = cl(y0 + 1, Hb);
+	return fminf(fminf(B[ya * Wb + xa], B[ya * Wb + xb]), fminf(B[yb * Wb + xa], B[yb * Wb + xb])) * 0.98f;
+}
 
 __device__ inline u32 q8(float x) { return (u32)(fmn(fmx(x, 0.0f), 1.0f) * 255.0f + 0.5f); }
 
@@ -318,16 +342,27 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 		return;
 	u32 *stack = stack_lds + threadIdx.x;
 	// Camera_GenRay (trace.frag:366-370), gl_FragCoord at pixel centres, origin top-left
-	float cx = (float(px) + 0.5f) / float(V.width), cy = (float(py) + 0.5f) / float(V.height);
-	cx = cx * 2.0f - 1.0f;
-	cy = cy * 2.0f - 1.0f;
+	const float u = (float(px) + 0.5f) / float(V.width), w = (float(py) + 0.5f) / float(V.height);
+	const float cx = u * 2.0f - 1.0f, cy = w * 2.0f - 1.0f;
 	float dx = V.look[0] - V.side[0] * cx - V.up[0] * cy, dy = V.look[1] - V.side[1] * cx - V.up[1] * cy,
 	      dz = V.look[2] - V.side[2] * cx - V.up[2] * cy;
 	float len = sqrtf(dx * dx + dy * dy + dz * dz);
 	dx = dx / len, dy = dy / len, dz = dz / len;
 	u32 reads = 0;
-	MarchOut h = march<kCount>(A.words, root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx,
-	                           dy, dz, stack, kThreads, reads);
+	MarchOut h;
+	if ((V.flags & HD_TRACE_BEAM) && A.beam) {
+		const float t0 = beam_sample(A.beam + A.beam_start[lo], beam_dim(V.width), beam_dim(V.height), u, w);
+		if (isinf(t0)) { // the block's coarse ray missed: so does this pixel
+			h.hit = false;
+			h.iter = 0, h.vox_size = 0, h.t = 0.0f, h.scale = 0.0f;
+			for (int i = 0; i < 3; ++i)
+				h.vox[i] = 0, h.hit_pos[i] = 0.0f, h.norm[i] = 0.0f, h.pick[i] = 0.0f;
+		} else
+			h = march<kCount>(A.words, root, A.leaf_level, V.proj_factor, t0, V.pos[0] + t0 * dx, V.pos[1] + t0 * dy,
+			                  V.pos[2] + t0 * dz, dx, dy, dz, stack, kThreads, reads);
+	} else
+		h = march<kCount>(A.words, root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx, dy,
+		                  dz, stack, kThreads, reads);
 	u32 iters = h.iter;
 	float lx = 4.0f, ly = 5.0f, lz = 3.0f;
 	float ll = sqrtf(lx * lx + ly * ly + lz * lz);
@@ -366,6 +401,37 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 	}
 }
 
+// One thread per beam texel of every HD_TRACE_BEAM view (views without the flag have no texels).
+__global__ void __launch_bounds__(kThreads) k_beam(TraceArgs A, float *__restrict__ beam) {
+	__shared__ u32 stack_lds[kStack * kThreads];
+	const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= A.beam_start[A.n_views])
+		return;
+	const u32 root = A.roots ? A.roots[0] : A.root;
+	u32 lo = 0, hi = A.n_views;
+	while (hi - lo > 1) {
+		u32 mid = (lo + hi) >> 1;
+		if (A.beam_start[mid] <= i)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	const hd_view V = A.views[lo];
+	const u32 Wb = beam_dim(V.width), Hb = beam_dim(V.height), t = i - A.beam_start[lo];
+	const u32 bx = t % Wb, by = t / Wb;
+	float cx = (float(bx) + 0.5f) / float(Wb), cy = (float(by) + 0.5f) / float(Hb);
+	cx = cx * 2.0f - 1.0f;
+	cy = cy * 2.0f - 1.0f;
+	float dx = V.look[0] - V.side[0] * cx - V.up[0] * cy, dy = V.look[1] - V.side[1] * cx - V.up[1] * cy,
+	      dz = V.look[2] - V.side[2] * cx - V.up[2] * cy;
+	float len = sqrtf(dx * dx + dy * dy + dz * dz);
+	dx = dx / len, dy = dy / len, dz = dz / len;
+	u32 reads = 0;
+	MarchOut h = march<false>(A.words, root, A.leaf_level, V.proj_factor * (1.0f / float(kBeamBlock)), 0.0f, V.pos[0],
+	                          V.pos[1], V.pos[2], dx, dy, dz, stack_lds + threadIdx.x, kThreads, reads);
+	beam[i] = h.hit ? fmaxf(h.t - h.scale, 0.0f) : __builtin_inff();
+}
+
 __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ W, u32 root, u32 leaf_level,
                                                          const float *__restrict__ o, const float *__restrict__ d,
                                                          u64 n, float *__restrict__ out) {
@@ -385,7 +451,7 @@ __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ 
 int trace_on_stream(Pool &p, hipStream_t s, u32 root, u32 color_root, const hd_view *d_views, const u32 *d_tiles,
                     u32 n_views, u32 n_tiles, u32 *rgba, const u32 *d_roots) {
 	TraceArgs A{p.words, p.cnodes, p.cleaves, root,    color_root, p.dc.L, p.dc.color_leaf,
-	            d_views, d_tiles,  n_views,   rgba,    nullptr,    nullptr, d_roots};
+	            d_views, d_tiles,  n_views,   rgba,    nullptr,    nullptr, d_roots, nullptr, nullptr};
 	k_trace<false><<<n_tiles, kThreads, 0, s>>>(A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
@@ -400,22 +466,33 @@ extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd
 	if (!h || !views || !n_views || !rgba)
 		return set_error("hd_trace: null argument"), HD_ERR_ARG;
 	Pool &p = h->p;
-	std::vector<u32> tiles(n_views + 1, 0);
+	std::vector<u32> tiles(n_views + 1, 0), beams(n_views + 1, 0);
 	for (u32 v = 0; v < n_views; ++v) {
 		if (!views[v].width || !views[v].height)
 			return set_error("hd_trace: empty view"), HD_ERR_ARG;
 		if ((views[v].flags & HD_TRACE_RECORDS) && !records)
 			return set_error("hd_trace: records requested without a buffer"), HD_ERR_ARG;
+		if ((views[v].flags & HD_TRACE_BEAM) && !std::isfinite(views[v].proj_factor))
+			return set_error("hd_trace: the beam prepass needs a finite proj_factor (LOD on)"), HD_ERR_ARG;
 		tiles[v + 1] = tiles[v] + div_up(views[v].width, kTile) * div_up(views[v].height, kTile);
 	}
+	HD_RET(hd_trace_beam_sizes(views, n_views, beams.data()));
 	hd_view *dv;
-	u32 *dt;
+	u32 *dt, *db = nullptr;
+	float *beam = nullptr;
 	HD_RET(p.scratch.get<hd_view>("trace_views", n_views, &dv));
 	HD_RET(p.scratch.get<u32>("trace_tiles", n_views + 1, &dt));
 	HD_TRY(hipMemcpyAsync(dv, views, n_views * sizeof(hd_view), hipMemcpyHostToDevice, p.stream));
 	HD_TRY(hipMemcpyAsync(dt, tiles.data(), (n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
+	if (beams[n_views]) {
+		HD_RET(p.scratch.get<u32>("trace_beam_start", n_views + 1, &db));
+		HD_RET(p.scratch.get<float>("trace_beam", beams[n_views], &beam));
+		HD_TRY(hipMemcpyAsync(db, beams.data(), (n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
+	}
 	TraceArgs A{p.words, p.cnodes, p.cleaves, root,   color_root, p.dc.L, p.dc.color_leaf,
-	            dv,      dt,       n_views,   rgba, records, nullptr, nullptr};
+	            dv,      dt,       n_views,   rgba, records, nullptr, nullptr, beam, db};
+	if (beams[n_views])
+		k_beam<<<div_up(beams[n_views], kThreads), kThreads, 0, p.stream>>>(A, beam);
 	k_trace<false><<<tiles[n_views], kThreads, 0, p.stream>>>(A);
 	HD_TRY(hipGetLastError());
 	HD_TRY(hipStreamSynchronize(p.stream));
@@ -431,11 +508,34 @@ extern "C" int hd_trace_async(hd_pool *h, uint32_t root, uint32_t color_root, co
                               uint32_t *records, uint64_t *counters) {
 	Pool &p = h->p;
 	TraceArgs A{p.words,      p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
-	            n_views,      rgba,     records,   (unsigned long long *)counters, nullptr};
+	            n_views,      rgba,     records,   (unsigned long long *)counters, nullptr, nullptr, nullptr};
 	if (counters)
 		k_trace<true><<<n_tiles, kThreads, 0, p.stream>>>(A);
 	else
 		k_trace<false><<<n_tiles, kThreads, 0, p.stream>>>(A);
+	HD_TRY(hipGetLastError());
+	return HD_OK;
+}
+
+extern "C" int hd_trace_beam_sizes(const hd_view *views, uint32_t n_views, uint32_t *beam_start) {
+	beam_start[0] = 0;
+	for (u32 v = 0; v < n_views; ++v)
+		beam_start[v + 1] = beam_start[v] + ((views[v].flags & HD_TRACE_BEAM)
+		                                         ? div_up(views[v].width, kBeamBlock) * div_up(views[v].height, kBeamBlock)
+		                                         : 0u);
+	return HD_OK;
+}
+
+// Asynchronous beam prepass + trace: views, tile and beam-texel prefixes already on the device.
+extern "C" int hd_trace_beam_async(hd_pool *h, uint32_t root, uint32_t color_root, const hd_view *d_views,
+                                   const uint32_t *d_tile_start, const uint32_t *d_beam_start, uint32_t n_views,
+                                   uint32_t n_tiles, uint32_t n_beam, float *d_beam, uint32_t *rgba) {
+	Pool &p = h->p;
+	TraceArgs A{p.words, p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
+	            n_views, rgba,     nullptr,   nullptr, nullptr, d_beam, d_beam_start};
+	if (n_beam)
+		k_beam<<<div_up(n_beam, kThreads), kThreads, 0, p.stream>>>(A, d_beam);
+	k_trace<false><<<n_tiles, kThreads, 0, p.stream>>>(A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
 }
