@@ -287,6 +287,15 @@ int hd_serve_run(hd_pool *pool, const hd_serve_config *cfg, uint32_t root, uint3
 /* Free node-pool pages (pages of 2^word_bits_per_page words; a bucket holds 2^page_bits_per_bucket). */
 int hd_pool_free_pages(hd_pool *pool, uint64_t *free_pages, uint64_t *total_pages);
 
+/* Checkpoint dump / load: config, per-bucket counters + used words, color octree nodes and leaf
+ * words with their counters, overflow flags and n_roots caller roots in one file. hd_pool_load
+ * creates a new pool from it (writes the roots back, *n_roots <= cap); edits, GC and traversal
+ * continue from the loaded state word for word. The heightfield (an editor input) is not saved. */
+int hd_pool_save(hd_pool *pool, const char *path, const uint32_t *roots, uint32_t n_roots);
+int hd_pool_load(const char *path, hd_pool **out, uint32_t *roots, uint32_t cap, uint32_t *n_roots);
+/* The config a pool was created with (a loaded pool's comes from its file). */
+int hd_pool_get_config(hd_pool *pool, hd_config *out);
+
 /* Device memory helpers for hosts without a GPU runtime of their own. */
 int hd_device_alloc(uint64_t bytes, void **out);
 int hd_device_free(void *ptr);

@@ -27,3 +27,22 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(LIB)
     missing = [n for n in declared() if not hasattr(lib, n)]
     assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_checkpoint_load_rejects_bad_files_before_touching_the_gpu(tmp_path):
+    # hd_pool_load reads and checks the file header before it creates a pool: no GPU call is made
+    lib = ctypes.CDLL(LIB)
+    lib.hd_last_error.restype = ctypes.c_char_p
+    h, n = ctypes.c_void_p(), ctypes.c_uint32(0)
+    roots = (ctypes.c_uint32 * 4)()
+    rc = lib.hd_pool_load(str(tmp_path / "missing").encode(), ctypes.byref(h), roots, 4, ctypes.byref(n))
+    assert rc != 0 and b"cannot open" in lib.hd_last_error()
+    bad = tmp_path / "bad"
+    bad.write_bytes(b"NOTADAG!" + bytes(200))
+    rc = lib.hd_pool_load(str(bad).encode(), ctypes.byref(h), roots, 4, ctypes.byref(n))
+    assert rc != 0 and b"not a version-1 checkpoint" in lib.hd_last_error()
+    bad.write_bytes(b"HDAG")
+    rc = lib.hd_pool_load(str(bad).encode(), ctypes.byref(h), roots, 4, ctypes.byref(n))
+    assert rc != 0 and b"truncated" in lib.hd_last_error()
+    assert not h.value

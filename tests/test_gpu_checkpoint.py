@@ -1,0 +1,72 @@
+"""Checkpoint dump / load (hd_pool_save / hd_pool_load).
+
+Bar: bit-exact. A pool loaded from a file equals the saved pool (roots, counters, used words, color
+pool, overflow flags); edits, a GC with extra roots and a render continued on the loaded pool give
+the same words and pixels as on the original. Malformed files are rejected with an error."""
+import numpy as np
+import pytest
+
+from test_gpu_async import assert_same_device_pools
+from test_oracle_pins import small_cfg
+from vkhashdag_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _capi():
+    from vkhashdag_amd import capi
+    return capi
+
+
+@pytest.mark.parametrize("keep_history", [False, True])
+def test_save_load_continue_bit_exact(tmp_path, keep_history):
+    capi = _capi()
+    res, seed = 128, 5
+    h = W.value_noise_heightmap(res, 4, seed, res // 6, res * 3 // 4)
+    cfg = small_cfg(res, 9, 9, 2, 3)
+    cfg.keep_history = keep_history
+    a = capi.DevicePool(cfg, heights=h)
+    a.edit(W.terrain_edit(res, seed=seed, skin=6))
+    a.edit(W.mixed_batch(40, res, h, 2, 14, seed=11), capi.MODE_BATCHED)
+    old = (a.root, a.color_root)
+    a.edit(W.terrain_sphere_edits(8, res, h, 3, 12, seed=12))
+    path = tmp_path / "pool.hdag"
+    a.save(path, extra_roots=[old[0]])
+    b, extra = capi.DevicePool.load(path, heights=h)
+    assert extra == [old[0]]
+    assert b.cfg.total_words == cfg.total_words and b.cfg.keep_history == keep_history
+    assert_same_device_pools(a, b, "after load")
+
+    e = W.mixed_batch(24, res, h, 2, 16, seed=13)
+    for p in (a, b):
+        p.edit(e, capi.MODE_BATCHED)
+        p.edit(W.terrain_sphere_edits(4, res, h, 2, 10, seed=14))
+    assert_same_device_pools(a, b, "after continued edits")
+    ra = a.gc(extra_roots=[old[0]])
+    rb = b.gc(extra_roots=extra)
+    assert np.array_equal(ra, rb)
+    assert_same_device_pools(a, b, "after GC")
+    vb = W.orbit_view((res / 2, res / 2, res / 3), res * 0.9, 3, 64, 48, res=res)
+    assert np.array_equal(a.render(vb, 64, 48), b.render(vb, 64, 48))
+
+
+def test_load_rejects_bad_files(tmp_path):
+    capi = _capi()
+    cfg = small_cfg(64, 6, 5, 2, 3)
+    a = capi.DevicePool(cfg)
+    a.edit(W.edits([W.edit(W.SPHERE_FILL, (32, 32, 32), r2=100, color=0x123456)]))
+    good = tmp_path / "good.hdag"
+    a.save(good)
+    data = good.read_bytes()
+    bad = tmp_path / "bad.hdag"
+    bad.write_bytes(b"NOTADAG!" + data[8:])
+    with pytest.raises(capi.HDError, match="checkpoint"):
+        capi.DevicePool.load(bad)
+    bad.write_bytes(data[: len(data) - 8])
+    with pytest.raises(capi.HDError, match="truncated"):
+        capi.DevicePool.load(bad)
+    with pytest.raises(capi.HDError, match="cannot open"):
+        capi.DevicePool.load(tmp_path / "missing.hdag")
+    b, extra = capi.DevicePool.load(good)
+    assert extra == []
+    assert_same_device_pools(a, b, "small pool")

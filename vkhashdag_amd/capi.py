@@ -104,6 +104,9 @@ def lib():
             "hd_timer_end": (C.c_int, [VP, C.POINTER(C.c_float)]),
             "hd_pool_read_live_words": (C.c_int, [VP, VP, U64, C.POINTER(U64)]),
             "hd_pool_write_live_words": (C.c_int, [VP, VP, VP, U64]),
+            "hd_pool_save": (C.c_int, [VP, C.c_char_p, VP, U32]),
+            "hd_pool_load": (C.c_int, [C.c_char_p, C.POINTER(VP), VP, U32, C.POINTER(U32)]),
+            "hd_pool_get_config": (C.c_int, [VP, C.POINTER(hd_config)]),
             "hd_trace_tiles": (C.c_int, [VP, U32, VP]),
             "hd_trace_beam_sizes": (C.c_int, [VP, U32, VP]),
             "hd_trace_beam_async": (C.c_int, [VP, U32, U32, VP, VP, VP, U32, U32, U32, VP, VP]),
@@ -195,6 +198,36 @@ class DevicePool:
         self.color_root = HD_COLOR_NULL
         if heights is not None:
             self.set_heightmap(heights)
+
+    # -- checkpoint dump / load (pool words + counters, color pool, roots)
+    def save(self, path, extra_roots=()):
+        """Write the pool and its roots (root, color_root, *extra_roots) to one file."""
+        roots = np.asarray([self.root, self.color_root, *extra_roots], np.uint32)
+        check(lib().hd_pool_save(self.h, os.fsencode(path), roots.ctypes.data, len(roots)))
+
+    @classmethod
+    def load(cls, path, heights=None, max_roots=64):
+        """A new pool from a file written by save(); returns (pool, extra_roots)."""
+        h, n = VP(), U32(0)
+        roots = np.zeros(max_roots, np.uint32)
+        check(lib().hd_pool_load(os.fsencode(path), C.byref(h), roots.ctypes.data, max_roots, C.byref(n)))
+        c = hd_config()
+        rc = lib().hd_pool_get_config(h.value, C.byref(c))
+        if rc != 0:
+            lib().hd_pool_destroy(h.value)
+            check(rc)
+        self = cls.__new__(cls)
+        self.h = h.value
+        self.cfg = PoolConfig(c.word_bits_per_page, c.page_bits_per_bucket, list(c.bucket_bits[: c.node_levels]),
+                              c.color_leaf_level, bool(c.color_keep_history), c.color_node_capacity,
+                              c.color_leaf_capacity)
+        if n.value < 2:
+            self.close()
+            raise HDError("checkpoint holds fewer than two roots (root, color_root)")
+        self.root, self.color_root = int(roots[0]), int(roots[1])
+        if heights is not None:
+            self.set_heightmap(heights)
+        return self, [int(r) for r in roots[2: n.value]]
 
     def close(self):
         if getattr(self, "h", None):

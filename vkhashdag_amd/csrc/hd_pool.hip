@@ -439,3 +439,141 @@ extern "C" int hd_pool_write_live_words(hd_pool *h, const uint32_t *bucket_words
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
 }
+
+// ---------------------------------------------------------------- checkpoint dump / load
+// The pool's state as one file: the config, every bucket's word counter and used words (packed in
+// bucket order: what DAGNodePool::Flush, src/DAGNodePool.cpp:56, uploads in a full flush), the color
+// octree nodes and leaf words with their counters (DAGColorPool::Flush, src/DAGColorPool.cpp:50),
+// the overflow flags and the caller's roots. The reference keeps no checkpoint of its own. Loading into a fresh pool restores a state from which edits, GC and traversal continue
+// word for word. The heightfield is an editor input, not pool state, and is not saved.
+namespace {
+struct CkptHeader {
+	char magic[8];
+	uint32_t version, n_roots;
+	hd_config cfg;
+	uint64_t total_buckets, n_live, color_nodes, color_leaf_words;
+	uint32_t overflow, pad_;
+};
+constexpr char kCkptMagic[8] = {'H', 'D', 'A', 'G', 'C', 'K', 'P', '1'};
+
+struct File {
+	FILE *f = nullptr;
+	~File() {
+		if (f)
+			fclose(f);
+	}
+};
+int put(FILE *f, const void *p, size_t bytes) {
+	if (bytes && fwrite(p, 1, bytes, f) != bytes)
+		return set_error("checkpoint: write failed"), HD_ERR_ARG;
+	return HD_OK;
+}
+int get(FILE *f, void *p, size_t bytes) {
+	if (bytes && fread(p, 1, bytes, f) != bytes)
+		return set_error("checkpoint: truncated file"), HD_ERR_ARG;
+	return HD_OK;
+}
+} // namespace
+
+extern "C" int hd_pool_save(hd_pool *h, const char *path, const uint32_t *roots, uint32_t n_roots) {
+	if (!h || !path || (!roots && n_roots))
+		return set_error("hd_pool_save: null argument"), HD_ERR_ARG;
+	Pool &p = h->p;
+	u32 meta[kMetaCount];
+	HD_RET(read_meta(p, meta));
+	CkptHeader hd{};
+	std::memcpy(hd.magic, kCkptMagic, 8);
+	hd.version = 1;
+	hd.n_roots = n_roots;
+	hd.cfg = p.cfg;
+	hd.total_buckets = p.total_buckets;
+	hd.color_nodes = meta[kMetaColorNodes];
+	hd.color_leaf_words = meta[kMetaColorLeaves];
+	hd.overflow = meta[kMetaOverflow];
+	HD_RET(hd_pool_read_live_words(h, nullptr, 0, &hd.n_live));
+	std::vector<u32> bw(p.total_buckets), live(hd.n_live ? hd.n_live : 1), cn(8 * hd.color_nodes + 1),
+	    cl(hd.color_leaf_words + 1);
+	HD_TRY(hipMemcpy(bw.data(), p.bucket_words, p.total_buckets * 4, hipMemcpyDeviceToHost));
+	u64 n = 0;
+	HD_RET(hd_pool_read_live_words(h, live.data(), live.size(), &n));
+	if (hd.color_nodes)
+		HD_TRY(hipMemcpy(cn.data(), p.cnodes, hd.color_nodes * 32, hipMemcpyDeviceToHost));
+	if (hd.color_leaf_words)
+		HD_TRY(hipMemcpy(cl.data(), p.cleaves, hd.color_leaf_words * 4, hipMemcpyDeviceToHost));
+	File F;
+	F.f = fopen(path, "wb");
+	if (!F.f)
+		return set_error("hd_pool_save: cannot open %s", path), HD_ERR_ARG;
+	HD_RET(put(F.f, &hd, sizeof(hd)));
+	HD_RET(put(F.f, bw.data(), bw.size() * 4));
+	HD_RET(put(F.f, live.data(), hd.n_live * 4));
+	HD_RET(put(F.f, cn.data(), hd.color_nodes * 32));
+	HD_RET(put(F.f, cl.data(), hd.color_leaf_words * 4));
+	HD_RET(put(F.f, roots, (size_t)n_roots * 4));
+	if (fflush(F.f) != 0)
+		return set_error("hd_pool_save: write failed"), HD_ERR_ARG;
+	return HD_OK;
+}
+
+extern "C" int hd_pool_load(const char *path, hd_pool **out, uint32_t *roots, uint32_t cap, uint32_t *n_roots) {
+	if (!path || !out || !n_roots || (!roots && cap))
+		return set_error("hd_pool_load: null argument"), HD_ERR_ARG;
+	File F;
+	F.f = fopen(path, "rb");
+	if (!F.f)
+		return set_error("hd_pool_load: cannot open %s", path), HD_ERR_ARG;
+	CkptHeader hd{};
+	HD_RET(get(F.f, &hd, sizeof(hd)));
+	if (std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.version != 1)
+		return set_error("hd_pool_load: %s is not a version-1 checkpoint", path), HD_ERR_ARG;
+	if (hd.n_roots > cap)
+		return set_error("hd_pool_load: %u roots, room for %u", hd.n_roots, cap), HD_ERR_ARG;
+	if (hd.color_nodes > hd.cfg.color_node_capacity || hd.color_leaf_words > hd.cfg.color_leaf_capacity)
+		return set_error("hd_pool_load: color counts exceed the saved capacities"), HD_ERR_ARG;
+	hd_pool *h = nullptr;
+	HD_RET(hd_pool_create(&hd.cfg, &h));
+	struct Guard {
+		hd_pool *h;
+		~Guard() {
+			if (h)
+				hd_pool_destroy(h);
+		}
+	} guard{h};
+	Pool &p = h->p;
+	if (p.total_buckets != hd.total_buckets)
+		return set_error("hd_pool_load: bucket count mismatch"), HD_ERR_ARG;
+	std::vector<u32> bw(hd.total_buckets), live(hd.n_live ? hd.n_live : 1), cn(8 * hd.color_nodes + 1),
+	    cl(hd.color_leaf_words + 1);
+	HD_RET(get(F.f, bw.data(), bw.size() * 4));
+	HD_RET(get(F.f, live.data(), hd.n_live * 4));
+	HD_RET(get(F.f, cn.data(), hd.color_nodes * 32));
+	HD_RET(get(F.f, cl.data(), hd.color_leaf_words * 4));
+	HD_RET(get(F.f, roots, (size_t)hd.n_roots * 4));
+	const u64 wpb = 1ull << p.dc.wbpb();
+	for (u64 b = 0; b < hd.total_buckets; ++b)
+		if (bw[b] > wpb)
+			return set_error("hd_pool_load: bucket %llu holds %u words (> %llu)", (unsigned long long)b, bw[b],
+			                 (unsigned long long)wpb),
+			       HD_ERR_ARG;
+	HD_RET(hd_pool_write_live_words(h, bw.data(), live.data(), hd.n_live));
+	if (hd.color_nodes)
+		HD_TRY(hipMemcpy(p.cnodes, cn.data(), hd.color_nodes * 32, hipMemcpyHostToDevice));
+	if (hd.color_leaf_words)
+		HD_TRY(hipMemcpy(p.cleaves, cl.data(), hd.color_leaf_words * 4, hipMemcpyHostToDevice));
+	u32 meta[kMetaCount] = {};
+	meta[kMetaColorNodes] = (u32)hd.color_nodes;
+	meta[kMetaColorLeaves] = (u32)hd.color_leaf_words;
+	meta[kMetaOverflow] = hd.overflow;
+	HD_TRY(hipMemcpy(p.meta, meta, sizeof(meta), hipMemcpyHostToDevice));
+	*n_roots = hd.n_roots;
+	*out = h;
+	guard.h = nullptr;
+	return HD_OK;
+}
+
+extern "C" int hd_pool_get_config(hd_pool *h, hd_config *out) {
+	if (!h || !out)
+		return set_error("hd_pool_get_config: null argument"), HD_ERR_ARG;
+	*out = h->p.cfg;
+	return HD_OK;
+}
