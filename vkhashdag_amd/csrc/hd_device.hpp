@@ -61,6 +61,46 @@ __host__ __device__ inline u32 murmur_span(const u32 *w, u32 n) {
 	h ^= h >> 16;
 	return h;
 }
+// murmur_span over a node held in a 9-word register array (static indices only, so the array
+// never goes to scratch memory); n <= 9.
+__device__ inline u32 murmur_regs(const u32 (&w)[9], u32 n) {
+	u32 h = 0;
+#pragma unroll
+	for (u32 i = 0; i < 9; ++i) {
+		if (i < n) {
+			u32 k = w[i] * 0xcc9e2d51u;
+			k = (k << 15) | (k >> 17);
+			k *= 0x1b873593u;
+			h ^= k;
+			h = (h << 13) | (h >> 19);
+			h = h * 5u + 0xe6546b64u;
+		}
+	}
+	h ^= n;
+	h ^= h >> 16;
+	h *= 0x85ebca6bu;
+	h ^= h >> 13;
+	h *= 0xc2b2ae35u;
+	h ^= h >> 16;
+	return h;
+}
+// The packed inner node of child mask nm over the 8 slot pointers nk (NodePool.hpp:214-226 word
+// layout: mask word, then the set children in slot order), compacted with static register
+// indices: word j (1..8) is the child of the slot whose rank among the set slots is j.
+__device__ inline void pack_inner(u32 nm, const u32 (&nk)[8], u32 (&c)[9]) {
+	c[0] = nm;
+#pragma unroll
+	for (u32 j = 1; j < 9; ++j)
+		c[j] = 0u;
+#pragma unroll
+	for (u32 i = 0; i < 8; ++i) {
+		const bool set = (nm >> i) & 1u;
+		const u32 pos = (u32)__builtin_popcount(nm & ((1u << i) - 1u)) + 1u;
+#pragma unroll
+		for (u32 j = 1; j <= i + 1; ++j)
+			c[j] = (set && pos == j) ? nk[i] : c[j];
+	}
+}
 __host__ __device__ inline u32 murmur_leaf(u32 w0, u32 w1) {
 	u64 h = (u64)w0 | ((u64)w1 << 32);
 	h ^= h >> 33;
@@ -353,7 +393,7 @@ __device__ inline u32 find_in_bucket(const u32 *__restrict__ words, const DevCon
 // find_in_bucket with the bucket's first 16 words fetched together with its word count, so a
 // bucket of up to 16 words costs one memory round trip. bw_out: the bucket's word count.
 __device__ inline u32 find_in_bucket_win(const u32 *__restrict__ words, const u32 *__restrict__ bucket_words,
-                                         const DevConfig &c, u32 bucket, const u32 *node, u32 n, bool leaf,
+                                         const DevConfig &c, u32 bucket, const u32 (&node)[9], u32 n, bool leaf,
                                          u32 &bw_out) {
 	const u64 base = (u64)bucket << c.wbpb();
 	u32 W[16];
@@ -399,14 +439,121 @@ __device__ inline u32 find_in_bucket_win(const u32 *__restrict__ words, const u3
 		}
 		if (sz == n && w0 == node[0]) {
 			bool eq = true;
-			for (u32 k = 1; k < n; ++k)
-				eq = eq && get(i + k) == node[k];
+#pragma unroll
+			for (u32 k = 1; k < 9; ++k)
+				if (k < n)
+					eq = eq && get(i + k) == node[k];
 			if (eq)
 				return (u32)(base + i);
 		}
 		i += sz;
 	}
 	return kNull;
+}
+// find_node over an inner-node bucket held in LDS (T[0, bw)); node in registers (static indices).
+__device__ inline u32 find_in_words(const u32 *T, const DevConfig &c, u64 base, u32 bw, const u32 (&node)[9],
+                                    u32 n) {
+	u32 i = 0;
+	while (i < bw) {
+		const u32 w0 = T[i], sz = inner_words(w0);
+		if (sz == 0) {
+			i = ((i >> c.wbpp) + 1u) << c.wbpp;
+			continue;
+		}
+		if (sz == n && w0 == node[0]) {
+			bool eq = true;
+#pragma unroll
+			for (u32 k = 1; k < 9; ++k)
+				if (k < n)
+					eq = eq && T[i + k] == node[k];
+			if (eq)
+				return (u32)(base + i);
+		}
+		i += sz;
+	}
+	return kNull;
+}
+// The same scan over global memory (one word per dependent load).
+__device__ inline u32 find_in_global(const u32 *__restrict__ words, const DevConfig &c, u64 base, u32 bw,
+                                     const u32 (&node)[9], u32 n) {
+	return find_in_words(words + base, c, base, bw, node, n);
+}
+
+// find_node for an inner node with an LDS staging area shared by a workgroup: the bucket's first
+// 64 words come with its word count in one round trip, the rest (if any) in one more round trip
+// of independent 64-word batches; the used words are copied to stage[*top ..] (bump-allocated,
+// cap words, reset by the caller between uses) and scanned there. A bucket that does not fit in
+// the remaining stage is scanned in global memory. Buckets of fewer than 64 words per bucket
+// take find_in_bucket_win.
+__device__ inline u32 find_in_bucket_staged(const u32 *__restrict__ words, const u32 *__restrict__ bucket_words,
+                                            const DevConfig &c, u32 bucket, const u32 (&node)[9], u32 n,
+                                            u32 &bw_out, u32 *stage, u32 *top, u32 cap) {
+	if (c.wbpb() < 6)
+		return find_in_bucket_win(words, bucket_words, c, bucket, node, n, false, bw_out);
+	const u64 base = (u64)bucket << c.wbpb();
+	const uint4 *q = reinterpret_cast<const uint4 *>(words + base);
+	uint4 v[16];
+#pragma unroll
+	for (u32 k = 0; k < 16; ++k)
+		v[k] = q[k];
+	const u32 bw = bucket_words[bucket];
+	bw_out = bw;
+	if (bw <= 16u) { // scan the first 16 words in registers
+		u32 W[16];
+#pragma unroll
+		for (u32 k = 0; k < 4; ++k) {
+			W[4 * k] = v[k].x;
+			W[4 * k + 1] = v[k].y;
+			W[4 * k + 2] = v[k].z;
+			W[4 * k + 3] = v[k].w;
+		}
+		auto get = [&](u32 i) -> u32 {
+			u32 x = 0;
+#pragma unroll
+			for (u32 k = 0; k < 16; ++k)
+				x = k == i ? W[k] : x;
+			return x;
+		};
+		u32 i = 0;
+		while (i < bw) {
+			const u32 w0 = get(i), sz = inner_words(w0);
+			if (sz == 0) { // page-tail padding: the next node starts the next page
+				i = ((i >> c.wbpp) + 1u) << c.wbpp;
+				continue;
+			}
+			if (sz == n && w0 == node[0]) {
+				bool eq = true;
+#pragma unroll
+				for (u32 k = 1; k < 9; ++k)
+					if (k < n)
+						eq = eq && get(i + k) == node[k];
+				if (eq)
+					return (u32)(base + i);
+			}
+			i += sz;
+		}
+		return kNull;
+	}
+	const u32 len = (bw + 3u) & ~3u;
+	const u32 off = atomicAdd(top, len);
+	if (off + len > cap)
+		return find_in_global(words, c, base, bw, node, n);
+	uint4 *T4 = reinterpret_cast<uint4 *>(stage + off);
+#pragma unroll
+	for (u32 k = 0; k < 16; ++k)
+		if (4u * k < len)
+			T4[k] = v[k];
+	for (u32 i = 64; i < len; i += 64) { // whole 64-word batches stay inside the bucket (64 | its size)
+		uint4 t[16];
+#pragma unroll
+		for (u32 k = 0; k < 16; ++k)
+			t[k] = q[(i >> 2) + k];
+#pragma unroll
+		for (u32 k = 0; k < 16; ++k)
+			if (i + 4u * k < len)
+				T4[(i >> 2) + k] = t[k];
+	}
+	return find_in_words(stage + off, c, base, bw, node, n);
 }
 // append_node (NodePool.hpp:134-157): full -> kNull; a node that would straddle a page zero-fills
 // the page tail and starts the next page. Updates bw.

@@ -1245,15 +1245,16 @@ struct GroupTable {
 
 // Result of a child entry once its level's upserts are done (k_resolve's rule).
 __device__ inline u32 child_result(const ALevel &C, const GroupTable &G, u32 c) {
-	if (c >= C.cap)
-		return kNull; // unreachable with sound bounds (the pass reports an overflow)
-	if (!C.pend[c])
-		return C.status[c];
-	u32 r = C.res[c];
-	if (r != kNull)
-		return r; // found among the bucket's existing nodes
-	u32 s = C.slot[c];
-	return G.gfail[s] ? C.ent[c].ptr : G.gres[s];
+	// every load is issued unconditionally (indices clamped into their arrays) so the lookups of a
+	// node's children overlap: two dependent round trips in all
+	const u32 cc = min(c, C.cap - 1u);
+	const u32 pend = C.pend[cc], st = C.status[cc], r = C.res[cc], s = C.slot[cc], pt = C.ent[cc].ptr;
+	const u32 si = s & G.mask;
+	const u32 gf = G.gfail[si], gr = G.gres[si];
+	// not pending: status; found among the bucket's existing nodes: res; else the group's result
+	// (the old pointer when the bucket was full)
+	const u32 v = !pend ? st : (r != kNull ? r : (gf ? pt : gr));
+	return c < C.cap ? v : kNull; // c >= cap is unreachable with sound bounds (the pass reports it)
 }
 
 // edit_leaf (k_leaf) + the find among the bucket's existing leaves: one wavefront per leaf.
@@ -1328,21 +1329,49 @@ struct Built {
 	bool need; // new content that is not in the bucket yet
 	u32 k, h, b, c[9];
 };
+// The words of an inner entry that its join reads: old pointer, proceed mask, first proceeding
+// child, the 8 old / terminal child pointers (loaded ahead of the join where latency allows).
+struct JoinIn {
+	u32 ptr, pmw, fc;
+	uint4 k0, k1;
+};
+__device__ inline JoinIn load_join(const ALevel &A, u32 e) {
+	JoinIn q;
+	q.ptr = A.ent[e].ptr;
+	q.pmw = A.pmask[e];
+	q.fc = A.first_child[e];
+	q.k0 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e];
+	q.k1 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e + 1];
+	return q;
+}
 template <typename ChildFn>
-__device__ inline Built build_node(const EditParams &P, u32 l, const ALevel &A, u32 e, const Entry &E, ChildFn child) {
+__device__ inline Built build_node(const EditParams &P, u32 l, const JoinIn &q, ChildFn child) {
 	Built o;
-	const u32 pmw = A.pmask[e], pmk = pmw & 0xFFu, fc = A.first_child[e];
-	const uint4 k0 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e];
-	const uint4 k1 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e + 1];
+	const u32 pmw = q.pmw, pmk = pmw & 0xFFu, fc = q.fc;
+	const uint4 k0 = q.k0, k1 = q.k1;
 	const u32 kv[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
 	bool changed = (pmw >> 8) != 0;
+	// the proceeding children are entries fc .. fc + cnt - 1 of the next level: look them all up
+	// before using any result, so their loads overlap
+	const u32 cnt = (u32)__builtin_popcount(pmk);
+	u32 cr[8];
+	if (cnt) {
+#pragma unroll
+		for (u32 j = 0; j < 8; ++j)
+			cr[j] = child(fc + min(j, cnt - 1u));
+	}
 	u32 nm = 0, nk[8];
+#pragma unroll
 	for (u32 i = 0; i < 8; ++i) {
 		u32 r = kv[i];
 		if ((pmk >> i) & 1u) {
-			u32 cr = child(fc + (u32)__builtin_popcount(pmk & ((1u << i) - 1u)));
-			changed |= cr != r;
-			r = cr;
+			const u32 rank = (u32)__builtin_popcount(pmk & ((1u << i) - 1u));
+			u32 c = cr[0];
+#pragma unroll
+			for (u32 j = 1; j <= i; ++j)
+				c = rank == j ? cr[j] : c;
+			changed |= c != r;
+			r = c;
 		}
 		nk[i] = r;
 		if (r != kNull)
@@ -1351,16 +1380,13 @@ __device__ inline Built build_node(const EditParams &P, u32 l, const ALevel &A, 
 	o.need = false;
 	o.k = 0;
 	if (!changed || nm == 0) {
-		o.fin = changed ? kNull : E.ptr;
+		o.fin = changed ? kNull : q.ptr;
 		return o;
 	}
-	o.c[0] = nm;
-	u32 k = 1;
-	for (u32 i = 0; i < 8; ++i)
-		if (nk[i] != kNull)
-			o.c[k++] = nk[i];
+	pack_inner(nm, nk, o.c);
+	const u32 k = (u32)__builtin_popcount(nm) + 1u;
 	o.k = k;
-	o.h = murmur_span(o.c, k);
+	o.h = murmur_regs(o.c, k);
 	o.b = P.c.level_base[l] + (o.h & ((1u << P.c.bucket_bits[l]) - 1u));
 	return o;
 }
@@ -1368,21 +1394,22 @@ __device__ inline Built build_node(const EditParams &P, u32 l, const ALevel &A, 
 // edit_node's join (k_build) for entry e of level l, with the children's results resolved from
 // level l + 1, then the find among the bucket's existing nodes.
 __device__ inline void build_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, const GroupTable &G,
-                                   u32 e) {
-	const Entry E = A.ent[e];
-	Built o = build_node(P, l, A, e, E, [&](u32 c) { return child_result(C, G, c); });
+                                   u32 e, u32 *stage, u32 *top, u32 cap) {
+	Built o = build_node(P, l, load_join(A, e), [&](u32 c) { return child_result(C, G, c); });
 	if (!o.k) {
 		A.pend[e] = 0;
 		A.status[e] = o.fin;
 		return;
 	}
-	for (u32 j = 0; j < o.k; ++j)
-		A.content[(u64)e * 9u + j] = o.c[j];
+#pragma unroll
+	for (u32 j = 0; j < 9; ++j)
+		if (j < o.k)
+			A.content[(u64)e * 9u + j] = o.c[j];
 	A.pend[e] = 1;
 	A.csize[e] = o.k;
 	A.bucket[e] = o.b;
 	u32 bw;
-	A.res[e] = find_in_bucket_win(P.words, P.bucket_words, P.c, o.b, o.c, o.k, false, bw);
+	A.res[e] = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, bw, stage, top, cap);
 }
 
 // Joins entry e (pending, not in the pool) to its content's group; the creator queues the group.
@@ -1493,10 +1520,20 @@ __device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, c
 
 __global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel *__restrict__ lv,
                                                const u32 *__restrict__ pm, GroupTable G) {
+	constexpr u32 kCap = 16384; // staging words per workgroup (64 KiB)
+	__shared__ __align__(16) u32 stage[kCap];
+	__shared__ u32 top;
 	const ALevel A = lv[l], C = lv[l + 1];
 	const u32 n = pm[kPmHdr + l];
-	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-		build_entry(P, l, A, C, G, e);
+	for (u32 e0 = blockIdx.x * blockDim.x; e0 < n; e0 += gridDim.x * blockDim.x) { // uniform trip count
+		if (threadIdx.x == 0)
+			top = 0;
+		__syncthreads();
+		const u32 e = e0 + threadIdx.x;
+		if (e < n)
+			build_entry(P, l, A, C, G, e, stage, &top, kCap);
+		__syncthreads();
+	}
 }
 __global__ void __launch_bounds__(kAT) a_dedup(EditParams P, u32 l, const ALevel *__restrict__ lv,
                                                const u32 *__restrict__ pm, GroupTable G, u32 *__restrict__ bcnt,
@@ -1865,6 +1902,8 @@ constexpr u32 kSmallLevel = 1024; // entries of a level handled inside a_bottom
 constexpr u32 kSTab = 2048;       // LDS dedup / bucket-map slots (load factor <= 1/2)
 constexpr u32 kColorLds = 4096;   // SetNode entries (all levels above the color leaves) kept in LDS
 
+constexpr u32 kStage = 5 * kSTab; // bucket staging words of the build (shares the dedup tables' LDS)
+
 struct BottomLds {
 	// per entry; [2] = this level / its child level, whose results the build resolves
 	u32 res[2][kSmallLevel];  // final pointer, or kNull while a group result is pending
@@ -1873,10 +1912,16 @@ struct BottomLds {
 	u32 content[kSmallLevel][9];
 	u32 csz[kSmallLevel], hash[kSmallLevel], bucket[kSmallLevel], bw[kSmallLevel], bmap[kSmallLevel];
 	u32 glist[kSmallLevel], gnext[kSmallLevel]; // groups (their slots) and the per-bucket group lists
-	u32 tab[kSTab], gres[kSTab];                // dedup table (lowest entry per content); result per slot
-	u32 bkey[kSTab], bmin[kSTab], bhead[kSTab], bn[kSTab]; // bucket map: lowest entry, groups
+	u32 gres[kSTab];                            // result per dedup slot (read by the parent's build)
+	union {
+		struct { // dedup and walk: cleared after the build
+			u32 tab[kSTab];                                        // lowest entry per content
+			u32 bkey[kSTab], bmin[kSTab], bhead[kSTab], bn[kSTab]; // bucket map: lowest entry, groups
+		};
+		__align__(16) u32 stage[kStage]; // build: the scanned buckets' words
+	};
 	unsigned char gfail[kSTab];
-	u32 gcount;
+	u32 gcount, stage_top;
 };
 struct ColorLds {
 	u32 r[kColorLds], cnt[kColorLds], base[kColorLds];
@@ -1897,8 +1942,12 @@ __device__ inline bool lds_same(const BottomLds &S, u32 a, u32 b) {
 	return true;
 }
 __device__ inline u32 lds_result(const BottomLds &S, u32 p, u32 c) {
-	const u32 s = S.slot[p][c];
-	return s == kNull ? S.res[p][c] : (S.gfail[s] ? S.ptr[p][c] : S.gres[s]);
+	// loads unconditional (indices clamped) so a node's child lookups overlap
+	const u32 cc = min(c, kSmallLevel - 1u);
+	const u32 s = S.slot[p][cc], r = S.res[p][cc], pt = S.ptr[p][cc];
+	const u32 si = s & (kSTab - 1u);
+	const u32 v = s == kNull ? r : (S.gfail[si] ? pt : S.gres[si]);
+	return c < kSmallLevel ? v : kNull;
 }
 
 // Walk of a small level's bucket by its lowest new entry (the first group's lowest entry): the
@@ -2074,66 +2123,98 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
                                                 int l_from, GroupTable G, u32 *__restrict__ roots,
                                                 u32 *__restrict__ acc, u64 node_cap) {
 	__shared__ ALds U;
+	__shared__ ALevel s_lv[HD_MAX_LEVELS]; // the small levels' arrays and entry counts, read once
+	__shared__ u32 s_n[HD_MAX_LEVELS];
 	BottomLds &S = U.b;
 	const u32 L = P.c.L;
+	if (l_from >= 0) { // levels 0 .. l_from + 1 (the first small level's children may be large)
+		const u32 nw = min((u32)l_from + 2u, L) * (u32)(sizeof(ALevel) / 4);
+		for (u32 i = threadIdx.x; i < nw; i += blockDim.x)
+			reinterpret_cast<u32 *>(s_lv)[i] = reinterpret_cast<const u32 *>(lv)[i];
+		for (u32 i = threadIdx.x; i <= (u32)l_from; i += blockDim.x)
+			s_n[i] = min(pm[kPmHdr + i], kSmallLevel);
+	}
+	if (threadIdx.x == 0)
+		S.stage_top = 0;
+	__syncthreads();
+	JoinIn pre{};         // this thread's first entry of the level, loaded during the previous level
+	bool have_pre = false;
 	for (int l = l_from; l >= 0; --l) {
-		const ALevel A = lv[l];
-		const u32 n = min(pm[kPmHdr + l], kSmallLevel);
+		const ALevel A = s_lv[l];
+		const u32 n = s_n[l];
 		const bool leaf = (u32)l + 1 == L;
 		const u32 cur = l & 1, chl = (l + 1) & 1;
-		for (u32 i = threadIdx.x; i < kSTab; i += blockDim.x) {
+		// build (or take a_leaf's result) + the find among the bucket's existing nodes; rounds of
+		// blockDim entries, the stage reset between rounds
+		for (u32 e0 = 0; e0 < n; e0 += blockDim.x) {
+			const u32 e = e0 + threadIdx.x;
+			if (e < n) {
+				u32 fin = kNull, optr = kNull;
+				bool need = false;
+				if (leaf) {
+					if (!A.pend[e])
+						fin = A.status[e];
+					else if ((fin = A.res[e]) == kNull) {
+						need = true;
+						u32 c0 = A.content[2 * (u64)e], c1 = A.content[2 * (u64)e + 1], b = A.bucket[e];
+						S.content[e][0] = c0;
+						S.content[e][1] = c1;
+						S.csz[e] = 2;
+						S.hash[e] = murmur_leaf(c0, c1);
+						S.bucket[e] = b;
+						S.bw[e] = P.bucket_words[b];
+						optr = A.ent[e].ptr;
+					}
+				} else {
+					const bool csmall = l < l_from;
+					const JoinIn q = (have_pre && e0 == 0) ? pre : load_join(A, e);
+					Built o = csmall ? build_node(P, (u32)l, q, [&](u32 c) { return lds_result(S, chl, c); })
+					                 : build_node(P, (u32)l, q, [&](u32 c) { return child_result(s_lv[l + 1], G, c); });
+					fin = o.fin;
+					if (o.k) {
+						u32 bw;
+						fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, bw, S.stage,
+						                            &S.stage_top, kStage);
+						if (fin == kNull) {
+							need = true;
+#pragma unroll
+							for (u32 j = 0; j < 9; ++j)
+								if (j < o.k)
+									S.content[e][j] = o.c[j];
+							S.csz[e] = o.k;
+							S.hash[e] = o.h;
+							S.bucket[e] = o.b;
+							S.bw[e] = bw;
+							optr = q.ptr;
+						}
+					}
+				}
+				S.res[cur][e] = fin;
+				S.slot[cur][e] = need ? 0u : kNull;
+				S.ptr[cur][e] = optr;
+			}
+			if (e0 + blockDim.x < n) {
+				__syncthreads();
+				if (threadIdx.x == 0)
+					S.stage_top = 0;
+				__syncthreads();
+			}
+		}
+		// the next level's first entry of this thread: its loads overlap this level's dedup and walk
+		have_pre = l > 0 && threadIdx.x < s_n[l - 1];
+		if (have_pre)
+			pre = load_join(s_lv[l - 1], threadIdx.x);
+		__syncthreads();
+		for (u32 i = threadIdx.x; i < kSTab; i += blockDim.x) { // the stage becomes the dedup tables
 			S.tab[i] = kNull;
 			S.bkey[i] = kNull;
 			S.bmin[i] = kNull;
 			S.bhead[i] = kNull;
 			S.bn[i] = 0;
 		}
-		if (threadIdx.x == 0)
+		if (threadIdx.x == 0) {
 			S.gcount = 0;
-		// build (or take a_leaf's result) + the find among the bucket's existing nodes
-		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			u32 fin = kNull, optr = kNull;
-			bool need = false;
-			if (leaf) {
-				if (!A.pend[e])
-					fin = A.status[e];
-				else if ((fin = A.res[e]) == kNull) {
-					need = true;
-					u32 c0 = A.content[2 * (u64)e], c1 = A.content[2 * (u64)e + 1], b = A.bucket[e];
-					S.content[e][0] = c0;
-					S.content[e][1] = c1;
-					S.csz[e] = 2;
-					S.hash[e] = murmur_leaf(c0, c1);
-					S.bucket[e] = b;
-					S.bw[e] = P.bucket_words[b];
-					optr = A.ent[e].ptr;
-				}
-			} else {
-				const ALevel C = lv[l + 1];
-				const bool csmall = l < l_from;
-				const Entry E = A.ent[e];
-				Built o = build_node(P, (u32)l, A, e, E, [&](u32 c) {
-					return csmall ? (c < kSmallLevel ? lds_result(S, chl, c) : kNull) : child_result(C, G, c);
-				});
-				fin = o.fin;
-				if (o.k) {
-					u32 bw;
-					fin = find_in_bucket_win(P.words, P.bucket_words, P.c, o.b, o.c, o.k, false, bw);
-					if (fin == kNull) {
-						need = true;
-						for (u32 j = 0; j < o.k; ++j)
-							S.content[e][j] = o.c[j];
-						S.csz[e] = o.k;
-						S.hash[e] = o.h;
-						S.bucket[e] = o.b;
-						S.bw[e] = bw;
-						optr = E.ptr;
-					}
-				}
-			}
-			S.res[cur][e] = fin;
-			S.slot[cur][e] = need ? 0u : kNull;
-			S.ptr[cur][e] = optr;
+			S.stage_top = 0;
 		}
 		__syncthreads();
 		// dedup (identical contents join the slot of their lowest entry; a slot's creator lists the
