@@ -353,6 +353,26 @@ __host__ __device__ inline u32 morton_part(u32 u) {
 __host__ __device__ inline u32 morton3(u32 x, u32 y, u32 z) { return morton_part(x) | (morton_part(y) << 1) | (morton_part(z) << 2); }
 
 
+// Loads and stores through a pointer read from memory (a struct field): the compiler cannot tell
+// its address space and would emit flat instructions, which also count against the LDS wait
+// counter (an LDS wait then waits for outstanding global loads too). These address global memory
+// explicitly.
+// (The host compilation pass only checks these device bodies; it sees plain accesses.)
+template <typename T> __device__ inline T ldg(const T *p, u64 i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return ((const __attribute__((address_space(1))) T *)p)[i];
+#else
+	return p[i];
+#endif
+}
+template <typename T> __device__ inline void stg(T *p, u64 i, T v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	((__attribute__((address_space(1))) T *)p)[i] = v;
+#else
+	p[i] = v;
+#endif
+}
+
 // ---------------------------------------------------------------- pool find / append
 // find_node (NodePool.hpp:79-132): scan the bucket's used words node by node; inner-node size
 // comes from the mask word, a zero word is page-tail padding (skip to the next page); leaf
@@ -479,14 +499,50 @@ __device__ inline u32 find_in_global(const u32 *__restrict__ words, const DevCon
 	return find_in_words(words + base, c, base, bw, node, n);
 }
 
+// Scan of staged bucket words where every word equal to node[0] starts a node: inner-node headers
+// are child masks (< 256), the level's child pointers are all >= 256 and page padding is 0 while
+// node[0] != 0. The first position whose n words equal the node is then the first equal node in
+// bucket order (find_node's answer), and the scan needs no parse chain: 16 words per step.
+// Reads up to 15 words past bw (inside the LDS allocation; those positions are masked off).
+__device__ inline u32 find_in_words_hv(const u32 *T, u64 base, u32 bw, const u32 (&node)[9], u32 n) {
+	for (u32 i = 0; i < bw; i += 16) {
+		const uint4 *T4 = reinterpret_cast<const uint4 *>(T + i);
+		const uint4 a = T4[0], b = T4[1], c = T4[2], d = T4[3];
+		const u32 w[16] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x, d.y, d.z, d.w};
+		u32 cand = 0;
+#pragma unroll
+		for (u32 k = 0; k < 16; ++k)
+			cand |= (w[k] == node[0] ? 1u : 0u) << k;
+		if (bw - i < 16u)
+			cand &= (1u << (bw - i)) - 1u;
+		while (cand) {
+			const u32 at = i + (u32)__builtin_ctz(cand);
+			cand &= cand - 1u;
+			bool eq = true;
+#pragma unroll
+			for (u32 k = 1; k < 9; ++k)
+				if (k < n)
+					eq = eq && T[at + k] == node[k];
+			if (eq)
+				return (u32)(base + at);
+		}
+	}
+	return kNull;
+}
+
+// Status of find_in_bucket_staged when the workgroup's stage had no room left: the caller retries
+// after the stage is reset (word indices of a pool stay below it).
+constexpr u32 kFindRetry = 0xFFFFFFFEu;
+
 // find_node for an inner node with an LDS staging area shared by a workgroup: the bucket's first
 // 64 words come with its word count in one round trip, the rest (if any) in one more round trip
-// of independent 64-word batches; the used words are copied to stage[*top ..] (bump-allocated,
-// cap words, reset by the caller between uses) and scanned there. A bucket that does not fit in
-// the remaining stage is scanned in global memory. Buckets of fewer than 64 words per bucket
-// take find_in_bucket_win.
+// of independent 64-word batches. A bucket of at most 16 words is scanned in registers; a longer
+// one is copied to stage[*top ..] (bump-allocated, cap words) and scanned there -- by header value
+// when hv (find_in_words_hv), else node by node. Returns kFindRetry when the stage is full (a
+// bucket larger than the whole stage is scanned in global memory). Configurations with buckets
+// of fewer than 64 words take find_in_bucket_win.
 __device__ inline u32 find_in_bucket_staged(const u32 *__restrict__ words, const u32 *__restrict__ bucket_words,
-                                            const DevConfig &c, u32 bucket, const u32 (&node)[9], u32 n,
+                                            const DevConfig &c, u32 bucket, const u32 (&node)[9], u32 n, bool hv,
                                             u32 &bw_out, u32 *stage, u32 *top, u32 cap) {
 	if (c.wbpb() < 6)
 		return find_in_bucket_win(words, bucket_words, c, bucket, node, n, false, bw_out);
@@ -535,9 +591,11 @@ __device__ inline u32 find_in_bucket_staged(const u32 *__restrict__ words, const
 		return kNull;
 	}
 	const u32 len = (bw + 3u) & ~3u;
+	if (len > cap)
+		return find_in_global(words, c, base, bw, node, n);
 	const u32 off = atomicAdd(top, len);
 	if (off + len > cap)
-		return find_in_global(words, c, base, bw, node, n);
+		return kFindRetry;
 	uint4 *T4 = reinterpret_cast<uint4 *>(stage + off);
 #pragma unroll
 	for (u32 k = 0; k < 16; ++k)
@@ -553,8 +611,14 @@ __device__ inline u32 find_in_bucket_staged(const u32 *__restrict__ words, const
 			if (i + 4u * k < len)
 				T4[(i >> 2) + k] = t[k];
 	}
-	return find_in_words(stage + off, c, base, bw, node, n);
+	return hv ? find_in_words_hv(stage + off, base, bw, node, n) : find_in_words(stage + off, c, base, bw, node, n);
 }
+// Whether every child pointer stored at level l (pointers into level l + 1) is >= 256, so inner
+// node headers (< 256) can be told apart by value (find_in_words_hv).
+__device__ inline bool headers_by_value(const DevConfig &c, u32 l) {
+	return l + 1u < c.L && ((u64)c.level_base[l + 1u] << c.wbpb()) >= 256u;
+}
+
 // append_node (NodePool.hpp:134-157): full -> kNull; a node that would straddle a page zero-fills
 // the page tail and starts the next page. Updates bw.
 __device__ inline u32 append_to_bucket(u32 *__restrict__ words, const DevConfig &c, u32 bucket, u32 &bw,
@@ -573,6 +637,28 @@ __device__ inline u32 append_to_bucket(u32 *__restrict__ words, const DevConfig 
 	u64 at = ((u64)bucket << c.wbpb()) + rel;
 	for (u32 k = 0; k < n; ++k)
 		words[at + k] = node[k];
+	bw = rel + n;
+	return (u32)at;
+}
+// append_to_bucket for a node held in registers (static indices).
+__device__ inline u32 append_regs(u32 *__restrict__ words, const DevConfig &c, u32 bucket, u32 &bw,
+                                  const u32 (&node)[9], u32 n) {
+	if (bw + n > c.words_per_bucket())
+		return kNull;
+	u32 slot = bw >> c.wbpp, off = bw & (c.words_per_page() - 1u);
+	if (off + n > c.words_per_page()) {
+		u64 pg = ((u64)bucket << c.wbpb()) + ((u64)slot << c.wbpp);
+		for (u32 k = off; k < c.words_per_page(); ++k)
+			words[pg + k] = 0u;
+		++slot;
+		off = 0;
+	}
+	u32 rel = (slot << c.wbpp) | off;
+	u64 at = ((u64)bucket << c.wbpb()) + rel;
+#pragma unroll
+	for (u32 k = 0; k < 9; ++k)
+		if (k < n)
+			words[at + k] = node[k];
 	bw = rel + n;
 	return (u32)at;
 }

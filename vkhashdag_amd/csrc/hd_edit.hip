@@ -1248,9 +1248,10 @@ __device__ inline u32 child_result(const ALevel &C, const GroupTable &G, u32 c) 
 	// every load is issued unconditionally (indices clamped into their arrays) so the lookups of a
 	// node's children overlap: two dependent round trips in all
 	const u32 cc = min(c, C.cap - 1u);
-	const u32 pend = C.pend[cc], st = C.status[cc], r = C.res[cc], s = C.slot[cc], pt = C.ent[cc].ptr;
+	const u32 pend = ldg(C.pend, cc), st = ldg(C.status, cc), r = ldg(C.res, cc), s = ldg(C.slot, cc);
+	const u32 pt = ldg(&C.ent[cc].ptr, 0);
 	const u32 si = s & G.mask;
-	const u32 gf = G.gfail[si], gr = G.gres[si];
+	const u32 gf = ldg(G.gfail, si), gr = ldg(G.gres, si);
 	// not pending: status; found among the bucket's existing nodes: res; else the group's result
 	// (the old pointer when the bucket was full)
 	const u32 v = !pend ? st : (r != kNull ? r : (gf ? pt : gr));
@@ -1330,52 +1331,57 @@ struct Built {
 	u32 k, h, b, c[9];
 };
 // The words of an inner entry that its join reads: old pointer, proceed mask, first proceeding
-// child, the 8 old / terminal child pointers (loaded ahead of the join where latency allows).
+// child, the 8 old / terminal child pointers, and its place in its parent (parent entry << 3 |
+// slot); loaded ahead of the join where latency allows.
 struct JoinIn {
-	u32 ptr, pmw, fc;
+	u32 ptr, pmw, fc, pidx;
 	uint4 k0, k1;
 };
 __device__ inline JoinIn load_join(const ALevel &A, u32 e) {
 	JoinIn q;
-	q.ptr = A.ent[e].ptr;
-	q.pmw = A.pmask[e];
-	q.fc = A.first_child[e];
-	q.k0 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e];
-	q.k1 = reinterpret_cast<const uint4 *>(A.kids)[2 * (u64)e + 1];
+	q.ptr = ldg(&A.ent[e].ptr, 0);
+	q.pidx = (ldg(&A.ent[e].parent, 0) << 3) | (ldg(&A.ent[e].sf, 0) & 7u);
+	q.pmw = ldg(A.pmask, e);
+	q.fc = ldg(A.first_child, e);
+	q.k0 = ldg(reinterpret_cast<const uint4 *>(A.kids), 2 * (u64)e);
+	q.k1 = ldg(reinterpret_cast<const uint4 *>(A.kids), 2 * (u64)e + 1);
 	return q;
 }
+// The results of an entry's proceeding children, by slot: they are entries fc .. fc + cnt - 1 of
+// the next level in slot order; all lookups are issued before any result is used.
 template <typename ChildFn>
-__device__ inline Built build_node(const EditParams &P, u32 l, const JoinIn &q, ChildFn child) {
+__device__ inline void children_by_slot(const JoinIn &q, ChildFn child, u32 (&cs)[8]) {
+	const u32 pmk = q.pmw & 0xFFu, cnt = (u32)__builtin_popcount(pmk);
+	u32 cr[8];
+#pragma unroll
+	for (u32 j = 0; j < 8; ++j)
+		cr[j] = cnt ? child(q.fc + min(j, cnt - 1u)) : kNull;
+#pragma unroll
+	for (u32 i = 0; i < 8; ++i) {
+		const u32 rank = (u32)__builtin_popcount(pmk & ((1u << i) - 1u));
+		u32 c = cr[0];
+#pragma unroll
+		for (u32 j = 1; j <= i; ++j)
+			c = rank == j ? cr[j] : c;
+		cs[i] = c;
+	}
+}
+// The join itself: slot i takes cs[i] when the child proceeded (pmask bit i), else its old or
+// terminal pointer; pmask bits 8-15 flag the terminal children whose pointer changed.
+__device__ inline Built build_node(const EditParams &P, u32 l, const JoinIn &q, const u32 (&cs)[8]) {
 	Built o;
-	const u32 pmw = q.pmw, pmk = pmw & 0xFFu, fc = q.fc;
+	const u32 pmw = q.pmw, pmk = pmw & 0xFFu;
 	const uint4 k0 = q.k0, k1 = q.k1;
 	const u32 kv[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
 	bool changed = (pmw >> 8) != 0;
-	// the proceeding children are entries fc .. fc + cnt - 1 of the next level: look them all up
-	// before using any result, so their loads overlap
-	const u32 cnt = (u32)__builtin_popcount(pmk);
-	u32 cr[8];
-	if (cnt) {
-#pragma unroll
-		for (u32 j = 0; j < 8; ++j)
-			cr[j] = child(fc + min(j, cnt - 1u));
-	}
 	u32 nm = 0, nk[8];
 #pragma unroll
 	for (u32 i = 0; i < 8; ++i) {
-		u32 r = kv[i];
-		if ((pmk >> i) & 1u) {
-			const u32 rank = (u32)__builtin_popcount(pmk & ((1u << i) - 1u));
-			u32 c = cr[0];
-#pragma unroll
-			for (u32 j = 1; j <= i; ++j)
-				c = rank == j ? cr[j] : c;
-			changed |= c != r;
-			r = c;
-		}
+		const bool pr = (pmk >> i) & 1u;
+		const u32 r = pr ? cs[i] : kv[i];
+		changed |= pr && r != kv[i];
 		nk[i] = r;
-		if (r != kNull)
-			nm |= 1u << i;
+		nm |= (r != kNull ? 1u : 0u) << i;
 	}
 	o.need = false;
 	o.k = 0;
@@ -1393,23 +1399,32 @@ __device__ inline Built build_node(const EditParams &P, u32 l, const JoinIn &q, 
 
 // edit_node's join (k_build) for entry e of level l, with the children's results resolved from
 // level l + 1, then the find among the bucket's existing nodes.
-__device__ inline void build_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, const GroupTable &G,
+// Returns false when the workgroup's stage was full (the entry is built again after a reset).
+__device__ inline bool build_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, const GroupTable &G,
                                    u32 e, u32 *stage, u32 *top, u32 cap) {
-	Built o = build_node(P, l, load_join(A, e), [&](u32 c) { return child_result(C, G, c); });
+	const JoinIn q = load_join(A, e);
+	u32 cs[8];
+	children_by_slot(q, [&](u32 c) { return child_result(C, G, c); }, cs);
+	Built o = build_node(P, l, q, cs);
 	if (!o.k) {
-		A.pend[e] = 0;
-		A.status[e] = o.fin;
-		return;
+		stg(A.pend, e, 0u);
+		stg(A.status, e, o.fin);
+		return true;
 	}
+	u32 bw;
+	const u32 f = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, headers_by_value(P.c, l), bw,
+	                                    stage, top, cap);
+	if (f == kFindRetry)
+		return false;
 #pragma unroll
 	for (u32 j = 0; j < 9; ++j)
 		if (j < o.k)
-			A.content[(u64)e * 9u + j] = o.c[j];
-	A.pend[e] = 1;
-	A.csize[e] = o.k;
-	A.bucket[e] = o.b;
-	u32 bw;
-	A.res[e] = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, bw, stage, top, cap);
+			stg(A.content, (u64)e * 9u + j, o.c[j]);
+	stg(A.pend, e, 1u);
+	stg(A.csize, e, o.k);
+	stg(A.bucket, e, o.b);
+	stg(A.res, e, f);
+	return true;
 }
 
 // Joins entry e (pending, not in the pool) to its content's group; the creator queues the group.
@@ -1467,9 +1482,12 @@ __device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, c
 		const u32 j = rep_of(s);
 		const u32 *node = A.content + (u64)j * stride;
 		u32 nsz = leaf ? 2u : A.csize[j];
-		u32 f = find_in_bucket_from(P.words, P.c, b, bw0, bw, node, nsz, leaf);
+		// the groups of a level hold pairwise distinct contents, so no group can match a node
+		// appended by an earlier group of this walk: the find among the existing nodes (the build)
+		// was the whole find
+		u32 f = kNull;
 		u32 fail = 0;
-		if (f == kNull) {
+		{
 			f = append_to_bucket(P.words, P.c, b, bw, node, nsz);
 			if (f == kNull)
 				fail = 1, any_fail = true;
@@ -1526,13 +1544,15 @@ __global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel
 	const ALevel A = lv[l], C = lv[l + 1];
 	const u32 n = pm[kPmHdr + l];
 	for (u32 e0 = blockIdx.x * blockDim.x; e0 < n; e0 += gridDim.x * blockDim.x) { // uniform trip count
-		if (threadIdx.x == 0)
-			top = 0;
-		__syncthreads();
 		const u32 e = e0 + threadIdx.x;
-		if (e < n)
-			build_entry(P, l, A, C, G, e, stage, &top, kCap);
-		__syncthreads();
+		bool todo = e < n;
+		do { // an entry whose bucket found the stage full is built again after the reset
+			if (threadIdx.x == 0)
+				top = 0;
+			__syncthreads();
+			if (todo)
+				todo = !build_entry(P, l, A, C, G, e, stage, &top, kCap);
+		} while (__syncthreads_or(todo));
 	}
 }
 __global__ void __launch_bounds__(kAT) a_dedup(EditParams P, u32 l, const ALevel *__restrict__ lv,
@@ -1900,6 +1920,7 @@ __device__ inline void setnode_block(const EditParams &P, const ALevel *__restri
 // ---- small levels in one workgroup, with the level bookkeeping in LDS
 constexpr u32 kSmallLevel = 1024; // entries of a level handled inside a_bottom
 constexpr u32 kSTab = 2048;       // LDS dedup / bucket-map slots (load factor <= 1/2)
+static_assert(kSTab == 1u << 11 && kSmallLevel * 2u <= kSTab);
 constexpr u32 kColorLds = 4096;   // SetNode entries (all levels above the color leaves) kept in LDS
 
 constexpr u32 kStage = 5 * kSTab; // bucket staging words of the build (shares the dedup tables' LDS)
@@ -1912,6 +1933,7 @@ struct BottomLds {
 	u32 content[kSmallLevel][9];
 	u32 csz[kSmallLevel], hash[kSmallLevel], bucket[kSmallLevel], bw[kSmallLevel], bmap[kSmallLevel];
 	u32 glist[kSmallLevel], gnext[kSmallLevel]; // groups (their slots) and the per-bucket group lists
+	u32 pslot[kSmallLevel];                     // parent entry << 3 | slot in the parent
 	u32 gres[kSTab];                            // result per dedup slot (read by the parent's build)
 	union {
 		struct { // dedup and walk: cleared after the build
@@ -1923,6 +1945,10 @@ struct BottomLds {
 	unsigned char gfail[kSTab];
 	u32 gcount, stage_top;
 };
+// the table clear writes the stage as uint4: tab, bkey, bmin, bhead (kNull) then bn (0)
+static_assert(offsetof(BottomLds, tab) == offsetof(BottomLds, stage) &&
+              offsetof(BottomLds, bn) == offsetof(BottomLds, stage) + 4 * kSTab * sizeof(u32) &&
+              offsetof(BottomLds, stage) % 16 == 0);
 struct ColorLds {
 	u32 r[kColorLds], cnt[kColorLds], base[kColorLds];
 	u32 off[HD_MAX_LEVELS + 1];
@@ -1941,12 +1967,10 @@ __device__ inline bool lds_same(const BottomLds &S, u32 a, u32 b) {
 			return false;
 	return true;
 }
+// Final pointer of entry c of a finished small level (its results are resolved after the walk).
+// The load is unconditional (index clamped) so the lookups of a node's children overlap.
 __device__ inline u32 lds_result(const BottomLds &S, u32 p, u32 c) {
-	// loads unconditional (indices clamped) so a node's child lookups overlap
-	const u32 cc = min(c, kSmallLevel - 1u);
-	const u32 s = S.slot[p][cc], r = S.res[p][cc], pt = S.ptr[p][cc];
-	const u32 si = s & (kSTab - 1u);
-	const u32 v = s == kNull ? r : (S.gfail[si] ? pt : S.gres[si]);
+	const u32 v = S.res[p][min(c, kSmallLevel - 1u)];
 	return c < kSmallLevel ? v : kNull;
 }
 
@@ -1961,9 +1985,9 @@ __device__ inline void small_walk_bucket(const EditParams &P, bool leaf, BottomL
 		const u32 s2 = S.glist[g2], j = S.tab[s2];
 		const u32 *node = S.content[j];
 		u32 nsz = S.csz[j];
-		u32 f = bw > bw0 ? find_in_bucket_from(P.words, P.c, b, bw0, bw, node, nsz, leaf) : kNull;
+		u32 f = kNull; // distinct groups: nothing appended by this walk can match (see walk_entry)
 		unsigned char fail = 0;
-		if (f == kNull) {
+		{
 			f = append_to_bucket(P.words, P.c, b, bw, node, nsz);
 			if (f == kNull)
 				fail = 1, any_fail = true;
@@ -2020,10 +2044,10 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 		const u32 n = pm[kPmHdr + l], o = S.off[l], oc = S.off[l + 1];
 		const bool cl = (u32)l + 1 == leafL;
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			Entry E = A.ent[e];
+			Entry E = ldg(A.ent, e);
 			u32 r = E.octree, c = 0;
 			if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
-				const u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
+				const u32 m = ldg(A.pmask, e) & 0xFFu, fc = ldg(A.first_child, e);
 				u32 ch[8];
 				bool all_null = true, same_color;
 				for (u32 i = 0; i < 8; ++i) {
@@ -2032,13 +2056,13 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 						if (ci >= C.cap)
 							ch[i] = kColorNull;
 						else if (cl)
-							ch[i] = C.ores[ci];
+							ch[i] = ldg(C.ores, ci);
 						else {
 							ch[i] = S.r[oc + ci];
 							c += S.cnt[oc + ci];
 						}
 					} else
-						ch[i] = A.okids[(u64)e * 8u + i];
+						ch[i] = ldg(A.okids, (u64)e * 8u + i);
 					all_null = all_null && ctag(ch[i]) == kTagNull;
 				}
 				same_color = ctag(ch[0]) == kTagColor;
@@ -2070,7 +2094,7 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 		const bool cl = l + 1 == leafL;
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
 			u32 run = S.base[o + e], r = S.r[o + e];
-			const u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
+			const u32 m = ldg(A.pmask, e) & 0xFFu, fc = ldg(A.first_child, e);
 			u32 ch[8];
 			for (u32 i = 0; i < 8; ++i) {
 				if ((m >> i) & 1u) {
@@ -2078,19 +2102,19 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 					if (ci >= C.cap)
 						ch[i] = kColorNull;
 					else if (cl)
-						ch[i] = C.ores[ci];
+						ch[i] = ldg(C.ores, ci);
 					else {
 						u32 cr = S.r[oc + ci], cc = S.cnt[oc + ci];
 						S.base[oc + ci] = run;
 						run += cc;
 						if (cr == kColorNewNode) {
 							u32 idx = run - 1u;
-							cr = (u64)idx < node_cap ? cmake(kTagNode, idx) : C.ent[ci].octree;
+							cr = (u64)idx < node_cap ? cmake(kTagNode, idx) : ldg(&C.ent[ci].octree, 0);
 						}
 						ch[i] = cr;
 					}
 				} else
-					ch[i] = A.okids[(u64)e * 8u + i];
+					ch[i] = ldg(A.okids, (u64)e * 8u + i);
 			}
 			if (r == kColorNewNode) {
 				if ((u64)run < node_cap) {
@@ -2098,7 +2122,7 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 						P.cnodes[(u64)run * 8u + i] = ch[i];
 					r = cmake(kTagNode, run);
 				} else {
-					r = A.ent[e].octree;
+					r = ldg(&A.ent[e].octree, 0);
 					atomicOr(&P.meta[kMetaOverflow], 2u);
 				}
 			}
@@ -2113,6 +2137,119 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 		if (leafL == 0 && pm[kPmHdr] > 0)
 			roots[1] = lv[0].ores[0];
 		P.meta[kMetaColorLeaves] += pm[kPmLeafTotal];
+	}
+}
+
+// The levels [0, l_top] when each has at most 64 entries, by wave 0 alone (no workgroup barriers,
+// no LDS tables): lane e builds entry e as the workgroup path does; the lanes whose content is new
+// then form groups of identical contents, handled one by one in ascending order of their first
+// entries -- the order in which the workgroup walk appends a bucket's groups -- with the group's
+// first lane appending it and every member taking the result (its old pointer when the bucket is
+// full); each result goes to the parent's content row at its slot.
+__device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, const ALevel *s_lv, const u32 *s_n,
+                                   const GroupTable &G, BottomLds &S, u32 *__restrict__ acc, JoinIn pre, bool have_pre) {
+	const u32 lane = threadIdx.x, L = P.c.L;
+	for (int l = l_top; l >= 0; --l) {
+		const u32 n = s_n[l];
+		const ALevel A = s_lv[l];
+		const bool leaf = (u32)l + 1 == L, act = lane < n;
+		const u32 cur = l & 1;
+		u32 fin = kNull, optr = kNull, b = 0, bw = 0, h = 0, k = 0, pidx = 0;
+		u32 c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+		bool need = false;
+		if (act) {
+			if (leaf) {
+				pidx = (ldg(&A.ent[lane].parent, 0) << 3) | (ldg(&A.ent[lane].sf, 0) & 7u);
+				if (!ldg(A.pend, lane))
+					fin = ldg(A.status, lane);
+				else if ((fin = ldg(A.res, lane)) == kNull) {
+					need = true;
+					c[0] = ldg(A.content, 2 * (u64)lane);
+					c[1] = ldg(A.content, 2 * (u64)lane + 1);
+					b = ldg(A.bucket, lane);
+					k = 2;
+					h = murmur_leaf(c[0], c[1]);
+					bw = P.bucket_words[b];
+					optr = ldg(&A.ent[lane].ptr, 0);
+				}
+			} else {
+				const JoinIn q = have_pre ? pre : load_join(A, lane);
+				pidx = q.pidx;
+				u32 cs[8];
+				if (l < l_from) {
+#pragma unroll
+					for (u32 i = 0; i < 8; ++i)
+						cs[i] = S.content[lane][i];
+				} else
+					children_by_slot(q, [&](u32 ch) { return child_result(s_lv[l + 1], G, ch); }, cs);
+				Built o = build_node(P, (u32)l, q, cs);
+				fin = o.fin;
+				if (o.k) {
+					fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, headers_by_value(P.c, (u32)l),
+					                            bw, S.stage, &S.stage_top, kStage);
+					if (fin == kFindRetry) // the stage is full: scan in global memory
+						fin = find_in_global(P.words, P.c, (u64)o.b << P.c.wbpb(), bw, o.c, o.k);
+					if (fin == kNull) {
+						need = true;
+#pragma unroll
+						for (u32 j = 0; j < 9; ++j)
+							c[j] = o.c[j];
+						k = o.k;
+						h = o.h;
+						b = o.b;
+						optr = q.ptr;
+					}
+				}
+			}
+		}
+		// the next level's entries: their loads overlap this level's groups
+		have_pre = l > 0 && lane < s_n[l - 1];
+		if (have_pre)
+			pre = load_join(s_lv[l - 1], lane);
+		u64 pending = __ballot(need);
+		u32 appended = 0;
+		bool full = false;
+		while (pending) {
+			const u32 ld = (u32)__builtin_ctzll(pending);
+			const u32 lh = __builtin_amdgcn_readlane(h, ld), lk = __builtin_amdgcn_readlane(k, ld);
+			const u32 lb = __builtin_amdgcn_readlane(b, ld);
+			bool same = need && h == lh && k == lk;
+#pragma unroll
+			for (u32 j = 0; j < 9; ++j)
+				same = same && c[j] == __builtin_amdgcn_readlane(c[j], ld);
+			const u64 members = __ballot(same);
+			u32 f = kNull, nbw = bw;
+			if (lane == ld) {
+				f = append_regs(P.words, P.c, lb, nbw, c, k);
+				P.bucket_words[lb] = nbw;
+			}
+			f = __builtin_amdgcn_readlane(f, ld);
+			nbw = __builtin_amdgcn_readlane(nbw, ld);
+			if (f == kNull)
+				full = true;
+			else
+				++appended;
+			if (same)
+				fin = f == kNull ? optr : f;
+			if (need && b == lb)
+				bw = nbw; // later groups of this bucket append after it
+			pending &= ~members;
+		}
+		if (lane == 0) {
+			if (appended)
+				atomicAdd(&acc[kAccUpserts], appended);
+			if (full)
+				atomicOr(&P.meta[kMetaOverflow], 1u);
+		}
+		if (act) {
+			S.res[cur][lane] = fin;
+			if (l > 0 && (pidx >> 3) < kSmallLevel)
+				S.content[pidx >> 3][pidx & 7u] = fin;
+		}
+		S.stage_top = 0; // every lane writes the same value; the next level allocates from 0
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	}
 }
 
@@ -2140,77 +2277,108 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 	JoinIn pre{};         // this thread's first entry of the level, loaded during the previous level
 	bool have_pre = false;
 	for (int l = l_from; l >= 0; --l) {
+		bool tiny = true; // this level and all above have at most 64 entries: wave 0 finishes alone
+		for (int i = 0; i <= l; ++i)
+			tiny = tiny && s_n[i] <= 64u;
+		if (tiny) {
+			if (threadIdx.x < 64)
+				tiny_levels(P, l, l_from, s_lv, s_n, G, S, acc, pre, have_pre);
+			__syncthreads();
+			break;
+		}
 		const ALevel A = s_lv[l];
 		const u32 n = s_n[l];
 		const bool leaf = (u32)l + 1 == L;
-		const u32 cur = l & 1, chl = (l + 1) & 1;
-		// build (or take a_leaf's result) + the find among the bucket's existing nodes; rounds of
-		// blockDim entries, the stage reset between rounds
+		const bool hv = headers_by_value(P.c, (u32)l);
+		const u32 cur = l & 1;
+		// build (or take a_leaf's result) + the find among the bucket's existing nodes, in rounds of
+		// blockDim entries; an entry whose bucket found the stage full is built again after a reset
 		for (u32 e0 = 0; e0 < n; e0 += blockDim.x) {
 			const u32 e = e0 + threadIdx.x;
-			if (e < n) {
-				u32 fin = kNull, optr = kNull;
-				bool need = false;
-				if (leaf) {
-					if (!A.pend[e])
-						fin = A.status[e];
-					else if ((fin = A.res[e]) == kNull) {
-						need = true;
-						u32 c0 = A.content[2 * (u64)e], c1 = A.content[2 * (u64)e + 1], b = A.bucket[e];
-						S.content[e][0] = c0;
-						S.content[e][1] = c1;
-						S.csz[e] = 2;
-						S.hash[e] = murmur_leaf(c0, c1);
-						S.bucket[e] = b;
-						S.bw[e] = P.bucket_words[b];
-						optr = A.ent[e].ptr;
-					}
-				} else {
-					const bool csmall = l < l_from;
-					const JoinIn q = (have_pre && e0 == 0) ? pre : load_join(A, e);
-					Built o = csmall ? build_node(P, (u32)l, q, [&](u32 c) { return lds_result(S, chl, c); })
-					                 : build_node(P, (u32)l, q, [&](u32 c) { return child_result(s_lv[l + 1], G, c); });
-					fin = o.fin;
-					if (o.k) {
-						u32 bw;
-						fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, bw, S.stage,
-						                            &S.stage_top, kStage);
-						if (fin == kNull) {
+			bool todo = e < n;
+			for (;;) {
+				if (todo) {
+					u32 fin = kNull, optr = kNull;
+					bool need = false;
+					if (leaf) {
+						const u32 par = ldg(&A.ent[e].parent, 0), sf = ldg(&A.ent[e].sf, 0);
+						S.pslot[e] = (par << 3) | (sf & 7u);
+						if (!ldg(A.pend, e))
+							fin = ldg(A.status, e);
+						else if ((fin = ldg(A.res, e)) == kNull) {
 							need = true;
-#pragma unroll
-							for (u32 j = 0; j < 9; ++j)
-								if (j < o.k)
-									S.content[e][j] = o.c[j];
-							S.csz[e] = o.k;
-							S.hash[e] = o.h;
-							S.bucket[e] = o.b;
-							S.bw[e] = bw;
-							optr = q.ptr;
+							u32 c0 = ldg(A.content, 2 * (u64)e), c1 = ldg(A.content, 2 * (u64)e + 1);
+							u32 b = ldg(A.bucket, e);
+							S.content[e][0] = c0;
+							S.content[e][1] = c1;
+							S.csz[e] = 2;
+							S.hash[e] = murmur_leaf(c0, c1);
+							S.bucket[e] = b;
+							S.bw[e] = P.bucket_words[b];
+							optr = ldg(&A.ent[e].ptr, 0);
 						}
+						todo = false;
+					} else {
+						const JoinIn q = (have_pre && e0 == 0) ? pre : load_join(A, e);
+						u32 cs[8];
+						if (l < l_from) { // the children's results sit in this entry's content row, by slot
+#pragma unroll
+							for (u32 i = 0; i < 8; ++i)
+								cs[i] = S.content[e][i];
+						} else
+							children_by_slot(q, [&](u32 c) { return child_result(s_lv[l + 1], G, c); }, cs);
+						Built o = build_node(P, (u32)l, q, cs);
+						fin = o.fin;
+						todo = false;
+						if (o.k) {
+							u32 bw;
+							fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, hv, bw, S.stage,
+							                            &S.stage_top, kStage);
+							if (fin == kFindRetry)
+								todo = true;
+							else if (fin == kNull) {
+								need = true;
+#pragma unroll
+								for (u32 j = 0; j < 9; ++j)
+									if (j < o.k)
+										S.content[e][j] = o.c[j];
+								S.csz[e] = o.k;
+								S.hash[e] = o.h;
+								S.bucket[e] = o.b;
+								S.bw[e] = bw;
+								optr = q.ptr;
+							}
+						}
+						if (!todo)
+							S.pslot[e] = q.pidx;
+					}
+					if (!todo) {
+						S.res[cur][e] = fin;
+						S.slot[cur][e] = need ? 0u : kNull;
+						S.ptr[cur][e] = optr;
 					}
 				}
-				S.res[cur][e] = fin;
-				S.slot[cur][e] = need ? 0u : kNull;
-				S.ptr[cur][e] = optr;
-			}
-			if (e0 + blockDim.x < n) {
-				__syncthreads();
+				const bool again = __syncthreads_or(todo);
+				if (!again && e0 + blockDim.x >= n)
+					break;
 				if (threadIdx.x == 0)
 					S.stage_top = 0;
 				__syncthreads();
+				if (!again)
+					break;
 			}
 		}
 		// the next level's first entry of this thread: its loads overlap this level's dedup and walk
 		have_pre = l > 0 && threadIdx.x < s_n[l - 1];
 		if (have_pre)
 			pre = load_join(s_lv[l - 1], threadIdx.x);
-		__syncthreads();
-		for (u32 i = threadIdx.x; i < kSTab; i += blockDim.x) { // the stage becomes the dedup tables
-			S.tab[i] = kNull;
-			S.bkey[i] = kNull;
-			S.bmin[i] = kNull;
-			S.bhead[i] = kNull;
-			S.bn[i] = 0;
+		// the stage becomes the dedup tables (tab, bkey, bmin, bhead = kNull; bn = 0), sized for
+		// the level: tsz slots, a power of two >= 2 n
+		const u32 tlg = min(11u, max(6u, 32u - (u32)__builtin_clz(max(2u * n, 2u) - 1u))); // kSTab = 1 << 11
+		const u32 tsz = 1u << tlg;
+		for (u32 i = threadIdx.x; i < 5u * tsz; i += blockDim.x) {
+			const u32 a = i >> tlg, k = i & (tsz - 1u);
+			S.stage[a * kSTab + k] = a < 4u ? kNull : 0u;
 		}
 		if (threadIdx.x == 0) {
 			S.gcount = 0;
@@ -2219,41 +2387,55 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 		__syncthreads();
 		// dedup (identical contents join the slot of their lowest entry; a slot's creator lists the
 		// group on its bucket) and the bucket's walker election (its lowest new entry)
+		const u32 tmask = tsz - 1u;
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
 			if (S.slot[cur][e] == kNull)
 				continue;
 			const u32 h = S.hash[e], b = S.bucket[e];
-			u32 m = (b * 0x9E3779B1u >> 7) & (kSTab - 1u);
+			u32 m = (b * 0x9E3779B1u >> 7) & tmask;
 			for (;;) {
 				u32 k = atomicCAS(&S.bkey[m], kNull, b);
 				if (k == kNull || k == b)
 					break;
-				m = (m + 1u) & (kSTab - 1u);
+				m = (m + 1u) & tmask;
 			}
 			atomicMin(&S.bmin[m], e);
 			S.bmap[e] = m;
-			u32 s = ((h ^ (h >> 15)) * 0x2c1b3c6du) & (kSTab - 1u);
+			u32 sl = ((h ^ (h >> 15)) * 0x2c1b3c6du) & tmask;
 			for (;;) {
-				u32 c = atomicCAS(&S.tab[s], kNull, e);
+				u32 c = atomicCAS(&S.tab[sl], kNull, e);
 				if (c == kNull) {
 					u32 g = atomicAdd(&S.gcount, 1u);
-					S.glist[g] = s;
+					S.glist[g] = sl;
 					S.gnext[g] = atomicExch(&S.bhead[m], g);
 					atomicAdd(&S.bn[m], 1u);
 					break;
 				}
 				if (S.hash[c] == h && lds_same(S, c, e)) {
-					atomicMin(&S.tab[s], e);
+					atomicMin(&S.tab[sl], e);
 					break;
 				}
-				s = (s + 1u) & (kSTab - 1u);
+				sl = (sl + 1u) & tmask;
 			}
-			S.slot[cur][e] = s;
+			S.slot[cur][e] = sl;
 		}
 		__syncthreads();
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x)
 			if (S.slot[cur][e] != kNull && S.bmin[S.bmap[e]] == e)
 				small_walk_bucket(P, leaf, S, e, acc);
+		__syncthreads();
+		// resolve: every entry's final pointer (its group's result; the old pointer when the bucket
+		// was full), written to its parent's content row at its slot for the parent's join
+		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
+			const u32 sl = S.slot[cur][e];
+			const u32 r = sl == kNull ? S.res[cur][e] : (S.gfail[sl] ? S.ptr[cur][e] : S.gres[sl]);
+			S.res[cur][e] = r;
+			if (l > 0) {
+				const u32 ps = S.pslot[e];
+				if ((ps >> 3) < kSmallLevel)
+					S.content[ps >> 3][ps & 7u] = r;
+			}
+		}
 		__syncthreads();
 	}
 	if (threadIdx.x == 0) {
