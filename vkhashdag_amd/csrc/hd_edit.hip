@@ -2449,10 +2449,14 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 			acc[kAccJobs] += pm[kPmHdr + P.c.color_leaf];
 		acc[kAccOverflow] |= pm[kPmOverflow];
 	}
-	if (!P.stateful)
-		return;
-	__syncthreads(); // the LDS now holds SetNode's state
-	ColorLds &Cs = U.c;
+}
+
+// SetNode of a colored pass (VBREditorWrapper's joins above the color leaves), one workgroup. It
+// needs the top-down entries and the encoded color leaves only, so it runs on the color stream
+// beside the node levels.
+__global__ void __launch_bounds__(kAT) a_setnode(EditParams P, const ALevel *__restrict__ lv,
+                                                 const u32 *__restrict__ pm, u32 *__restrict__ roots, u64 node_cap) {
+	__shared__ ColorLds Cs;
 	if (threadIdx.x == 0) {
 		u32 o = 0;
 		for (u32 l = 0; l <= P.c.color_leaf; ++l) {
@@ -3138,6 +3142,16 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
 	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= kSmallLevel)
 		++l_from;
+	if (colors) { // the leaf encode and SetNode on the color stream, beside the node levels
+		HD_TRY(hipEventRecord(p.ev_leaf, p.stream));
+		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_leaf, 0));
+		u32 G = grid_for(pp.cap[leafL], 1, 8192);
+		a_encode_size<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app);
+		a_scan_app<<<1, kAT, 0, p.cstream>>>(leafL, B.pm, B.app, B.appo);
+		a_encode_write<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap);
+		a_setnode<<<1, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
+		HD_TRY(hipEventRecord(p.ev_color, p.cstream));
+	}
 	for (int l = (int)L - 1; l > l_from; --l) {
 		u32 G = grid_for(pp.cap[l], kAT, 4096);
 		if (l + 1 < (int)L)
@@ -3145,13 +3159,9 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		a_dedup<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head);
 		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.acc);
 	}
-	if (colors) {
-		u32 G = grid_for(pp.cap[leafL], 1, 8192);
-		a_encode_size<<<G, kEncThreads, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app);
-		a_scan_app<<<1, kAT, 0, p.stream>>>(leafL, B.pm, B.app, B.appo);
-		a_encode_write<<<G, kEncThreads, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap);
-	}
 	a_bottom<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, l_from, B.G, B.roots, B.acc, p.cnode_cap);
+	if (colors) // the next pass (and any host read) follows both streams
+		HD_TRY(hipStreamWaitEvent(p.stream, p.ev_color, 0));
 	HD_TRY(hipGetLastError());
 	return HD_OK;
 }

@@ -96,6 +96,10 @@ struct Pool {
 	DevHeightmap hm{};
 	DevHeightmap *d_hm = nullptr; // device copy of hm (editors read it through a pointer)
 	hipStream_t stream = nullptr;
+	// color side stream of a sync-free pass: the leaf encode and SetNode run there, beside the node
+	// levels (joined before the next pass)
+	hipStream_t cstream = nullptr;
+	hipEvent_t ev_leaf = nullptr, ev_color = nullptr;
 	Scratch scratch;
 	// statistics of the last edit (host)
 	u64 last_entries = 0, last_upserts = 0, last_jobs = 0;

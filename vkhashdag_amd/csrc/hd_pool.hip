@@ -175,6 +175,12 @@ int hd_pool_create(const hd_config *cfg, hd_pool **out) {
 	hipError_t e;
 	if ((e = hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking)) != hipSuccess)
 		return fail(e);
+	if ((e = hipStreamCreateWithFlags(&p.cstream, hipStreamNonBlocking)) != hipSuccess)
+		return fail(e);
+	if ((e = hipEventCreateWithFlags(&p.ev_leaf, hipEventDisableTiming)) != hipSuccess)
+		return fail(e);
+	if ((e = hipEventCreateWithFlags(&p.ev_color, hipEventDisableTiming)) != hipSuccess)
+		return fail(e);
 	if ((e = hipMalloc(&p.words, (words + kWordsPad) * 4)) != hipSuccess)
 		return fail(e);
 	if ((e = hipMalloc(&p.bucket_words, buckets * 4)) != hipSuccess)
@@ -209,6 +215,8 @@ int hd_pool_destroy(hd_pool *h) {
 	Pool &p = h->p;
 	if (p.stream)
 		(void)hipStreamSynchronize(p.stream);
+	if (p.cstream)
+		(void)hipStreamSynchronize(p.cstream);
 	p.scratch.release();
 	for (void *ptr : {(void *)p.words, (void *)p.bucket_words, (void *)p.meta, (void *)p.d_filled, (void *)p.cnodes,
 	                  (void *)p.cleaves, (void *)p.hm_mn, (void *)p.hm_mx, (void *)p.d_hm})
@@ -216,6 +224,11 @@ int hd_pool_destroy(hd_pool *h) {
 			(void)hipFree(ptr);
 	if (p.stream)
 		(void)hipStreamDestroy(p.stream);
+	if (p.cstream)
+		(void)hipStreamDestroy(p.cstream);
+	for (hipEvent_t ev : {p.ev_leaf, p.ev_color})
+		if (ev)
+			(void)hipEventDestroy(ev);
 	delete h;
 	return HD_OK;
 }
