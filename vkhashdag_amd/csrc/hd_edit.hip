@@ -1091,7 +1091,54 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			sent[0][0] = lv[0].ent[0];
 	}
 	__syncthreads();
-	for (u32 l = 0; l < l_end; ++l) {
+	// levels of at most 8 entries (64 pairs) by wave 0 alone: ballots and a wave scan replace the
+	// block scan and its barriers
+	__shared__ u32 s_level;
+	if (threadIdx.x < 64) {
+		const u32 lane = threadIdx.x;
+		u32 l = 0, n = scount;
+		for (; l < l_end && n <= 8u; ++l) {
+			const ALevel A = lv[l], N = lv[l + 1];
+			const Entry *cur = sent[l & 1];
+			Entry *nxt = sent[(l + 1) & 1];
+			const bool in = lane < 8u * n;
+			PairOut o{0, 0, make_uint4(0, 0, 0, 0), kNull};
+			Entry E{};
+			if (in) {
+				E = cur[lane >> 3];
+				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap);
+			}
+			const u64 bp = __ballot(o.flag & 1u), bt = __ballot((o.flag >> 1) & 1u);
+			const u64 below = (1ull << lane) - 1ull;
+			u32 lc = o.lcnt; // inclusive wave scan of the list lengths
+#pragma unroll
+			for (u32 d = 1; d < 64; d <<= 1) {
+				const u32 y = __shfl_up(lc, d);
+				if (lane >= d)
+					lc += y;
+			}
+			const Pair tot{(u32)__popcll(bp), __shfl(lc, 63)};
+			if (in) {
+				const u32 m8 = (lane & 7u) == 0 ? (u32)((bp >> lane) & 0xFFu) | ((u32)((bt >> lane) & 0xFFu) << 8) : 0u;
+				const u32 idx = (u32)__popcll(bp & below);
+				Entry ne = emit_pair_e(P, l, A, N, pm, lane, E, o.kid, m8, o, idx, lc - o.lcnt);
+				if ((o.flag & 1u) && idx < kTopEnt)
+					nxt[idx] = ne;
+			}
+			if (lane == 0)
+				set_level_count(P, l, N, pm, tot);
+			n = min(tot.a, N.cap);
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		}
+		if (lane == 0) {
+			s_level = l;
+			scount = n;
+		}
+	}
+	__syncthreads();
+	for (u32 l = s_level; l < l_end; ++l) {
 		const ALevel A = lv[l], N = lv[l + 1];
 		const u32 n = min(scount, kTopEnt);
 		const Entry *cur = sent[l & 1];
