@@ -2309,6 +2309,7 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 	__shared__ ALds U;
 	__shared__ ALevel s_lv[HD_MAX_LEVELS]; // the small levels' arrays and entry counts, read once
 	__shared__ u32 s_n[HD_MAX_LEVELS];
+	__shared__ Pair s_scan[kAT / 64];
 	BottomLds &S = U.b;
 	const u32 L = P.c.L;
 	if (l_from >= 0) { // levels 0 .. l_from + 1 (the first small level's children may be large)
@@ -2342,76 +2343,126 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 		// blockDim entries; an entry whose bucket found the stage full is built again after a reset
 		for (u32 e0 = 0; e0 < n; e0 += blockDim.x) {
 			const u32 e = e0 + threadIdx.x;
-			bool todo = e < n;
-			for (;;) {
-				if (todo) {
+			if (leaf) {
+				if (e < n) {
 					u32 fin = kNull, optr = kNull;
 					bool need = false;
-					if (leaf) {
-						const u32 par = ldg(&A.ent[e].parent, 0), sf = ldg(&A.ent[e].sf, 0);
-						S.pslot[e] = (par << 3) | (sf & 7u);
-						if (!ldg(A.pend, e))
-							fin = ldg(A.status, e);
-						else if ((fin = ldg(A.res, e)) == kNull) {
-							need = true;
-							u32 c0 = ldg(A.content, 2 * (u64)e), c1 = ldg(A.content, 2 * (u64)e + 1);
-							u32 b = ldg(A.bucket, e);
-							S.content[e][0] = c0;
-							S.content[e][1] = c1;
-							S.csz[e] = 2;
-							S.hash[e] = murmur_leaf(c0, c1);
-							S.bucket[e] = b;
-							S.bw[e] = P.bucket_words[b];
-							optr = ldg(&A.ent[e].ptr, 0);
-						}
-						todo = false;
-					} else {
-						const JoinIn q = (have_pre && e0 == 0) ? pre : load_join(A, e);
-						u32 cs[8];
-						if (l < l_from) { // the children's results sit in this entry's content row, by slot
+					const u32 par = ldg(&A.ent[e].parent, 0), sf = ldg(&A.ent[e].sf, 0);
+					S.pslot[e] = (par << 3) | (sf & 7u);
+					if (!ldg(A.pend, e))
+						fin = ldg(A.status, e);
+					else if ((fin = ldg(A.res, e)) == kNull) {
+						need = true;
+						u32 c0 = ldg(A.content, 2 * (u64)e), c1 = ldg(A.content, 2 * (u64)e + 1);
+						u32 b = ldg(A.bucket, e);
+						S.content[e][0] = c0;
+						S.content[e][1] = c1;
+						S.csz[e] = 2;
+						S.hash[e] = murmur_leaf(c0, c1);
+						S.bucket[e] = b;
+						S.bw[e] = P.bucket_words[b];
+						optr = ldg(&A.ent[e].ptr, 0);
+					}
+					S.res[cur][e] = fin;
+					S.slot[cur][e] = need ? 0u : kNull;
+					S.ptr[cur][e] = optr;
+				}
+				__syncthreads();
+				continue;
+			}
+			// join, then the find among the bucket's existing nodes: the block copies the buckets
+			// of its entries to the stage with coalesced loads (a bucket per thread would need a
+			// cache line per lane), rounds of as many buckets as the stage holds
+			bool todo = e < n;
+			Built o{};
+			o.k = 0;
+			u32 bwv = 0, optr = kNull, pidx = 0;
+			if (todo) {
+				const JoinIn q = (have_pre && e0 == 0) ? pre : load_join(A, e);
+				u32 cs[8];
+				if (l < l_from) { // the children's results sit in this entry's content row, by slot
 #pragma unroll
-							for (u32 i = 0; i < 8; ++i)
-								cs[i] = S.content[e][i];
-						} else
-							children_by_slot(q, [&](u32 c) { return child_result(s_lv[l + 1], G, c); }, cs);
-						Built o = build_node(P, (u32)l, q, cs);
-						fin = o.fin;
-						todo = false;
-						if (o.k) {
-							u32 bw;
-							fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, hv, bw, S.stage,
-							                            &S.stage_top, kStage);
-							if (fin == kFindRetry)
-								todo = true;
-							else if (fin == kNull) {
-								need = true;
+					for (u32 i = 0; i < 8; ++i)
+						cs[i] = S.content[e][i];
+				} else
+					children_by_slot(q, [&](u32 c) { return child_result(s_lv[l + 1], G, c); }, cs);
+				o = build_node(P, (u32)l, q, cs);
+				optr = q.ptr;
+				pidx = q.pidx;
+				if (o.k)
+					bwv = P.bucket_words[o.b];
+			}
+			// a bucket larger than the whole stage is scanned in global memory
+			const bool huge = ((bwv + 3u) & ~3u) > kStage;
+			for (;;) {
+				const u32 len = todo && o.k && !huge ? (bwv + 3u) & ~3u : 0u;
+				Pair tot;
+				const Pair off = block_excl_scan2(Pair{len, 0u}, s_scan, tot);
+				const bool fits = off.a + len <= kStage;
+				S.glist[threadIdx.x] = fits ? off.a : kStage; // non-decreasing in thread order
+				S.gnext[threadIdx.x] = o.b;
+				S.bmap[threadIdx.x] = fits ? len : 0u; // a unit past its owner's words is not copied
+				__syncthreads();
+				const u32 units = min(tot.a, kStage) >> 2;
+				const u32 wbpb = P.c.wbpb();
 #pragma unroll
-								for (u32 j = 0; j < 9; ++j)
-									if (j < o.k)
-										S.content[e][j] = o.c[j];
-								S.csz[e] = o.k;
-								S.hash[e] = o.h;
-								S.bucket[e] = o.b;
-								S.bw[e] = bw;
-								optr = q.ptr;
+				for (u32 half = 0; half < 2; ++half) {
+					uint4 v[5];
+					u32 at[5];
+#pragma unroll
+					for (u32 k = 0; k < 5; ++k) {
+						const u32 u = threadIdx.x + (half * 5u + k) * kAT;
+						at[k] = kNull;
+						if (u < units) { // owner: the last thread whose stage offset is <= 4u
+							u32 lo = 0, hi = kAT;
+							while (hi - lo > 1) {
+								const u32 mid = (lo + hi) >> 1;
+								if (S.glist[mid] <= 4u * u)
+									lo = mid;
+								else
+									hi = mid;
+							}
+							const u32 w = 4u * u - S.glist[lo];
+							if (w < S.bmap[lo]) {
+								v[k] = ldg(reinterpret_cast<const uint4 *>(P.words + ((u64)S.gnext[lo] << wbpb) + w), 0);
+								at[k] = u;
 							}
 						}
-						if (!todo)
-							S.pslot[e] = q.pidx;
 					}
-					if (!todo) {
-						S.res[cur][e] = fin;
-						S.slot[cur][e] = need ? 0u : kNull;
-						S.ptr[cur][e] = optr;
-					}
+#pragma unroll
+					for (u32 k = 0; k < 5; ++k)
+						if (at[k] != kNull)
+							reinterpret_cast<uint4 *>(S.stage)[at[k]] = v[k];
 				}
-				const bool again = __syncthreads_or(todo);
-				if (!again && e0 + blockDim.x >= n)
-					break;
-				if (threadIdx.x == 0)
-					S.stage_top = 0;
 				__syncthreads();
-				if (!again)
+				if (todo && (!o.k || len == 0 || fits)) {
+					u32 fin = o.fin;
+					bool need = false;
+					if (o.k) {
+						const u64 base = (u64)o.b << wbpb;
+						fin = huge     ? find_in_global(P.words, P.c, base, bwv, o.c, o.k)
+						      : len == 0 ? kNull
+						      : hv       ? find_in_words_hv(S.stage + off.a, base, bwv, o.c, o.k)
+						                 : find_in_words(S.stage + off.a, P.c, base, bwv, o.c, o.k);
+						if (fin == kNull) {
+							need = true;
+#pragma unroll
+							for (u32 j = 0; j < 9; ++j)
+								if (j < o.k)
+									S.content[e][j] = o.c[j];
+							S.csz[e] = o.k;
+							S.hash[e] = o.h;
+							S.bucket[e] = o.b;
+							S.bw[e] = bwv;
+						}
+					}
+					S.pslot[e] = pidx;
+					S.res[cur][e] = fin;
+					S.slot[cur][e] = need ? 0u : kNull;
+					S.ptr[cur][e] = need ? optr : kNull;
+					todo = false;
+				}
+				if (!__syncthreads_or(todo))
 					break;
 			}
 		}
