@@ -44,6 +44,7 @@ def main():
     ap.add_argument("--mode", choices=["plain", "batched"], default="plain")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tuning", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-edits", type=int, default=4096)
     ap.add_argument("--plain-prefix", type=int, default=512, help="edits of the batch also run plain, for the ratio")
@@ -62,9 +63,13 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    from vkhashdag_amd import autotune
     from vkhashdag_amd import capi
     from vkhashdag_amd import workloads as W
 
+    # launch parameters from the autotuner's stored set (speed only, never results)
+    if not args.no_tuning and autotune.load() is not None:
+        log(f"tuning {capi.get_tuning()} from {autotune.DEFAULT_PATH}")
     res = args.res
     cfg = W.config_for_resolution(res, bucket_bits=16, wbpp=9, pbpb=2, color_leaf_depth=4,
                                   color_node_capacity=1 << 24, color_leaf_capacity=1 << 28)
@@ -205,6 +210,7 @@ def main():
             "resolution": res,
             "frame": [args.width, args.height],
             "edit_mode": args.mode,
+            "tuning": capi.get_tuning(),
         },
         "edit_ms_per_step": float(np.mean(edit_ms)),
         "rays_per_s": rays_per_s,

@@ -296,6 +296,24 @@ int hd_pool_load(const char *path, hd_pool **out, uint32_t *roots, uint32_t cap,
 /* The config a pool was created with (a loaded pool's comes from its file). */
 int hd_pool_get_config(hd_pool *pool, hd_config *out);
 
+/* Launch parameters. trace_tile, top_pairs and small_level change speed only (the same words,
+ * roots and pixels); batch_pass_edits also renumbers the nodes a batched edit appends (the DAG up
+ * to renumbering, the voxels and the colors stay the same -- the batched path's own bar).
+ * Process-wide; a call made after hd_tuning_set uses the new values (a frame keeps the tile it was
+ * created with). The autotuner (vkhashdag_amd/autotune.py) measures the candidates on the device
+ * and stores the best set. No counterpart in the reference (its parameters are compile-time). */
+typedef struct hd_tuning {
+	uint32_t trace_tile;       /* pixel tile edge of the frame kernel: 8 or 16 (default 16) */
+	uint32_t top_pairs;        /* the one-workgroup top-down kernel takes levels of at most this many
+	                            * (entry, slot) pairs: [64, 4096] (default 4096) */
+	uint32_t small_level;      /* the one-workgroup bottom-up kernel takes levels of at most this many
+	                            * entries: [64, 1024] (default 1024) */
+	uint32_t batch_pass_edits; /* most edits one batched pass applies; a larger group of pairwise
+	                            * disjoint edits is split into consecutive passes; 0 = no limit */
+} hd_tuning;
+int hd_tuning_get(hd_tuning *out);
+int hd_tuning_set(const hd_tuning *t);
+
 /* Device memory helpers for hosts without a GPU runtime of their own. */
 int hd_device_alloc(uint64_t bytes, void **out);
 int hd_device_free(void *ptr);

@@ -46,3 +46,21 @@ def test_checkpoint_load_rejects_bad_files_before_touching_the_gpu(tmp_path):
     rc = lib.hd_pool_load(str(bad).encode(), ctypes.byref(h), roots, 4, ctypes.byref(n))
     assert rc != 0 and b"truncated" in lib.hd_last_error()
     assert not h.value
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_tuning_set_get_and_validation():
+    # process-wide launch parameters: no GPU call is made
+    import sys
+    sys.path.insert(0, ROOT)
+    from vkhashdag_amd import capi
+    old = capi.get_tuning()
+    try:
+        capi.set_tuning(trace_tile=8, top_pairs=512, small_level=128, batch_pass_edits=3)
+        assert capi.get_tuning() == dict(trace_tile=8, top_pairs=512, small_level=128, batch_pass_edits=3)
+        for bad in (dict(trace_tile=4), dict(top_pairs=8192), dict(small_level=2048), dict(small_level=10)):
+            with pytest.raises(capi.HDError):
+                capi.set_tuning(**bad)
+        assert capi.get_tuning()["trace_tile"] == 8  # a rejected set changes nothing
+    finally:
+        capi.set_tuning(**old)

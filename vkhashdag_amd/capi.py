@@ -61,6 +61,14 @@ class hd_stats(C.Structure):
                 ("last_ms", C.c_double), ("last_groups", C.c_uint32), ("pad_", C.c_uint32)]
 
 
+class hd_tuning(C.Structure):
+    _fields_ = [("trace_tile", C.c_uint32), ("top_pairs", C.c_uint32), ("small_level", C.c_uint32),
+                ("batch_pass_edits", C.c_uint32)]
+
+
+TUNING_FIELDS = tuple(f for f, _ in hd_tuning._fields_)
+
+
 class hd_view(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("look", C.c_float * 3), ("side", C.c_float * 3), ("up", C.c_float * 3),
                 ("proj_factor", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("flags", C.c_uint32),
@@ -107,6 +115,8 @@ def lib():
             "hd_pool_save": (C.c_int, [VP, C.c_char_p, VP, U32]),
             "hd_pool_load": (C.c_int, [C.c_char_p, C.POINTER(VP), VP, U32, C.POINTER(U32)]),
             "hd_pool_get_config": (C.c_int, [VP, C.POINTER(hd_config)]),
+            "hd_tuning_get": (C.c_int, [C.POINTER(hd_tuning)]),
+            "hd_tuning_set": (C.c_int, [C.POINTER(hd_tuning)]),
             "hd_trace_tiles": (C.c_int, [VP, U32, VP]),
             "hd_trace_beam_sizes": (C.c_int, [VP, U32, VP]),
             "hd_trace_beam_async": (C.c_int, [VP, U32, U32, VP, VP, VP, U32, U32, U32, VP, VP]),
@@ -435,6 +445,24 @@ class Frame:
 
     def __del__(self):
         self.close()
+
+
+def get_tuning() -> dict:
+    """The launch parameters in effect (hd_tuning): speed only, never results."""
+    t = hd_tuning()
+    check(lib().hd_tuning_get(C.byref(t)))
+    return {f: int(getattr(t, f)) for f in TUNING_FIELDS}
+
+
+def set_tuning(**kw) -> dict:
+    """Sets some launch parameters (the others keep their values); returns the previous set."""
+    old = get_tuning()
+    unknown = set(kw) - set(TUNING_FIELDS)
+    if unknown:
+        raise HDError(f"unknown tuning fields {sorted(unknown)}")
+    t = hd_tuning(**{**old, **{k: int(v) for k, v in kw.items()}})
+    check(lib().hd_tuning_set(C.byref(t)))
+    return old
 
 
 def device_count() -> int:

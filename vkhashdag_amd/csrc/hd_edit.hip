@@ -3217,7 +3217,8 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	const EditParams P = pass_params(p, B.edits + pp.first, pp.count, pp.stateful);
 	const u32 V = pp.stateful ? 1u << (3u * P.leaf_bits) : 0u;
 	u32 top_end = 0;
-	while (top_end + 1 < L && 8 * pp.cap[top_end] <= kTopPairs)
+	const u64 top_pairs = std::min<u64>(g_tuning.top_pairs, kTopPairs);
+	while (top_end + 1 < L && 8 * pp.cap[top_end] <= top_pairs)
 		++top_end;
 	if (reserve)
 		HD_RET(gen_reserve(p, B, 1));
@@ -3238,7 +3239,8 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	}
 	a_leaf<<<grid_for(pp.cap[L - 1], kAT / 64, 16384), kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V);
 	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
-	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= kSmallLevel)
+	const u64 small_level = std::min<u32>(g_tuning.small_level, kSmallLevel);
+	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= small_level)
 		++l_from;
 	if (colors) { // the leaf encode and SetNode on the color stream, beside the node levels
 		HD_TRY(hipEventRecord(p.ev_leaf, p.stream));
@@ -3332,15 +3334,19 @@ int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n,
 	std::vector<hd_edit_desc> flat;
 	std::vector<PassPlan> plans;
 	flat.reserve(n);
-	for (auto &g : groups) {
-		PassPlan pp;
-		pp.first = (u32)flat.size();
-		pp.count = (u32)g.size();
-		for (u32 k : g)
-			flat.push_back(edits[k]);
-		plans.push_back(pp);
-	}
-	p.last_groups = (u32)groups.size();
+	// a group's edits are pairwise disjoint, so any split of it into consecutive passes applies them
+	// in an order they commute under
+	const u32 cap = g_tuning.batch_pass_edits ? g_tuning.batch_pass_edits : ~0u;
+	for (auto &g : groups)
+		for (size_t b = 0; b < g.size(); b += cap) {
+			PassPlan pp;
+			pp.first = (u32)flat.size();
+			pp.count = (u32)std::min<size_t>(cap, g.size() - b);
+			for (size_t j = b; j < b + pp.count; ++j)
+				flat.push_back(edits[g[j]]);
+			plans.push_back(pp);
+		}
+	p.last_groups = (u32)plans.size();
 	return edit_sequence(p, root, croot, flat, plans, root_out, croot_out);
 }
 
@@ -3378,6 +3384,7 @@ struct FrameState {
 	hd_view *d_views = nullptr;
 	u32 *d_tiles = nullptr, *d_rgba = nullptr;
 	u32 max_tiles = 0;
+	u32 tile = 16; // the frame kernel's pixel tile when the graph was captured
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	~FrameState() {
@@ -3475,7 +3482,8 @@ static int frame_create(Pool &p, const hd_frame_config &cfg, u32 root, u32 croot
 		return set_error("hd_frame_create: color leaves deeper than 8^7 voxels"), HD_ERR_UNSUPPORTED;
 	HD_RET(async_setup(p, F.plans, flat, F.B, F.pre, &F.gen_mirror));
 	// views, tiles, output
-	const u32 tiles1 = div_up(cfg.max_width, 16) * div_up(cfg.max_height, 16);
+	F.tile = g_tuning.trace_tile;
+	const u32 tiles1 = div_up(cfg.max_width, F.tile) * div_up(cfg.max_height, F.tile);
 	F.max_tiles = tiles1 * cfg.n_views;
 	HD_RET(p.scratch.get<hd_view>(F.pre + "views", cfg.n_views, &F.d_views));
 	HD_RET(p.scratch.get<u32>(F.pre + "tiles", cfg.n_views + 1, &F.d_tiles));
@@ -3561,7 +3569,7 @@ static int frame_set(FrameState &F, const hd_edit_desc *edits, u32 n, const hd_v
 			off += (u64)w.width * w.height;
 		}
 		hv[v] = w;
-		ht[v + 1] = ht[v] + (v < n_views ? div_up(w.width, 16) * div_up(w.height, 16) : 0u);
+		ht[v + 1] = ht[v] + (v < n_views ? div_up(w.width, F.tile) * div_up(w.height, F.tile) : 0u);
 	}
 	HD_TRY(hipMemcpyAsync(F.B.edits, he, (u64)cfg.max_passes * cfg.max_edits * sizeof(hd_edit_desc),
 	                      hipMemcpyHostToDevice, p.stream));

@@ -108,6 +108,16 @@ struct Pool {
 	u32 edit_gen = 0; // generation of the sync-free pass's group table (hd_edit.hip)
 };
 
+// Launch parameters (hd_tuning_set; process-wide): speed only, except that batch_pass_edits also
+// renumbers the nodes of batched edits.
+struct Tuning {
+	u32 trace_tile = 16;      // frame kernel pixel tile edge: 8 or 16
+	u32 top_pairs = 4096;     // single-workgroup top-down levels: at most this many (entry, slot) pairs
+	u32 small_level = 1024;   // single-workgroup bottom-up levels: at most this many entries
+	u32 batch_pass_edits = 0; // most edits per batched pass (a larger scheduling group is split); 0: no limit
+};
+extern Tuning g_tuning; // hd_pool.hip
+
 int make_filled(Pool &p);               // hd_pool.hip
 // Multi-view trace of (root, color_root) queued on stream s; views and tile prefix on the device.
 // d_roots (optional): read the roots from device memory instead (graph replays).

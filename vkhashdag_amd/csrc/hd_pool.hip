@@ -590,3 +590,32 @@ extern "C" int hd_pool_get_config(hd_pool *h, hd_config *out) {
 	*out = h->p.cfg;
 	return HD_OK;
 }
+
+// ---------------------------------------------------------------- tuning
+namespace hd {
+Tuning g_tuning;
+}
+extern "C" int hd_tuning_get(hd_tuning *out) {
+	if (!out)
+		return set_error("hd_tuning_get: null argument"), HD_ERR_ARG;
+	out->trace_tile = g_tuning.trace_tile;
+	out->top_pairs = g_tuning.top_pairs;
+	out->small_level = g_tuning.small_level;
+	out->batch_pass_edits = g_tuning.batch_pass_edits;
+	return HD_OK;
+}
+extern "C" int hd_tuning_set(const hd_tuning *t) {
+	if (!t)
+		return set_error("hd_tuning_set: null argument"), HD_ERR_ARG;
+	if (t->trace_tile != 8 && t->trace_tile != 16)
+		return set_error("hd_tuning_set: trace_tile must be 8 or 16 (got %u)", t->trace_tile), HD_ERR_ARG;
+	if (t->top_pairs < 64 || t->top_pairs > 4096)
+		return set_error("hd_tuning_set: top_pairs must be in [64, 4096] (got %u)", t->top_pairs), HD_ERR_ARG;
+	if (t->small_level < 64 || t->small_level > 1024)
+		return set_error("hd_tuning_set: small_level must be in [64, 1024] (got %u)", t->small_level), HD_ERR_ARG;
+	g_tuning.trace_tile = t->trace_tile;
+	g_tuning.top_pairs = t->top_pairs;
+	g_tuning.small_level = t->small_level;
+	g_tuning.batch_pass_edits = t->batch_pass_edits;
+	return HD_OK;
+}

@@ -16,8 +16,8 @@
 namespace hd {
 
 constexpr u32 kStack = 23;  // STACK_SIZE, fraction bits of float (trace.frag:52-54)
-constexpr u32 kTile = 16;   // 16x16 pixels per workgroup
-constexpr u32 kThreads = kTile * kTile;
+constexpr u32 kThreads = 256; // workgroup of the beam and traversal kernels
+// The frame kernel's pixel tile (workgroup) is kT x kT, kT in {8, 16}, picked by the tuning.
 constexpr u32 kMaxIter = 1u << 16;
 
 __device__ inline float fmx(float a, float b) { return a < b ? b : a; } // std::max / GLSL max order
@@ -320,8 +320,9 @@ __device__ inline float beam_sample(const float *__restrict__ B, u32 Wb, u32 Hb,
 
 __device__ inline u32 q8(float x) { return (u32)(fmn(fmx(x, 0.0f), 1.0f) * 255.0f + 0.5f); }
 
-template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(TraceArgs A) {
-	__shared__ u32 stack_lds[kStack * kThreads];
+template <bool kCount, u32 kT> __global__ void __launch_bounds__(kT *kT) k_trace(TraceArgs A) {
+	constexpr u32 kTT = kT * kT;
+	__shared__ u32 stack_lds[kStack * kTT];
 	// view of this tile (binary search over the tile prefix); a fixed grid may run past the last tile
 	u32 tile = blockIdx.x, lo = 0, hi = A.n_views;
 	if (tile >= A.tile_start[A.n_views])
@@ -336,8 +337,8 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 	}
 	const hd_view V = A.views[lo];
 	u32 t = tile - A.tile_start[lo];
-	u32 tiles_x = (V.width + kTile - 1) / kTile;
-	u32 px = (t % tiles_x) * kTile + (threadIdx.x % kTile), py = (t / tiles_x) * kTile + (threadIdx.x / kTile);
+	u32 tiles_x = (V.width + kT - 1) / kT;
+	u32 px = (t % tiles_x) * kT + (threadIdx.x % kT), py = (t / tiles_x) * kT + (threadIdx.x / kT);
 	if (px >= V.width || py >= V.height)
 		return;
 	u32 *stack = stack_lds + threadIdx.x;
@@ -359,10 +360,10 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 				h.vox[i] = 0, h.hit_pos[i] = 0.0f, h.norm[i] = 0.0f, h.pick[i] = 0.0f;
 		} else
 			h = march<kCount>(A.words, root, A.leaf_level, V.proj_factor, t0, V.pos[0] + t0 * dx, V.pos[1] + t0 * dy,
-			                  V.pos[2] + t0 * dz, dx, dy, dz, stack, kThreads, reads);
+			                  V.pos[2] + t0 * dz, dx, dy, dz, stack, kTT, reads);
 	} else
 		h = march<kCount>(A.words, root, A.leaf_level, V.proj_factor, 0.0f, V.pos[0], V.pos[1], V.pos[2], dx, dy,
-		                  dz, stack, kThreads, reads);
+		                  dz, stack, kTT, reads);
 	u32 iters = h.iter;
 	float lx = 4.0f, ly = 5.0f, lz = 3.0f;
 	float ll = sqrtf(lx * lx + ly * ly + lz * lz);
@@ -370,7 +371,7 @@ template <bool kCount> __global__ void __launch_bounds__(kThreads) k_trace(Trace
 	bool shadow = false;
 	if (h.hit && (V.flags & HD_TRACE_SHADOW)) {
 		MarchOut s = march<kCount>(A.words, root, A.leaf_level, __builtin_inff(), 0.0f, h.hit_pos[0], h.hit_pos[1],
-		                           h.hit_pos[2], lx, ly, lz, stack, kThreads, reads);
+		                           h.hit_pos[2], lx, ly, lz, stack, kTT, reads);
 		shadow = s.hit;
 		iters += s.iter;
 	}
@@ -448,11 +449,30 @@ __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ 
 	out[4 * i + 3] = h.hit ? 1.0f : 0.0f;
 }
 
+// The frame kernel at the tuned tile (tiles counted with the same tile by trace_tile_count).
+static void launch_trace(bool count, u32 n_tiles, hipStream_t s, const TraceArgs &A) {
+	if (g_tuning.trace_tile == 8) {
+		if (count)
+			k_trace<true, 8><<<n_tiles, 64, 0, s>>>(A);
+		else
+			k_trace<false, 8><<<n_tiles, 64, 0, s>>>(A);
+	} else {
+		if (count)
+			k_trace<true, 16><<<n_tiles, 256, 0, s>>>(A);
+		else
+			k_trace<false, 16><<<n_tiles, 256, 0, s>>>(A);
+	}
+}
+static u32 trace_tile_count(const hd_view &v) {
+	const u32 t = g_tuning.trace_tile;
+	return div_up(v.width, t) * div_up(v.height, t);
+}
+
 int trace_on_stream(Pool &p, hipStream_t s, u32 root, u32 color_root, const hd_view *d_views, const u32 *d_tiles,
                     u32 n_views, u32 n_tiles, u32 *rgba, const u32 *d_roots) {
 	TraceArgs A{p.words, p.cnodes, p.cleaves, root,    color_root, p.dc.L, p.dc.color_leaf,
 	            d_views, d_tiles,  n_views,   rgba,    nullptr,    nullptr, d_roots, nullptr, nullptr};
-	k_trace<false><<<n_tiles, kThreads, 0, s>>>(A);
+	launch_trace(false, n_tiles, s, A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
 }
@@ -474,7 +494,7 @@ extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd
 			return set_error("hd_trace: records requested without a buffer"), HD_ERR_ARG;
 		if ((views[v].flags & HD_TRACE_BEAM) && !std::isfinite(views[v].proj_factor))
 			return set_error("hd_trace: the beam prepass needs a finite proj_factor (LOD on)"), HD_ERR_ARG;
-		tiles[v + 1] = tiles[v] + div_up(views[v].width, kTile) * div_up(views[v].height, kTile);
+		tiles[v + 1] = tiles[v] + trace_tile_count(views[v]);
 	}
 	HD_RET(hd_trace_beam_sizes(views, n_views, beams.data()));
 	hd_view *dv;
@@ -493,7 +513,7 @@ extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd
 	            dv,      dt,       n_views,   rgba, records, nullptr, nullptr, beam, db};
 	if (beams[n_views])
 		k_beam<<<div_up(beams[n_views], kThreads), kThreads, 0, p.stream>>>(A, beam);
-	k_trace<false><<<tiles[n_views], kThreads, 0, p.stream>>>(A);
+	launch_trace(false, tiles[n_views], p.stream, A);
 	HD_TRY(hipGetLastError());
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
@@ -509,10 +529,7 @@ extern "C" int hd_trace_async(hd_pool *h, uint32_t root, uint32_t color_root, co
 	Pool &p = h->p;
 	TraceArgs A{p.words,      p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
 	            n_views,      rgba,     records,   (unsigned long long *)counters, nullptr, nullptr, nullptr};
-	if (counters)
-		k_trace<true><<<n_tiles, kThreads, 0, p.stream>>>(A);
-	else
-		k_trace<false><<<n_tiles, kThreads, 0, p.stream>>>(A);
+	launch_trace(counters != nullptr, n_tiles, p.stream, A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
 }
@@ -535,7 +552,7 @@ extern "C" int hd_trace_beam_async(hd_pool *h, uint32_t root, uint32_t color_roo
 	            n_views, rgba,     nullptr,   nullptr, nullptr, d_beam, d_beam_start};
 	if (n_beam)
 		k_beam<<<div_up(n_beam, kThreads), kThreads, 0, p.stream>>>(A, d_beam);
-	k_trace<false><<<n_tiles, kThreads, 0, p.stream>>>(A);
+	launch_trace(false, n_tiles, p.stream, A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
 }
@@ -543,7 +560,7 @@ extern "C" int hd_trace_beam_async(hd_pool *h, uint32_t root, uint32_t color_roo
 extern "C" int hd_trace_tiles(const hd_view *views, uint32_t n_views, uint32_t *tile_start) {
 	tile_start[0] = 0;
 	for (u32 v = 0; v < n_views; ++v)
-		tile_start[v + 1] = tile_start[v] + div_up(views[v].width, kTile) * div_up(views[v].height, kTile);
+		tile_start[v + 1] = tile_start[v] + trace_tile_count(views[v]);
 	return HD_OK;
 }
 
