@@ -320,6 +320,8 @@ int hd_device_free(void *ptr);
 int hd_memcpy_h2d(void *dst, const void *src, uint64_t bytes);
 int hd_memcpy_d2h(void *dst, const void *src, uint64_t bytes);
 int hd_device_count(int *n);
+/* "name (arch, N CUs)" of device dev, NUL-terminated within cap bytes. */
+int hd_device_name(int dev, char *buf, uint32_t cap);
 /* Device-to-device copy by a plain 16-byte-vector kernel on the pool's stream (pool may be null:
  * the null stream); bytes and both pointers 16-byte aligned. Its HBM traffic is exactly bytes read
  * and bytes written, which calibrates hardware byte counters. Blocking. */

@@ -173,12 +173,9 @@ def tune(res: int = 4096, seed: int = 1, plain_edits: int = 64, batch_edits: int
 
 
 def _device_name() -> str:
-    try:
-        import torch
+    from . import capi
 
-        return torch.cuda.get_device_name(0)
-    except Exception:  # noqa: BLE001 -- informational only
-        return "unknown"
+    return capi.device_name(0)
 
 
 def main(argv=None) -> int:

@@ -115,6 +115,7 @@ def lib():
             "hd_pool_save": (C.c_int, [VP, C.c_char_p, VP, U32]),
             "hd_pool_load": (C.c_int, [C.c_char_p, C.POINTER(VP), VP, U32, C.POINTER(U32)]),
             "hd_pool_get_config": (C.c_int, [VP, C.POINTER(hd_config)]),
+            "hd_device_name": (C.c_int, [C.c_int, C.c_char_p, U32]),
             "hd_tuning_get": (C.c_int, [C.POINTER(hd_tuning)]),
             "hd_tuning_set": (C.c_int, [C.POINTER(hd_tuning)]),
             "hd_trace_tiles": (C.c_int, [VP, U32, VP]),
@@ -463,6 +464,12 @@ def set_tuning(**kw) -> dict:
     t = hd_tuning(**{**old, **{k: int(v) for k, v in kw.items()}})
     check(lib().hd_tuning_set(C.byref(t)))
     return old
+
+
+def device_name(dev: int = 0) -> str:
+    buf = C.create_string_buffer(256)
+    check(lib().hd_device_name(dev, buf, 256))
+    return buf.value.decode()
 
 
 def device_count() -> int:

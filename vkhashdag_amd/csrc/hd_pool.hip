@@ -619,3 +619,12 @@ extern "C" int hd_tuning_set(const hd_tuning *t) {
 	g_tuning.batch_pass_edits = t->batch_pass_edits;
 	return HD_OK;
 }
+
+extern "C" int hd_device_name(int dev, char *buf, uint32_t cap) {
+	if (!buf || !cap)
+		return set_error("hd_device_name: null argument"), HD_ERR_ARG;
+	hipDeviceProp_t prop;
+	HD_TRY(hipGetDeviceProperties(&prop, dev));
+	std::snprintf(buf, cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+	return HD_OK;
+}
