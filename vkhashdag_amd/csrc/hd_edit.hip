@@ -1722,18 +1722,48 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const
 	}
 }
 // single block: exclusive prefix of the append sizes in entry (DFS) order
-__global__ void __launch_bounds__(kAT) a_scan_app(u32 leafL, u32 *__restrict__ pm, const u32 *__restrict__ app,
-                                                  u32 *__restrict__ appo) {
-	__shared__ Pair sh[kAT];
-	const u32 n = pm[kPmHdr + leafL];
+// Exclusive scan of the jobs' encoded sizes, one workgroup: 1024 threads x 8 consecutive jobs per
+// tile (a running sum in registers, a wave scan, the waves' totals through LDS).
+constexpr u32 kScanThreads = 1024, kScanPer = 8;
+__global__ void __launch_bounds__(kScanThreads) a_scan_app(u32 leafL, u32 *__restrict__ pm,
+                                                           const u32 *__restrict__ app, u32 *__restrict__ appo) {
+	__shared__ u32 swave[kScanThreads / 64];
+	const u32 n = pm[kPmHdr + leafL], lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
 	u32 run = 0;
-	for (u32 t0 = 0; t0 < n; t0 += kAT) {
-		u32 t = t0 + threadIdx.x;
-		Pair tile;
-		Pair off = block_excl_scan2(Pair{t < n ? app[t] : 0u, 0}, sh, tile);
-		if (t < n)
-			appo[t] = run + off.a;
-		run += tile.a;
+	for (u32 b = 0; b < n; b += kScanThreads * kScanPer) {
+		const u32 first = b + threadIdx.x * kScanPer;
+		u32 v[kScanPer], sum = 0;
+#pragma unroll
+		for (u32 k = 0; k < kScanPer; ++k) {
+			v[k] = first + k < n ? app[first + k] : 0u;
+			sum += v[k];
+		}
+		u32 x = sum; // inclusive wave scan of the per-thread sums
+#pragma unroll
+		for (u32 d = 1; d < 64; d <<= 1) {
+			const u32 y = __shfl_up(x, d);
+			if (lane >= d)
+				x += y;
+		}
+		if (lane == 63)
+			swave[w] = x;
+		__syncthreads();
+		u32 pre = 0, tot = 0;
+#pragma unroll
+		for (u32 k = 0; k < kScanThreads / 64; ++k) {
+			const u32 t = swave[k];
+			pre += k < w ? t : 0u;
+			tot += t;
+		}
+		u32 o = run + pre + x - sum;
+#pragma unroll
+		for (u32 k = 0; k < kScanPer; ++k) {
+			if (first + k < n)
+				appo[first + k] = o;
+			o += v[k];
+		}
+		run += tot;
+		__syncthreads(); // swave reuse
 	}
 	if (threadIdx.x == 0)
 		pm[kPmLeafTotal] = run;
@@ -1864,6 +1894,20 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_write(EditParams P, cons
 // SetNode (k_setnode_a/b/c) for every level above the color leaves, in one workgroup: pass A
 // bottom-up decisions + subtree append counts, pass B top-down post-order indices, pass C
 // bottom-up writes; then the pool counters and the color root.
+// The 8 children's color results of entry e with every load issued before any is used: proceeding
+// children from the next level's results (entries first_child + rank), the others from okids.
+__device__ inline void color_children(const ALevel &A, const ALevel &C, u32 e, u32 (&ch)[8]) {
+	const u32 m = ldg(A.pmask, e) & 0xFFu, fc = ldg(A.first_child, e);
+	const uint4 o0 = ldg(reinterpret_cast<const uint4 *>(A.okids), 2 * (u64)e);
+	const uint4 o1 = ldg(reinterpret_cast<const uint4 *>(A.okids), 2 * (u64)e + 1);
+	const u32 ok[8] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, o1.z, o1.w};
+#pragma unroll
+	for (u32 i = 0; i < 8; ++i) {
+		const u32 c = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
+		const u32 v = ldg(C.ores, min(c, C.cap - 1u));
+		ch[i] = ((m >> i) & 1u) ? (c < C.cap ? v : kColorNull) : ok[i];
+	}
+}
 __device__ inline u32 color_child_result(const ALevel &A, const ALevel &C, u32 e, u32 i) {
 	u32 m = A.pmask[e] & 0xFFu;
 	if ((m >> i) & 1u) {
@@ -1884,10 +1928,9 @@ __device__ inline void setnode_block(const EditParams &P, const ALevel *__restri
 			if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
 				u32 ch[8];
 				bool all_null = true, same_color;
-				for (u32 i = 0; i < 8; ++i) {
-					ch[i] = color_child_result(A, C, e, i);
+				color_children(A, C, e, ch);
+				for (u32 i = 0; i < 8; ++i)
 					all_null = all_null && ctag(ch[i]) == kTagNull;
-				}
 				same_color = ctag(ch[0]) == kTagColor;
 				for (u32 i = 1; i < 8 && same_color; ++i)
 					same_color = ch[i] == ch[0];
@@ -1943,8 +1986,10 @@ __device__ inline void setnode_block(const EditParams &P, const ALevel *__restri
 				continue;
 			u32 idx = A.own[e], r;
 			if ((u64)idx < node_cap) {
+				u32 ch[8];
+				color_children(A, C, e, ch);
 				for (u32 i = 0; i < 8; ++i)
-					P.cnodes[(u64)idx * 8u + i] = color_child_result(A, C, e, i);
+					P.cnodes[(u64)idx * 8u + i] = ch[i];
 				r = cmake(kTagNode, idx);
 			} else {
 				r = A.ent[e].octree;
@@ -2552,7 +2597,8 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 // SetNode of a colored pass (VBREditorWrapper's joins above the color leaves), one workgroup. It
 // needs the top-down entries and the encoded color leaves only, so it runs on the color stream
 // beside the node levels.
-__global__ void __launch_bounds__(kAT) a_setnode(EditParams P, const ALevel *__restrict__ lv,
+constexpr u32 kSetNodeThreads = 1024;
+__global__ void __launch_bounds__(kSetNodeThreads) a_setnode(EditParams P, const ALevel *__restrict__ lv,
                                                  const u32 *__restrict__ pm, u32 *__restrict__ roots, u64 node_cap) {
 	__shared__ ColorLds Cs;
 	if (threadIdx.x == 0) {
@@ -3247,9 +3293,9 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_leaf, 0));
 		u32 G = grid_for(pp.cap[leafL], 1, 8192);
 		a_encode_size<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app);
-		a_scan_app<<<1, kAT, 0, p.cstream>>>(leafL, B.pm, B.app, B.appo);
+		a_scan_app<<<1, kScanThreads, 0, p.cstream>>>(leafL, B.pm, B.app, B.appo);
 		a_encode_write<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap);
-		a_setnode<<<1, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
+		a_setnode<<<1, kSetNodeThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
 		HD_TRY(hipEventRecord(p.ev_color, p.cstream));
 	}
 	for (int l = (int)L - 1; l > l_from; --l) {
