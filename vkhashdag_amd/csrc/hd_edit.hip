@@ -1224,17 +1224,85 @@ __global__ void __launch_bounds__(kAT) a_emit(EditParams P, u32 l, const ALevel 
 }
 
 // ---- colors of a sync-free pass: dense decode, pushes
-__global__ void a_dense_init(const ALevel *__restrict__ lv, u32 leafL, const u32 *__restrict__ pm, u32 log2v,
-                             const u32 *__restrict__ cleaves, Color *__restrict__ dense) {
+// One workgroup per color-leaf job (grid-stride): a writer job's voxels get the old chunk's colors
+// (or its fill color). A chunk's macro table and run headers are staged in LDS, and each thread
+// decodes a run of kDenseRun consecutive voxels: one binary search for its first voxel, then it
+// walks the headers forward (runs are in voxel order).
+constexpr u32 kDenseThreads = 256, kDenseRun = 16, kDenseHdrWords = 8192;
+__global__ void __launch_bounds__(kDenseThreads) a_dense_init(const ALevel *__restrict__ lv, u32 leafL,
+                                                              const u32 *__restrict__ pm, u32 log2v,
+                                                              const u32 *__restrict__ cleaves,
+                                                              Color *__restrict__ dense) {
+	__shared__ u32 hdr[kDenseHdrWords]; // macro table (2M words) then headers (2B words)
 	const Entry *ent = lv[leafL].ent;
-	u64 total = (u64)pm[kPmHdr + leafL] << log2v;
-	const u32 vm = (1u << log2v) - 1u;
-	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (u64)gridDim.x * blockDim.x) {
-		u32 e = (u32)(i >> log2v), v = (u32)i & vm;
-		Entry E = ent[e];
+	const u32 jobs = pm[kPmHdr + leafL], V = 1u << log2v;
+	for (u32 job = blockIdx.x; job < jobs; job += gridDim.x) {
+		const Entry E = ent[job];
 		if (!(E.sf & kSfWriter))
+			continue; // uniform
+		Color *out = dense + (u64)job * V;
+		if (ctag(E.octree) != kTagLeaf) {
+			const Color fill = get_fill(E.octree);
+			for (u32 v = threadIdx.x; v < V; v += blockDim.x)
+				out[v] = fill;
 			continue;
-		dense[i] = ctag(E.octree) == kTagLeaf ? chunk_voxel(cleaves, cdata(E.octree), v) : get_fill(E.octree);
+		}
+		const u32 idx = cdata(E.octree), M = cleaves[idx + 1u], B = cleaves[idx + 2u];
+		if (2u * (M + B) > kDenseHdrWords) { // a chunk too large to stage: decode each voxel in place
+			for (u32 v = threadIdx.x; v < V; v += blockDim.x)
+				out[v] = chunk_voxel(cleaves, idx, v);
+			continue;
+		}
+		const u32 mo = idx + 4u, wo = mo + 2u * M + 2u * B;
+		__syncthreads(); // the previous job's readers are done with hdr
+		for (u32 k = threadIdx.x; k < 2u * (M + B); k += blockDim.x)
+			hdr[k] = cleaves[mo + k];
+		__syncthreads();
+		const u32 *mac = hdr, *hd = hdr + 2u * M;
+		for (u32 v0 = threadIdx.x * kDenseRun; v0 < V; v0 += blockDim.x * kDenseRun) {
+			u32 m = v0 >> 14, h = 0, lb = 0, ws = 0;
+			auto seek = [&](u32 vid) { // last header of macro m with voxoff <= vid
+				const u32 fb = mac[2u * m];
+				lb = m + 1u < M ? mac[2u * (m + 1u)] : B;
+				ws = mac[2u * m + 1u];
+				u32 lo = fb, cnt = lb - fb;
+				while (cnt > 0) {
+					const u32 step = cnt >> 1;
+					if ((hd[2u * (lo + step) + 1u] >> 18) <= vid)
+						lo += step + 1u, cnt -= step + 1u;
+					else
+						cnt = step;
+				}
+				h = lo > fb ? lo - 1u : fb; // a macro's first header starts at voxel 0
+			};
+			if (m < M)
+				seek(v0 & 0x3FFFu);
+			for (u32 k = 0; k < kDenseRun && v0 + k < V; ++k) {
+				const u32 v = v0 + k, vid = v & 0x3FFFu;
+				if ((v >> 14) != m) { // a new macro block starts
+					m = v >> 14;
+					if (m < M)
+						seek(vid);
+				}
+				Color c = Color::none();
+				if (m < M) {
+					while (h + 1u < lb && (hd[2u * (h + 1u) + 1u] >> 18) <= vid)
+						++h;
+					const u32 hc = hd[2u * h], hp = hd[2u * h + 1u], bpw = (hp >> 16) & 3u;
+					if (bpw == 0)
+						c = Color{hc, 1u << 16};
+					else {
+						const u32 bit = ws + (hp & 0xFFFFu) + (vid - (hp >> 18)) * bpw, o = bit & 31u;
+						u32 w = cleaves[wo + (bit >> 5)] >> o;
+						if (o + bpw > 32u)
+							w |= cleaves[wo + (bit >> 5) + 1u] << (32u - o);
+						w &= (1u << bpw) - 1u;
+						c = Color{hc, w | (bpw << 8) | (1u << 16)};
+					}
+				}
+				out[v] = c;
+			}
+		}
 	}
 }
 // Pushes cover terminal subtrees and leaf edits cover proceeding leaves: disjoint voxel ranges,
@@ -3277,7 +3345,7 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	}
 	const bool colors = pp.stateful && pp.cap[leafL] > 0;
 	if (colors) {
-		a_dense_init<<<grid_for(pp.cap[leafL] * V, 256, 16384), 256, 0, p.stream>>>(B.d_lv, leafL, B.pm,
+		a_dense_init<<<grid_for(pp.cap[leafL], 1, 8192), kDenseThreads, 0, p.stream>>>(B.d_lv, leafL, B.pm,
 		                                                                            3u * P.leaf_bits, p.cleaves, B.dense);
 		if (pp.ev_cap)
 			a_push<<<grid_for(pp.ev_cap, 1, 8192), 256, 0, p.stream>>>(B.ev, B.pm, (u32)pp.ev_cap, V,
