@@ -554,7 +554,9 @@ __device__ inline u32 find_in_bucket_staged(const u32 *__restrict__ words, const
 		v[k] = q[k];
 	const u32 bw = bucket_words[bucket];
 	bw_out = bw;
-	if (bw <= 16u) { // scan the first 16 words in registers
+	if (bw == 0u)
+		return kNull;
+	if (bw <= 16u && !hv) { // scan the first 16 words in registers (node by node)
 		u32 W[16];
 #pragma unroll
 		for (u32 k = 0; k < 4; ++k) {
