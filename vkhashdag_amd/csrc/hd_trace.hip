@@ -322,7 +322,7 @@ __device__ inline u32 q8(float x) { return (u32)(fmn(fmx(x, 0.0f), 1.0f) * 255.0
 
 template <bool kCount, u32 kT> __global__ void __launch_bounds__(kT *kT) k_trace(TraceArgs A) {
 	constexpr u32 kTT = kT * kT;
-	__shared__ u32 stack_lds[kStack * kTT];
+	extern __shared__ u32 stack_lds[]; // node levels x kTT words (trace_lds_bytes)
 	// view of this tile (binary search over the tile prefix); a fixed grid may run past the last tile
 	u32 tile = blockIdx.x, lo = 0, hi = A.n_views;
 	if (tile >= A.tile_start[A.n_views])
@@ -404,7 +404,7 @@ template <bool kCount, u32 kT> __global__ void __launch_bounds__(kT *kT) k_trace
 
 // One thread per beam texel of every HD_TRACE_BEAM view (views without the flag have no texels).
 __global__ void __launch_bounds__(kThreads) k_beam(TraceArgs A, float *__restrict__ beam) {
-	__shared__ u32 stack_lds[kStack * kThreads];
+	extern __shared__ u32 stack_lds[];
 	const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= A.beam_start[A.n_views])
 		return;
@@ -436,7 +436,7 @@ __global__ void __launch_bounds__(kThreads) k_beam(TraceArgs A, float *__restric
 __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ W, u32 root, u32 leaf_level,
                                                          const float *__restrict__ o, const float *__restrict__ d,
                                                          u64 n, float *__restrict__ out) {
-	__shared__ u32 stack_lds[kStack * kThreads];
+	extern __shared__ u32 stack_lds[];
 	u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
@@ -450,17 +450,23 @@ __global__ void __launch_bounds__(kThreads) k_traversal(const u32 *__restrict__ 
 }
 
 // The frame kernel at the tuned tile (tiles counted with the same tile by trace_tile_count).
+// The march keeps one stack word per node level and thread in LDS (pushes and pops stay at scales
+// >= the leaf scale, so the slots are scale - leaf_scale in [0, levels)); sizing the stack to the
+// pool's levels rather than the 23 float scales leaves room for more resident waves.
+static u32 trace_lds_bytes(u32 levels, u32 threads) { return levels * threads * 4u; }
 static void launch_trace(bool count, u32 n_tiles, hipStream_t s, const TraceArgs &A) {
 	if (g_tuning.trace_tile == 8) {
+		const u32 lds = trace_lds_bytes(A.leaf_level, 64);
 		if (count)
-			k_trace<true, 8><<<n_tiles, 64, 0, s>>>(A);
+			k_trace<true, 8><<<n_tiles, 64, lds, s>>>(A);
 		else
-			k_trace<false, 8><<<n_tiles, 64, 0, s>>>(A);
+			k_trace<false, 8><<<n_tiles, 64, lds, s>>>(A);
 	} else {
+		const u32 lds = trace_lds_bytes(A.leaf_level, 256);
 		if (count)
-			k_trace<true, 16><<<n_tiles, 256, 0, s>>>(A);
+			k_trace<true, 16><<<n_tiles, 256, lds, s>>>(A);
 		else
-			k_trace<false, 16><<<n_tiles, 256, 0, s>>>(A);
+			k_trace<false, 16><<<n_tiles, 256, lds, s>>>(A);
 	}
 }
 static u32 trace_tile_count(const hd_view &v) {
@@ -512,7 +518,7 @@ extern "C" int hd_trace(hd_pool *h, uint32_t root, uint32_t color_root, const hd
 	TraceArgs A{p.words, p.cnodes, p.cleaves, root,   color_root, p.dc.L, p.dc.color_leaf,
 	            dv,      dt,       n_views,   rgba, records, nullptr, nullptr, beam, db};
 	if (beams[n_views])
-		k_beam<<<div_up(beams[n_views], kThreads), kThreads, 0, p.stream>>>(A, beam);
+		k_beam<<<div_up(beams[n_views], kThreads), kThreads, trace_lds_bytes(p.dc.L, kThreads), p.stream>>>(A, beam);
 	launch_trace(false, tiles[n_views], p.stream, A);
 	HD_TRY(hipGetLastError());
 	HD_TRY(hipStreamSynchronize(p.stream));
@@ -551,7 +557,7 @@ extern "C" int hd_trace_beam_async(hd_pool *h, uint32_t root, uint32_t color_roo
 	TraceArgs A{p.words, p.cnodes, p.cleaves, root, color_root, p.dc.L, p.dc.color_leaf, d_views, d_tile_start,
 	            n_views, rgba,     nullptr,   nullptr, nullptr, d_beam, d_beam_start};
 	if (n_beam)
-		k_beam<<<div_up(n_beam, kThreads), kThreads, 0, p.stream>>>(A, d_beam);
+		k_beam<<<div_up(n_beam, kThreads), kThreads, trace_lds_bytes(p.dc.L, kThreads), p.stream>>>(A, d_beam);
 	launch_trace(false, n_tiles, p.stream, A);
 	HD_TRY(hipGetLastError());
 	return HD_OK;
@@ -568,7 +574,8 @@ extern "C" int hd_traversal(hd_pool *h, uint32_t root, const float *o, const flo
 	Pool &p = h->p;
 	if (!n)
 		return HD_OK;
-	k_traversal<<<div_up(n, kThreads), kThreads, 0, p.stream>>>(p.words, root, p.dc.L, o, d, n, out);
+	k_traversal<<<div_up(n, kThreads), kThreads, trace_lds_bytes(p.dc.L, kThreads), p.stream>>>(p.words, root, p.dc.L, o,
+	                                                                                         d, n, out);
 	HD_TRY(hipGetLastError());
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
