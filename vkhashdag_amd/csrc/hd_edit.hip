@@ -3330,16 +3330,19 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
 	const EditParams P = pass_params(p, B.edits + pp.first, pp.count, pp.stateful);
 	const u32 V = pp.stateful ? 1u << (3u * P.leaf_bits) : 0u;
+	// A pair of a multi-edit pass tests its parent's edit list: count those tests too (lists hold at
+	// most the pass's edits, and shrink with depth), so list-heavy levels go to the wide kernels
+	const u64 lf = (std::min<u32>(pp.count, 128) + 7) / 8;
 	u32 top_end = 0;
 	const u64 top_pairs = std::min<u64>(g_tuning.top_pairs, kTopPairs);
-	while (top_end + 1 < L && 8 * pp.cap[top_end] <= top_pairs)
+	while (top_end + 1 < L && 8 * pp.cap[top_end] * lf <= top_pairs)
 		++top_end;
 	if (reserve)
 		HD_RET(gen_reserve(p, B, 1));
 	a_top<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.roots, B.pm, kPmHdr + 2 * L, top_end, B.ev, (u32)pp.ev_cap,
 	                               B.gen_ctr);
 	for (u32 l = top_end; l + 1 < L; ++l) {
-		u32 G = grid_for(8 * pp.cap[l], 2048, kMaxEmitBlocks);
+		u32 G = grid_for(8 * pp.cap[l] * lf, 2048, kMaxEmitBlocks);
 		a_count<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.ev, (u32)pp.ev_cap, B.bsum);
 		a_emit<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.bsum);
 	}
