@@ -625,6 +625,7 @@ extern "C" int hd_device_name(int dev, char *buf, uint32_t cap) {
 		return set_error("hd_device_name: null argument"), HD_ERR_ARG;
 	hipDeviceProp_t prop;
 	HD_TRY(hipGetDeviceProperties(&prop, dev));
-	std::snprintf(buf, cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+	std::snprintf(buf, cap, "%s (%s, %d CUs)", prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName,
+	              prop.multiProcessorCount);
 	return HD_OK;
 }
