@@ -1380,8 +1380,14 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 	const ALevel A = lv[level];
 	const u32 n = pm[kPmHdr + level];
 	const u32 waves = gridDim.x * (kAT / 64u);
-	for (u32 e = (u32)(((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6); e < n; e += waves) {
+	// the wave's leaf index is uniform: keep it (and what is read through it) in scalar registers
+	const u32 e0 = __builtin_amdgcn_readfirstlane((u32)(((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+	for (u32 e = e0; e < n; e += waves) {
 		Entry E = A.ent[e];
+		E.x = __builtin_amdgcn_readfirstlane(E.x), E.y = __builtin_amdgcn_readfirstlane(E.y);
+		E.z = __builtin_amdgcn_readfirstlane(E.z), E.ptr = __builtin_amdgcn_readfirstlane(E.ptr);
+		E.sf = __builtin_amdgcn_readfirstlane(E.sf), E.job = __builtin_amdgcn_readfirstlane(E.job);
+		E.loff = __builtin_amdgcn_readfirstlane(E.loff), E.llen = __builtin_amdgcn_readfirstlane(E.llen);
 		u32 w0 = 0, w1 = 0;
 		if (E.ptr != kNull) {
 			w0 = P.words[E.ptr];
