@@ -3018,10 +3018,50 @@ static void edit_aabb(const hd_edit_desc &d, u32 res, i64 lo[3], i64 hi[3]) {
 	}
 }
 
+// Whether edits a and b may share a voxel: their voxel AABBs meet and, for a sphere, its ball of
+// radius isqrt(r2) + 1 (which holds every voxel p with |p - c|^2 <= r2) meets the other shape's
+// ball or box. Conservative: false only for disjoint voxel sets.
+static bool may_share_voxels(const hd_edit_desc &a, const i64 *ba, const hd_edit_desc &b, const i64 *bb) {
+	if (!(ba[0] <= bb[3] && bb[0] <= ba[3] && ba[1] <= bb[4] && bb[1] <= ba[4] && ba[2] <= bb[5] && bb[2] <= ba[5]))
+		return false;
+	auto isqrt = [](u64 v) {
+		i64 r = (i64)std::sqrt((double)v);
+		while ((r + 1) * (r + 1) <= (i64)v)
+			++r;
+		while (r * r > (i64)v)
+			--r;
+		return r;
+	};
+	const bool sa = a.kind <= HD_EDIT_SPHERE_PAINT, sb = b.kind <= HD_EDIT_SPHERE_PAINT;
+	const bool xa = a.kind >= HD_EDIT_AABB_FILL && a.kind <= HD_EDIT_AABB_PAINT;
+	const bool xb = b.kind >= HD_EDIT_AABB_FILL && b.kind <= HD_EDIT_AABB_PAINT;
+	if (sa && sb) {
+		i64 d2 = 0;
+		for (int i = 0; i < 3; ++i) {
+			const i64 d = (i64)a.p0[i] - (i64)b.p0[i];
+			d2 += d * d;
+		}
+		const i64 R = isqrt(a.r2) + isqrt(b.r2) + 2;
+		return d2 <= R * R;
+	}
+	if ((sa && xb) || (sb && xa)) {
+		const hd_edit_desc &sp = sa ? a : b;
+		const i64 *bx = sa ? bb : ba; // the box's voxel range [lo, hi]
+		i64 d2 = 0;
+		for (int i = 0; i < 3; ++i) {
+			const i64 c = (i64)sp.p0[i], q = std::min(std::max(c, bx[i]), bx[i + 3]);
+			d2 += (c - q) * (c - q);
+		}
+		const i64 R = isqrt(sp.r2) + 1;
+		return d2 <= R * R;
+	}
+	return true; // boxes (their AABBs are exact) and column fills
+}
+
 // Level scheduling of a batch: edit k goes to the first group after every group holding an earlier
-// edit whose voxel AABB meets k's. Edits within a group have disjoint voxel sets, so they commute
-// and one pass applies them; groups run in order, so every voxel sees its edits in submission order.
-// A coarse 3-D grid of cells keeps the overlap search local.
+// edit that may share a voxel with k (may_share_voxels). Edits within a group have disjoint voxel
+// sets, so they commute and one pass applies them; groups run in order, so every voxel sees its
+// edits in submission order. A coarse 3-D grid of cells keeps the overlap search local.
 void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std::vector<u32>> &groups) {
 	const u32 cell_bits = res >= 64 ? std::max(4u, (u32)std::log2((double)res) - 5u) : 2u; // ~32^3 cells
 	const u32 cells = std::max(1u, res >> cell_bits);
@@ -3042,9 +3082,7 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 				for (i64 y = c0[1]; y <= c1[1]; ++y)
 					for (i64 x = c0[0]; x <= c1[0]; ++x)
 						for (u32 j : grid[((size_t)z * cells + y) * cells + x]) {
-							const i64 *o = &box[6 * (size_t)j];
-							if (gid[j] + 1 > g && o[0] <= b[3] && b[0] <= o[3] && o[1] <= b[4] && b[1] <= o[4] &&
-							    o[2] <= b[5] && b[2] <= o[5])
+							if (gid[j] + 1 > g && may_share_voxels(edits[j], &box[6 * (size_t)j], edits[k], b))
 								g = gid[j] + 1;
 						}
 		}
