@@ -50,6 +50,24 @@ def test_plain_edits_bit_exact(case):
     assert np.array_equal(dev.filled(), ora.filled())
 
 
+def test_fragmented_color_leaf_bit_exact():
+    # 1500 one-voxel colored fills in one 16^3 color leaf: a color change at almost every voxel makes
+    # the encoded chunk (2 header words per change) outgrow the encoder's 4096-word staging slot, so
+    # the later passes encode the chunk again from the dense colors instead of copying it
+    capi = _capi()
+    cfg = small_cfg(64, 8, 9, 2, 4)
+    rng = np.random.default_rng(7)
+    pts = rng.choice(16 ** 3, 1500, replace=False)
+    e = W.edits([W.edit(W.SPHERE_FILL, (16 + int(p) % 16, 16 + int(p) // 16 % 16, 16 + int(p) // 256), r2=0,
+                        color=int(rng.integers(1 << 24))) for p in pts])
+    dev, ora = capi.DevicePool(cfg), O.OraclePool(cfg)
+    for i in range(0, len(e), 250):
+        dev.edit(e[i:i + 250], capi.MODE_PLAIN)
+        ora.edit(e[i:i + 250])
+        assert_same_pool(dev, ora, f"after {i + 250} edits")
+    assert dev.stats().color_leaf_words > 4096
+
+
 def test_terrain_column_fill_bit_exact():
     capi = _capi()
     res = 128

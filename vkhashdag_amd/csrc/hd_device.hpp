@@ -330,6 +330,23 @@ struct DevEditor {
 			t = kNotAffected;
 		return t;
 	}
+	// edit_voxel_vbr without the voxel's current color: asg is set when the edit assigns c (the color
+	// stays the current one otherwise).
+	__device__ bool edit_voxel_vbr_asg(u32 x, u32 y, u32 z, bool voxel, Color &c, bool &asg) const {
+		bool in = in_range(x, y, z);
+		if (d.kind == HD_EDIT_COLUMN_FILL) {
+			if (in || !voxel) {
+				c = Color::rgb8(column_rgb8(x, y, z));
+				asg = true;
+			}
+			return voxel || in;
+		}
+		if (in || !voxel) {
+			c = color();
+			asg = true;
+		}
+		return is_paint() ? voxel : (voxel || in);
+	}
 	__device__ bool edit_voxel_vbr(u32 x, u32 y, u32 z, bool voxel, Color &c) const {
 		bool in = in_range(x, y, z);
 		if (d.kind == HD_EDIT_COLUMN_FILL) {

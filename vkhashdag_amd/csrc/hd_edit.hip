@@ -788,6 +788,7 @@ __global__ void k_add_meta(u32 *meta, u32 which, const u32 *a, const u32 *b, u32
 constexpr u32 kAT = 256;       // threads per block of the sync-free kernels
 constexpr u32 kTopPairs = 4096; // a level whose 8 n pairs fit this bound expands inside a_top
 constexpr u32 kGather = 32;    // bucket lists up to this length are sorted in registers
+constexpr u32 kWalkList = 16;  // groups per bucket listed by arrival for the walker
 
 struct Pair {
 	u32 a, b;
@@ -821,7 +822,7 @@ __device__ inline Pair block_excl_scan2(Pair x, Pair *sh, Pair &total) {
 }
 
 // per-pass device counters: pm[kPmHdr + l] entries of level l, pm[kPmHdr + L + l] list words
-enum PassMeta : u32 { kPmOverflow = 0, kPmEvents = 1, kPmLeafTotal = 2, kPmGen = 3, kPmHdr = 4 };
+enum PassMeta : u32 { kPmOverflow = 0, kPmEvents = 1, kPmLeafTotal = 2, kPmGen = 3, kPmRest = 4, kPmHdr = 5 };
 // counters accumulated over the passes of one hd_edit call
 enum AccMeta : u32 { kAccEntries = 0, kAccUpserts = 1, kAccJobs = 2, kAccOverflow = 3, kAccCount = 4 };
 
@@ -837,6 +838,8 @@ struct ALevel {
 	u32 *ores, *ccnt, *cbase, *own; // color results, SetNode passes
 	u32 cap, list_cap;
 };
+
+constexpr u32 kListBatch = 8; // edit-list words loaded per round trip
 
 struct PairOut {
 	u32 flag, lcnt;
@@ -877,14 +880,21 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 	else {
 		cnt = 0;
 		bool active = false;
-		for (u32 k = 0; k < E.llen; ++k) {
-			u32 idx = A.lists[E.loff + k];
-			DevEditor ed = P.editor(idx);
-			if (ed.geom(cl, cx, cy, cz) != kNotAffected) {
-				++cnt;
-				single = idx;
-				active |= ed.stateful();
-			}
+		for (u32 k0 = 0; k0 < E.llen; k0 += kListBatch) { // list words of a batch in one round trip
+			u32 ix[kListBatch];
+#pragma unroll
+			for (u32 j = 0; j < kListBatch; ++j)
+				ix[j] = k0 + j < E.llen ? ldg(A.lists, (u64)E.loff + k0 + j) : 0u;
+#pragma unroll
+			for (u32 j = 0; j < kListBatch; ++j)
+				if (k0 + j < E.llen) {
+					DevEditor ed = P.editor(ix[j]);
+					if (ed.geom(cl, cx, cy, cz) != kNotAffected) {
+						++cnt;
+						single = ix[j];
+						active |= ed.stateful();
+					}
+				}
 		}
 		if (cnt == 1)
 			c = classify_single(P, P.editor(single), cl, cx, cy, cz, old, octree, sf_in);
@@ -894,9 +904,14 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 				c.sf |= kSfActive | (cl == leafL ? (kSfWriter | kSfNewJob) : 0u);
 		}
 	}
-	if (c.push) {
+	if (c.push) { // one bump of the shared event counter per wavefront (the active pushing lanes)
 		u32 d = cl - leafL, msk = (1u << d) - 1u, sh = 3u * (P.c.voxel_level - cl);
-		u32 k = atomicAdd(&pm[kPmEvents], 1u);
+		const u64 m = __ballot(1);
+		const u32 lane = __lane_id(), leader = (u32)__builtin_ctzll(m);
+		u32 k0 = 0;
+		if (lane == leader)
+			k0 = atomicAdd(&pm[kPmEvents], (u32)__popcll(m));
+		const u32 k = (u32)__shfl((int)k0, (int)leader) + (u32)__popcll(m & ((1ull << lane) - 1ull));
 		if (k < ev_cap)
 			ev[k] = PushEvent{E.job, morton3(cx & msk, cy & msk, cz & msk) << sh, 1u << sh, c.col.colors, c.col.wb, 0, 0, 0};
 		else
@@ -940,10 +955,19 @@ __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A,
 			ln = 0;
 		} else {
 			u32 w = lo;
-			for (u32 k = 0; k < E.llen; ++k) {
-				u32 ix = A.lists[E.loff + k];
-				if (P.editor(ix).geom(l + 1, cx, cy, cz) != kNotAffected)
-					N.lists[w++] = ix;
+			for (u32 k0 = 0; k0 < E.llen; k0 += kListBatch) { // loads and tests first, then the stores
+				u32 ix[kListBatch], keep = 0;
+#pragma unroll
+				for (u32 j = 0; j < kListBatch; ++j)
+					ix[j] = k0 + j < E.llen ? ldg(A.lists, (u64)E.loff + k0 + j) : 0u;
+#pragma unroll
+				for (u32 j = 0; j < kListBatch; ++j)
+					if (k0 + j < E.llen && P.editor(ix[j]).geom(l + 1, cx, cy, cz) != kNotAffected)
+						keep |= 1u << j;
+#pragma unroll
+				for (u32 j = 0; j < kListBatch; ++j)
+					if ((keep >> j) & 1u)
+						stg(N.lists, (u64)w++, ix[j]);
 			}
 		}
 	} else
@@ -1058,7 +1082,7 @@ __device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__
 // Single workgroup: clears the pass counters, classifies the root and expands levels [0, l_end)
 // (every one small enough for one block), with the current level's entries and count in LDS.
 constexpr u32 kTopEnt = kTopPairs / 8;
-constexpr u32 kTopEdits = 64; // a pass with at most this many edits reads them from LDS in a_top
+constexpr u32 kTopEdits = 1024; // a pass with at most this many edits reads them from LDS in a_top
 __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__restrict__ lv, u32 *__restrict__ roots,
                                              u32 *__restrict__ pm, u32 pm_words, u32 l_end, PushEvent *ev, u32 ev_cap,
                                              u32 *__restrict__ gen_ctr) {
@@ -1228,18 +1252,46 @@ __global__ void __launch_bounds__(kAT) a_emit(EditParams P, u32 l, const ALevel 
 // (or its fill color). A chunk's macro table and run headers are staged in LDS, and each thread
 // decodes a run of kDenseRun consecutive voxels: one binary search for its first voxel, then it
 // walks the headers forward (runs are in voxel order).
+// The color-leaf entries are visited 64 at a time per block: block b takes entries c0 + j G
+// (j < 64, G blocks, c0 = b + 64 G i), so the writer jobs of a run of neighbouring entries go to
+// different blocks. Wave 0 reads the 64 flags in one round trip and handles the entries without a
+// writer (`other`, in parallel); the block then runs its writer jobs one after another (the
+// returned mask is block-uniform). A block per entry would spend a dependent load on every entry
+// without a writer.
+template <class Other>
+__device__ inline u64 writer_chunk(const Entry *__restrict__ ent, u32 n, u32 c0, u64 *s_wm, Other other) {
+	if (threadIdx.x < 64u) {
+		const u32 e = c0 + threadIdx.x * gridDim.x;
+		bool w = false;
+		if (e < n) {
+			w = (ldg(&ent[e].sf, 0) & kSfWriter) != 0u;
+			if (!w)
+				other(e);
+		}
+		const u64 m = __ballot(w);
+		if (threadIdx.x == 0)
+			*s_wm = m;
+	}
+	__syncthreads();
+	const u64 m = *s_wm;
+	__syncthreads(); // the next chunk rewrites s_wm
+	return m;
+}
+constexpr u32 kJobChunk = 64;
+
 constexpr u32 kDenseThreads = 256, kDenseRun = 16, kDenseHdrWords = 8192;
 __global__ void __launch_bounds__(kDenseThreads) a_dense_init(const ALevel *__restrict__ lv, u32 leafL,
                                                               const u32 *__restrict__ pm, u32 log2v,
                                                               const u32 *__restrict__ cleaves,
                                                               Color *__restrict__ dense) {
 	__shared__ u32 hdr[kDenseHdrWords]; // macro table (2M words) then headers (2B words)
+	__shared__ u64 s_wm;
 	const Entry *ent = lv[leafL].ent;
 	const u32 jobs = pm[kPmHdr + leafL], V = 1u << log2v;
-	for (u32 job = blockIdx.x; job < jobs; job += gridDim.x) {
+	for (u32 c0 = blockIdx.x; c0 < jobs; c0 += gridDim.x * kJobChunk)
+	for (u64 m = writer_chunk(ent, jobs, c0, &s_wm, [](u32) {}); m; m &= m - 1ull) {
+		const u32 job = c0 + (u32)__builtin_ctzll(m) * gridDim.x;
 		const Entry E = ent[job];
-		if (!(E.sf & kSfWriter))
-			continue; // uniform
 		Color *out = dense + (u64)job * V;
 		if (ctag(E.octree) != kTagLeaf) {
 			const Color fill = get_fill(E.octree);
@@ -1373,9 +1425,15 @@ __device__ inline u32 child_result(const ALevel &C, const GroupTable &G, u32 c) 
 	return c < C.cap ? v : kNull; // c >= cap is unreachable with sound bounds (the pass reports it)
 }
 
-// edit_leaf (k_leaf) + the find among the bucket's existing leaves: one wavefront per leaf.
+// edit_leaf (k_leaf) + the find among the bucket's existing leaves: one wavefront per leaf. A
+// leaf inside a color job records the colors its edits assign (lcol, by lane) and which voxels
+// they are (lmask); a_leafcolor moves them into the job's dense colors, so this kernel does not
+// wait for the dense colors to be decoded.
+__device__ inline bool leaf_writes_colors(const EditParams &P, const ALevel *__restrict__ lv, const Entry &E) {
+	return (E.sf & kSfWriter) && !(E.sf & kSfFinal) && (E.sf & kSfActive) && E.job < lv[P.c.color_leaf].cap;
+}
 __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
-                                              Color *__restrict__ dense, u32 V) {
+                                              Color *__restrict__ lcol, u64 *__restrict__ lmask) {
 	const u32 lane = threadIdx.x & 63u, level = P.c.L - 1;
 	const ALevel A = lv[level];
 	const u32 n = pm[kPmHdr + level];
@@ -1397,28 +1455,24 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		u32 vx = (E.x << 2) | ((lane >> 2) & 2u) | (lane & 1u);
 		u32 vy = (E.y << 2) | ((lane >> 3) & 2u) | ((lane >> 1) & 1u);
 		u32 vz = (E.z << 2) | ((lane >> 4) & 2u) | ((lane >> 2) & 1u);
-		const bool writer = (E.sf & kSfWriter) && !(E.sf & kSfFinal) && E.job < lv[P.c.color_leaf].cap;
-		Color *cp = nullptr;
+		const bool writer = leaf_writes_colors(P, lv, E);
 		Color col = Color::none();
-		if (writer) {
-			u32 msk = (1u << P.leaf_bits) - 1u;
-			cp = dense + (u64)E.job * V + morton3(vx & msk, vy & msk, vz & msk);
-			col = *cp;
-		}
+		bool asg = false;
 		bool nv = v;
 		for (u32 k = 0; k < E.llen; ++k) {
 			DevEditor ed = P.editor(E.llen == 1 ? E.loff : A.lists[E.loff + k]);
 			if (!ed.stateful())
 				nv = ed.edit_voxel(vx, vy, vz, nv);
-			else if (writer)
-				nv = ed.edit_voxel_vbr(vx, vy, vz, nv, col);
-			else {
-				Color dummy = Color::none();
-				nv = ed.edit_voxel_vbr(vx, vy, vz, nv, dummy);
-			}
+			else
+				nv = ed.edit_voxel_vbr_asg(vx, vy, vz, nv, col, asg);
 		}
-		if (writer && (E.sf & kSfActive))
-			*cp = col;
+		if (writer) {
+			const u64 am = __ballot(asg);
+			if (asg)
+				lcol[(u64)e * 64u + lane] = col;
+			if (lane == 0)
+				lmask[e] = am;
+		}
 		unsigned long long m = __ballot(nv);
 		u32 n0 = (u32)m, n1 = (u32)(m >> 32);
 		bool pending = (n0 != w0 || n1 != w1) && m != 0ull; // wavefront-uniform
@@ -1438,6 +1492,33 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 			A.content[2 * (u64)e + 1] = n1;
 			A.bucket[e] = b;
 			A.res[e] = f;
+		}
+	}
+}
+
+// The colors a_leaf's edits assigned, into the jobs' dense colors (after a_dense_init and a_push:
+// pushes cover terminal subtrees, leaves proceeding ones, so the voxel ranges are disjoint). One
+// wavefront per leaf.
+__global__ void __launch_bounds__(kAT) a_leafcolor(EditParams P, const ALevel *__restrict__ lv,
+                                                   const u32 *__restrict__ pm, const Color *__restrict__ lcol,
+                                                   const u64 *__restrict__ lmask, Color *__restrict__ dense, u32 V) {
+	const u32 lane = threadIdx.x & 63u, level = P.c.L - 1;
+	const ALevel A = lv[level];
+	const u32 n = pm[kPmHdr + level];
+	const u32 waves = gridDim.x * (kAT / 64u);
+	const u32 e0 = __builtin_amdgcn_readfirstlane((u32)(((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+	for (u32 e = e0; e < n; e += waves) {
+		Entry E = A.ent[e];
+		E.sf = __builtin_amdgcn_readfirstlane(E.sf), E.job = __builtin_amdgcn_readfirstlane(E.job);
+		if (!leaf_writes_colors(P, lv, E))
+			continue;
+		const u64 am = lmask[e];
+		if ((am >> lane) & 1ull) {
+			const u32 vx = (E.x << 2) | ((lane >> 2) & 2u) | (lane & 1u);
+			const u32 vy = (E.y << 2) | ((lane >> 3) & 2u) | ((lane >> 1) & 1u);
+			const u32 vz = (E.z << 2) | ((lane >> 4) & 2u) | ((lane >> 2) & 1u);
+			const u32 msk = (1u << P.leaf_bits) - 1u;
+			dense[(u64)E.job * V + morton3(vx & msk, vy & msk, vz & msk)] = lcol[(u64)e * 64u + lane];
 		}
 	}
 }
@@ -1519,10 +1600,10 @@ __device__ inline Built build_node(const EditParams &P, u32 l, const JoinIn &q, 
 }
 
 // edit_node's join (k_build) for entry e of level l, with the children's results resolved from
-// level l + 1, then the find among the bucket's existing nodes.
-// Returns false when the workgroup's stage was full (the entry is built again after a reset).
-__device__ inline bool build_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, const GroupTable &G,
-                                   u32 e, u32 *stage, u32 *top, u32 cap) {
+// level l + 1. An entry whose content is known final (unchanged / emptied) is written here and
+// returns o.k == 0.
+__device__ inline Built join_entry(const EditParams &P, u32 l, const ALevel &A, const ALevel &C, const GroupTable &G,
+                                   u32 e) {
 	const JoinIn q = load_join(A, e);
 	u32 cs[8];
 	children_by_slot(q, [&](u32 c) { return child_result(C, G, c); }, cs);
@@ -1530,8 +1611,13 @@ __device__ inline bool build_entry(const EditParams &P, u32 l, const ALevel &A, 
 	if (!o.k) {
 		stg(A.pend, e, 0u);
 		stg(A.status, e, o.fin);
-		return true;
 	}
+	return o;
+}
+// The find of a joined entry among its bucket's existing nodes. Returns false when the
+// workgroup's stage was full (the find runs again after a reset; the join is kept).
+__device__ inline bool find_entry(const EditParams &P, u32 l, const ALevel &A, u32 e, const Built &o, u32 *stage,
+                                  u32 *top, u32 cap) {
 	u32 bw;
 	const u32 f = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, headers_by_value(P.c, l), bw,
 	                                    stage, top, cap);
@@ -1548,9 +1634,10 @@ __device__ inline bool build_entry(const EditParams &P, u32 l, const ALevel &A, 
 	return true;
 }
 
-// Joins entry e (pending, not in the pool) to its content's group; the creator queues the group.
+// Joins entry e (pending, not in the pool) to its content's group; the creator queues the group
+// (its arrival index lpos; the first kWalkList groups of a bucket are also listed by arrival).
 __device__ inline void dedup_entry(const ALevel &A, const GroupTable &G, u32 gen, bool leaf, u32 e,
-                                   u32 *__restrict__ bcnt, u32 *__restrict__ head) {
+                                   u32 *__restrict__ bcnt, u32 *__restrict__ head, u32 *__restrict__ blist) {
 	A.lpos[e] = kNull;
 	if (!A.pend[e] || A.res[e] != kNull)
 		return;
@@ -1566,7 +1653,10 @@ __device__ inline void dedup_entry(const ALevel &A, const GroupTable &G, u32 gen
 			if (atomicCAS((unsigned long long *)&G.tab[s], (unsigned long long)cur, (unsigned long long)mine) == cur) {
 				A.slot[e] = s;
 				u32 b = A.bucket[e];
-				A.lpos[e] = atomicAdd(&bcnt[b], 1u);
+				const u32 lp = atomicAdd(&bcnt[b], 1u);
+				A.lpos[e] = lp;
+				if (lp < kWalkList)
+					blist[(u64)b * kWalkList + lp] = s;
 				G.gnext[s] = atomicExch(&head[b], s);
 				return;
 			}
@@ -1585,76 +1675,142 @@ __device__ inline void dedup_entry(const ALevel &A, const GroupTable &G, u32 gen
 	}
 }
 
-// The walker of a bucket (the group queued first) replays the bucket's new groups in the order of
-// their first entries (DFS order): find among the words appended by this walk, else append_node
-// (NodePool.hpp:187-215).
-__device__ inline void walk_entry(const EditParams &P, u32 l, const ALevel &A, const GroupTable &G, u32 e,
-                                  u32 *__restrict__ bcnt, u32 *__restrict__ head, u32 *__restrict__ acc) {
+// The walker of a bucket appends the bucket's new groups in the order of their first entries (DFS
+// order), as append_node (NodePool.hpp:134-157, 187-215) would: the groups of a level hold pairwise
+// distinct contents, so no group can match a node appended by an earlier group of this walk (the
+// find among the existing nodes, in the build, was the whole find).
+// Short lists (kWalkList groups at most, read by arrival index) are ranked, placed and copied with
+// static register indexing: every load of one phase is independent of the others, so a walk costs a
+// few memory round trips whatever its length.
+template <u32 kG>
+__device__ inline u32 walk_short(const EditParams &P, bool leaf, const ALevel &A, const GroupTable &G, u32 b, u32 k,
+                                  u32 bw, const u32 *__restrict__ blist) {
+	const u32 stride = leaf ? 2u : 9u, wpp = P.c.words_per_page(), wpb = P.c.words_per_bucket();
+	u32 sl[kG], key[kG], nz[kG], rk[kG], pos[kG];
+#pragma unroll
+	for (u32 x = 0; x < kG; ++x)
+		sl[x] = x < k ? ldg(blist, (u64)b * kWalkList + x) : 0u;
+#pragma unroll
+	for (u32 x = 0; x < kG; ++x)
+		key[x] = x < k ? (u32)ldg(G.tab, sl[x]) : 0xFFFFFFFFu; // the group's first entry
+#pragma unroll
+	for (u32 x = 0; x < kG; ++x)
+		nz[x] = x < k ? (leaf ? 2u : ldg(A.csize, key[x])) : 0u;
+#pragma unroll
+	for (u32 x = 0; x < kG; ++x) { // rank by first entry (distinct keys)
+		u32 r = 0;
+#pragma unroll
+		for (u32 y = 0; y < kG; ++y)
+			r += key[y] < key[x] ? 1u : 0u;
+		rk[x] = r;
+		pos[x] = kNull;
+	}
+	u32 appended = 0;
+	bool any_fail = false;
+	const u64 bb = (u64)b << P.c.wbpb();
+#pragma unroll
+	for (u32 r = 0; r < kG; ++r) { // append_to_bucket in rank order: positions and page-tail zeros
+		if (r >= k)
+			break;
+		u32 n = 0;
+#pragma unroll
+		for (u32 x = 0; x < kG; ++x)
+			n = rk[x] == r ? nz[x] : n;
+		u32 at = kNull;
+		if (bw + n <= wpb) {
+			u32 slot = bw >> P.c.wbpp, off = bw & (wpp - 1u);
+			if (off + n > wpp) {
+				for (u32 q = off; q < wpp; ++q)
+					stg(P.words, bb + ((u64)slot << P.c.wbpp) + q, 0u);
+				++slot;
+				off = 0;
+			}
+			const u32 rel = (slot << P.c.wbpp) | off;
+			at = (u32)(bb + rel);
+			bw = rel + n;
+			++appended;
+		} else
+			any_fail = true;
+#pragma unroll
+		for (u32 x = 0; x < kG; ++x)
+			pos[x] = rk[x] == r ? at : pos[x];
+	}
+	// results, then the contents, four groups per round of loads
+#pragma unroll
+	for (u32 x = 0; x < kG; ++x)
+		if (x < k) {
+			stg(G.gres, sl[x], pos[x]);
+			stg(G.gfail, sl[x], pos[x] == kNull ? 1u : 0u);
+		}
+#pragma unroll
+	for (u32 x0 = 0; x0 < kG; x0 += 4) {
+		if (x0 >= k)
+			break;
+		u32 w[4][9];
+#pragma unroll
+		for (u32 d = 0; d < 4; ++d)
+#pragma unroll
+			for (u32 j = 0; j < 9; ++j)
+				w[d][j] = x0 + d < k && pos[x0 + d] != kNull && j < nz[x0 + d]
+				              ? ldg(A.content, (u64)key[x0 + d] * stride + j)
+				              : 0u;
+#pragma unroll
+		for (u32 d = 0; d < 4; ++d)
+#pragma unroll
+			for (u32 j = 0; j < 9; ++j)
+				if (x0 + d < k && pos[x0 + d] != kNull && j < nz[x0 + d])
+					stg(P.words, (u64)pos[x0 + d] + j, w[d][j]);
+	}
+	P.bucket_words[b] = bw;
+	if (any_fail)
+		atomicOr(&P.meta[kMetaOverflow], 1u);
+	return appended;
+}
+
+// Returns the nodes appended (0 unless e walks its bucket).
+__device__ inline u32 walk_entry(const EditParams &P, u32 l, const ALevel &A, const GroupTable &G, u32 e,
+                                  u32 *__restrict__ bcnt, u32 *__restrict__ head, const u32 *__restrict__ blist) {
 	if (A.lpos[e] != 0)
-		return;
+		return 0;
 	const bool leaf = l == P.c.L - 1;
 	const u32 stride = leaf ? 2u : 9u;
 	const u32 b = A.bucket[e], k = bcnt[b];
 	const u32 bw0 = P.bucket_words[b];
-	u32 bw = bw0, appended = 0;
-	bool any_fail = false;
-	auto rep_of = [&](u32 s) { return (u32)G.tab[s]; };
-	auto upsert = [&](u32 s) {
-		const u32 j = rep_of(s);
-		const u32 *node = A.content + (u64)j * stride;
-		u32 nsz = leaf ? 2u : A.csize[j];
-		// the groups of a level hold pairwise distinct contents, so no group can match a node
-		// appended by an earlier group of this walk: the find among the existing nodes (the build)
-		// was the whole find
-		u32 f = kNull;
-		u32 fail = 0;
-		{
-			f = append_to_bucket(P.words, P.c, b, bw, node, nsz);
-			if (f == kNull)
-				fail = 1, any_fail = true;
-			else
-				++appended;
-		}
-		G.gres[s] = f;
-		G.gfail[s] = fail;
-	};
-	if (k <= kGather) {
-		u32 ord[kGather], key[kGather];
-		u32 g = 0;
-		for (u32 s = head[b]; s != kNull && g < kGather; s = G.gnext[s]) {
-			u32 r = rep_of(s), y = g++;
-			while (y > 0 && key[y - 1] > r) { // insertion sort by first entry
-				key[y] = key[y - 1];
-				ord[y] = ord[y - 1];
-				--y;
-			}
-			key[y] = r;
-			ord[y] = s;
-		}
-		for (u32 x = 0; x < g; ++x)
-			upsert(ord[x]);
-	} else { // long lists (distinct new nodes of one bucket): repeated minimum search
+	u32 appended = 0;
+	if (k <= 4u)
+		appended = walk_short<4>(P, leaf, A, G, b, k, bw0, blist);
+	else if (k <= kWalkList)
+		appended = walk_short<kWalkList>(P, leaf, A, G, b, k, bw0, blist);
+	else { // long lists (distinct new nodes of one bucket): repeated minimum search over the chain
+		u32 bw = bw0;
+		bool any_fail = false;
 		u32 last = 0;
 		for (u32 r = 0; r < k; ++r) {
 			u32 best = kNull, bs = kNull;
 			for (u32 s = head[b]; s != kNull; s = G.gnext[s]) {
-				u32 j = rep_of(s);
+				u32 j = (u32)G.tab[s];
 				if ((r == 0 || j > last) && j < best)
 					best = j, bs = s;
 			}
 			if (bs == kNull)
 				break;
-			upsert(bs);
+			const u32 *node = A.content + (u64)best * stride;
+			const u32 f = append_to_bucket(P.words, P.c, b, bw, node, leaf ? 2u : A.csize[best]);
+			if (f == kNull)
+				any_fail = true;
+			else
+				++appended;
+			G.gres[bs] = f;
+			G.gfail[bs] = f == kNull ? 1u : 0u;
 			last = best;
 		}
+		P.bucket_words[b] = bw;
+		if (any_fail)
+			atomicOr(&P.meta[kMetaOverflow], 1u);
 	}
-	P.bucket_words[b] = bw;
 	bcnt[b] = 0;
 	head[b] = kNull;
-	if (appended)
-		atomicAdd(&acc[kAccUpserts], appended);
-	if (any_fail)
-		atomicOr(&P.meta[kMetaOverflow], 1u);
+	return appended;
 }
 
 __global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel *__restrict__ lv,
@@ -1666,32 +1822,44 @@ __global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel
 	const u32 n = pm[kPmHdr + l];
 	for (u32 e0 = blockIdx.x * blockDim.x; e0 < n; e0 += gridDim.x * blockDim.x) { // uniform trip count
 		const u32 e = e0 + threadIdx.x;
-		bool todo = e < n;
-		do { // an entry whose bucket found the stage full is built again after the reset
+		Built o{};
+		o.k = 0;
+		if (e < n)
+			o = join_entry(P, l, A, C, G, e);
+		bool todo = o.k != 0;
+		do { // an entry whose bucket found the stage full looks again after the reset
 			if (threadIdx.x == 0)
 				top = 0;
 			__syncthreads();
 			if (todo)
-				todo = !build_entry(P, l, A, C, G, e, stage, &top, kCap);
+				todo = !find_entry(P, l, A, e, o, stage, &top, kCap);
 		} while (__syncthreads_or(todo));
 	}
 }
 __global__ void __launch_bounds__(kAT) a_dedup(EditParams P, u32 l, const ALevel *__restrict__ lv,
                                                const u32 *__restrict__ pm, GroupTable G, u32 *__restrict__ bcnt,
-                                               u32 *__restrict__ head) {
+                                               u32 *__restrict__ head, u32 *__restrict__ blist) {
 	const ALevel A = lv[l];
 	const u32 n = pm[kPmHdr + l], gen = pm[kPmGen] + l;
 	const bool leaf = l == P.c.L - 1;
 	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-		dedup_entry(A, G, gen, leaf, e, bcnt, head);
+		dedup_entry(A, G, gen, leaf, e, bcnt, head, blist);
 }
 __global__ void __launch_bounds__(kAT) a_walk(EditParams P, u32 l, const ALevel *__restrict__ lv,
                                               const u32 *__restrict__ pm, GroupTable G, u32 *__restrict__ bcnt,
-                                              u32 *__restrict__ head, u32 *__restrict__ acc) {
+                                              u32 *__restrict__ head, const u32 *__restrict__ blist,
+                                              u32 *__restrict__ acc) {
 	const ALevel A = lv[l];
 	const u32 n = pm[kPmHdr + l];
+	u32 app = 0;
 	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-		walk_entry(P, l, A, G, e, bcnt, head, acc);
+		app += walk_entry(P, l, A, G, e, bcnt, head, blist);
+	// one counter update per wavefront: a bump per walker would serialise on the one address
+#pragma unroll
+	for (u32 d = 32; d > 0; d >>= 1)
+		app += (u32)__shfl_xor((int)app, (int)d);
+	if ((threadIdx.x & 63u) == 0 && app)
+		atomicAdd(&acc[kAccUpserts], app);
 }
 
 // ---- colors: encode + SetLeaf (one block per color leaf job, grid-stride)
@@ -1746,35 +1914,78 @@ __device__ inline void enc_load(const Color *__restrict__ d, u32 V, EncRegs &R) 
 	R.x.heads = (u32)__builtin_popcount(R.heads);
 }
 
+// Encoded chunks of at most kEncTmp words are also written, in their final layout, to the job's
+// slot of a staging buffer by a_encode_size (built in LDS, copied out coalesced): a_encode_write
+// then copies them instead of reading the job's dense colors again (larger chunks are encoded
+// again from the dense colors).
+constexpr u32 kEncTmp = 4096;
 __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const ALevel *__restrict__ lv,
-                                                              const u32 *__restrict__ pm, const Color *__restrict__ dense,
-                                                              u32 V, u32 *__restrict__ info, u32 *__restrict__ app) {
+                                                              u32 *__restrict__ pm, const Color *__restrict__ dense,
+                                                              u32 V, u32 *__restrict__ info, u32 *__restrict__ app,
+                                                              u32 *__restrict__ etmp, u32 *__restrict__ rest) {
 	__shared__ EncScan sh[kEncThreads];
+	__shared__ u32 stmp[kEncTmp];
+	__shared__ u64 s_wm;
 	const u32 leafL = P.c.color_leaf;
 	const Entry *ent = lv[leafL].ent;
 	const u32 n = pm[kPmHdr + leafL];
-	for (u32 e = blockIdx.x; e < n; e += gridDim.x) {
-		Entry E = ent[e];
-		if (!(E.sf & kSfWriter)) {
-			if (threadIdx.x == 0)
-				app[e] = 0;
-			continue;
-		}
+	for (u32 c0 = blockIdx.x; c0 < n; c0 += gridDim.x * kJobChunk)
+	for (u64 m = writer_chunk(ent, n, c0, &s_wm, [&](u32 e) { app[e] = 0; }); m; m &= m - 1ull) {
+		const u32 e = c0 + (u32)__builtin_ctzll(m) * gridDim.x;
+		const Entry E = ent[e];
 		const Color *d = dense + (u64)e * V;
-		EncScan x{0, 0};
+		EncScan tot{0, 0};
+		bool staged = false;
 		if (V <= kEncPer * kEncThreads) {
 			EncRegs R;
 			enc_load(d, V, R);
-			x = R.x;
+			EncScan run = block_excl_scan(R.x, sh, tot);
+			// one macro block (V <= 2^14): [size, M = 1, B, W, macro heads, macro bits, B headers, weights]
+			const u32 B = tot.heads, W = (tot.bits + 31u) >> 5, ds = 6u + 2u * B + W;
+			staged = ds <= kEncTmp; // block-uniform
+			if (staged) {
+				const u32 bo = 6u, wo = bo + 2u * B;
+				for (u32 w = threadIdx.x; w < W; w += kEncThreads)
+					stmp[wo + w] = 0u;
+				if (threadIdx.x == 0) {
+					stmp[0] = 0u;
+					stmp[1] = 1u;
+					stmp[2] = B;
+					stmp[3] = W;
+					stmp[4] = 0u;
+					stmp[5] = 0u;
+				}
+				__syncthreads();
+#pragma unroll
+				for (u32 k = 0; k < kEncPer; ++k)
+					if (k < R.n) {
+						const u32 v = R.v0 + k, bpw = (R.w[k] >> 8) & 0xFFu;
+						if ((R.heads >> k) & 1u) {
+							stmp[bo + 2u * run.heads] = R.c[k];
+							stmp[bo + 2u * run.heads + 1u] = (v << 18) | (bpw << 16) | run.bits;
+							run.heads += 1u;
+						}
+						if (bpw) {
+							const u32 w = R.w[k] & ((1u << bpw) - 1u), o = run.bits & 31u;
+							atomicOr(&stmp[wo + (run.bits >> 5)], w << o);
+							if (o + bpw > 32u)
+								atomicOr(&stmp[wo + (run.bits >> 5) + 1u], w >> (32u - o));
+						}
+						run.bits += bpw;
+					}
+				__syncthreads();
+				for (u32 w = threadIdx.x; w < ds; w += kEncThreads)
+					etmp[(u64)e * kEncTmp + w] = stmp[w];
+			}
 		} else {
+			EncScan x{0, 0};
 			u32 per = (V + kEncThreads - 1) / kEncThreads, v0 = threadIdx.x * per, v1 = min(V, v0 + per);
 			for (u32 v = v0; v < v1; ++v) {
 				x.heads += is_head(d, v) ? 1u : 0u;
 				x.bits += (d[v].wb >> 8) & 0xFFu;
 			}
+			block_excl_scan(x, sh, tot);
 		}
-		EncScan tot;
-		block_excl_scan(x, sh, tot);
 		if (threadIdx.x == 0) {
 			u32 M = (V + 0x3FFFu) >> 14, W = (tot.bits + 31u) >> 5;
 			u32 ds = 2u * M + 2u * tot.heads + W + 4u;
@@ -1789,9 +2000,11 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const
 			}
 			info[4 * e] = ds;
 			info[4 * e + 1] = as;
-			info[4 * e + 2] = inplace;
+			info[4 * e + 2] = inplace | (staged ? 2u : 0u);
 			info[4 * e + 3] = tot.heads;
 			app[e] = as;
+			if (!staged)
+				rest[atomicAdd(&pm[kPmRest], 1u)] = e;
 		}
 	}
 }
@@ -1845,19 +2058,66 @@ __global__ void __launch_bounds__(kScanThreads) a_scan_app(u32 leafL, u32 *__res
 __global__ void __launch_bounds__(kEncThreads) a_encode_write(EditParams P, const ALevel *__restrict__ lv,
                                                                const u32 *__restrict__ pm, const Color *__restrict__ dense,
                                                                u32 V, const u32 *__restrict__ info,
-                                                               const u32 *__restrict__ appo, u64 leaf_cap) {
-	__shared__ EncScan sh[kEncThreads];
-	__shared__ u32 macro_ws[160];
+                                                               const u32 *__restrict__ appo, u64 leaf_cap,
+                                                               const u32 *__restrict__ etmp) {
+	// the staged chunks: a copy each (the writers whose chunk was too large are in a_encode_rest's list)
 	const u32 leafL = P.c.color_leaf;
 	const Entry *ent = lv[leafL].ent;
 	u32 *ores = lv[leafL].ores;
 	const u32 n = pm[kPmHdr + leafL];
 	const u32 leaf_base = P.meta[kMetaColorLeaves];
-	for (u32 e = blockIdx.x; e < n; e += gridDim.x) {
-		Entry E = ent[e];
+	__shared__ u64 s_wm;
+	for (u32 c0 = blockIdx.x; c0 < n; c0 += gridDim.x * kJobChunk)
+	for (u64 m = writer_chunk(ent, n, c0, &s_wm, [&](u32 e) { ores[e] = ldg(&ent[e].octree, 0); }); m;
+	     m &= m - 1ull) {
+		const u32 e = c0 + (u32)__builtin_ctzll(m) * gridDim.x;
+		const Entry E = ent[e];
 		u32 result = E.octree;
 		if (E.sf & kSfWriter) {
-			u32 ds = info[4 * e], as = info[4 * e + 1], inplace = info[4 * e + 2], B = info[4 * e + 3];
+			u32 ds = info[4 * e], as = info[4 * e + 1], inplace = info[4 * e + 2] & 1u, B = info[4 * e + 3];
+			const bool staged = (info[4 * e + 2] >> 1) & 1u;
+			u32 idx = inplace ? cdata(E.octree) : leaf_base + appo[e];
+			if (!staged)
+				continue; // a_encode_rest encodes it (and sets its result)
+			if (inplace || (u64)idx + as <= leaf_cap) { // the chunk a_encode_size built
+				u32 *L = P.cleaves;
+				const u32 *T = etmp + (u64)e * kEncTmp;
+				for (u32 w = 1u + threadIdx.x; w < ds; w += kEncThreads)
+					L[idx + w] = T[w];
+				if (!inplace) {
+					if (threadIdx.x == 0)
+						L[idx] = as;
+					for (u32 w = ds + threadIdx.x; w < as; w += kEncThreads)
+						L[idx + w] = 0u;
+				}
+				result = inplace ? E.octree : cmake(kTagLeaf, idx);
+			} else if (threadIdx.x == 0)
+				atomicOr(&P.meta[kMetaOverflow], 2u);
+		}
+		if (threadIdx.x == 0)
+			ores[e] = result;
+	}
+}
+// The writer jobs whose encoded chunk did not fit a staging slot (listed by a_encode_size): encoded
+// from the dense colors, one workgroup per job.
+__global__ void __launch_bounds__(kEncThreads) a_encode_rest(EditParams P, const ALevel *__restrict__ lv,
+                                                              const u32 *__restrict__ pm, const Color *__restrict__ dense,
+                                                              u32 V, const u32 *__restrict__ info,
+                                                              const u32 *__restrict__ appo, u64 leaf_cap,
+                                                              const u32 *__restrict__ rest) {
+	__shared__ EncScan sh[kEncThreads];
+	__shared__ u32 macro_ws[160];
+	const u32 leafL = P.c.color_leaf;
+	const Entry *ent = lv[leafL].ent;
+	u32 *ores = lv[leafL].ores;
+	const u32 leaf_base = P.meta[kMetaColorLeaves];
+	const u32 nrest = pm[kPmRest];
+	for (u32 i = blockIdx.x; i < nrest; i += gridDim.x) {
+		const u32 e = rest[i];
+		const Entry E = ent[e];
+		u32 result = E.octree;
+		if (E.sf & kSfWriter) {
+			u32 ds = info[4 * e], as = info[4 * e + 1], inplace = info[4 * e + 2] & 1u, B = info[4 * e + 3];
 			u32 idx = inplace ? cdata(E.octree) : leaf_base + appo[e];
 			if ((inplace || (u64)idx + as <= leaf_cap) && V <= kEncPer * kEncThreads) {
 				u32 *L = P.cleaves;
@@ -1900,7 +2160,7 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_write(EditParams P, cons
 					for (u32 w = ds + threadIdx.x; w < as; w += kEncThreads)
 						L[idx + w] = 0u;
 				result = inplace ? E.octree : cmake(kTagLeaf, idx);
-			} else if (inplace || (u64)idx + as <= leaf_cap) {
+			} else if (inplace || (u64)idx + as <= leaf_cap) { // color leaves of more than 4096 voxels
 				u32 *L = P.cleaves;
 				const Color *d = dense + (u64)e * V;
 				u32 M = (V + 0x3FFFu) >> 14;
@@ -3196,7 +3456,8 @@ static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
 	}
 	b += maxpairs * 24 + pp.ev_cap * sizeof(PushEvent);
 	if (pp.stateful)
-		b += pp.cap[leafL] * ((u64)sizeof(Color) << (3 * (vl - leafL))) + pp.cap[leafL] * 24;
+		b += pp.cap[leafL] * ((u64)sizeof(Color) << (3 * (vl - leafL))) + pp.cap[leafL] * (28 + 4 * (u64)kEncTmp) +
+		     pp.cap[L - 1] * (64 * sizeof(Color) + 8);
 	pp.bytes = b;
 	if (b > kAsyncBudget || (pp.stateful && vl - leafL > 7))
 		pp.async = false;
@@ -3205,12 +3466,13 @@ static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
 struct AsyncBufs {
 	std::vector<ALevel> lv;
 	ALevel *d_lv = nullptr;
-	u32 *pm = nullptr, *roots = nullptr, *acc = nullptr, *bcnt = nullptr, *head = nullptr;
-	u32 *flag = nullptr, *lcnt = nullptr, *info = nullptr, *app = nullptr, *appo = nullptr;
+	u32 *pm = nullptr, *roots = nullptr, *acc = nullptr, *bcnt = nullptr, *head = nullptr, *blist = nullptr;
+	u32 *flag = nullptr, *lcnt = nullptr, *info = nullptr, *app = nullptr, *appo = nullptr, *etmp = nullptr, *rest = nullptr;
 	uint4 *cstate = nullptr;
 	Pair *bsum = nullptr;
 	PushEvent *ev = nullptr;
-	Color *dense = nullptr;
+	Color *dense = nullptr, *lcol = nullptr;
+	u64 *lmask = nullptr;
 	hd_edit_desc *edits = nullptr;
 	GroupTable G{};
 	u64 tab_slots = 0;
@@ -3287,6 +3549,11 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		HD_RET(p.scratch.get<u32>(pre + "info", 4 * jobs, &B.info));
 		HD_RET(p.scratch.get<u32>(pre + "app", jobs, &B.app));
 		HD_RET(p.scratch.get<u32>(pre + "appo", jobs, &B.appo));
+		HD_RET(p.scratch.get<u32>(pre + "etmp", jobs * kEncTmp, &B.etmp));
+		HD_RET(p.scratch.get<u32>(pre + "rest", jobs, &B.rest));
+		const u64 leaves = std::max<u64>(cap[L - 1], 1);
+		HD_RET(p.scratch.get<Color>(pre + "lcol", leaves * 64, &B.lcol));
+		HD_RET(p.scratch.get<u64>(pre + "lmask", leaves, &B.lmask));
 	}
 	HD_RET(p.scratch.get<hd_edit_desc>(pre + "edits", std::max<size_t>(flat.size(), 1), &B.edits));
 	HD_TRY(hipMemcpyAsync(B.edits, flat.data(), flat.size() * sizeof(hd_edit_desc), hipMemcpyHostToDevice, p.stream));
@@ -3323,6 +3590,7 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 	bool fresh = p.scratch.bufs.find(kb) == p.scratch.bufs.end();
 	HD_RET(p.scratch.get<u32>(kb, p.total_buckets, &B.bcnt));
 	HD_RET(p.scratch.get<u32>(kh, p.total_buckets, &B.head));
+	HD_RET(p.scratch.get<u32>("a_blist", (u64)p.total_buckets * kWalkList, &B.blist));
 	if (fresh) {
 		u32 g = std::min<u64>(div_up(p.total_buckets, 256), 4096);
 		a_fill_u32<<<g, 256, 0, p.stream>>>(B.bcnt, p.total_buckets, 0u);
@@ -3391,14 +3659,17 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		a_emit<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.bsum);
 	}
 	const bool colors = pp.stateful && pp.cap[leafL] > 0;
-	if (colors) {
-		a_dense_init<<<grid_for(pp.cap[leafL], 1, 8192), kDenseThreads, 0, p.stream>>>(B.d_lv, leafL, B.pm,
-		                                                                            3u * P.leaf_bits, p.cleaves, B.dense);
+	if (colors) { // the jobs' dense colors on the color stream, beside the leaves
+		HD_TRY(hipEventRecord(p.ev_top, p.stream));
+		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_top, 0));
+		a_dense_init<<<grid_for(pp.cap[leafL], 1, 8192), kDenseThreads, 0, p.cstream>>>(B.d_lv, leafL, B.pm,
+		                                                                             3u * P.leaf_bits, p.cleaves, B.dense);
 		if (pp.ev_cap)
-			a_push<<<grid_for(pp.ev_cap, 1, 8192), 256, 0, p.stream>>>(B.ev, B.pm, (u32)pp.ev_cap, V,
-			                                                          B.lv[leafL].cap, B.dense);
+			a_push<<<grid_for(pp.ev_cap, 1, 8192), 256, 0, p.cstream>>>(B.ev, B.pm, (u32)pp.ev_cap, V,
+			                                                           B.lv[leafL].cap, B.dense);
 	}
-	a_leaf<<<grid_for(pp.cap[L - 1], kAT / 64, 16384), kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.dense, V);
+	const u32 leaf_grid = grid_for(pp.cap[L - 1], kAT / 64, 16384);
+	a_leaf<<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
 	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
 	const u64 small_level = std::min<u32>(g_tuning.small_level, kSmallLevel);
 	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= small_level)
@@ -3406,10 +3677,14 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	if (colors) { // the leaf encode and SetNode on the color stream, beside the node levels
 		HD_TRY(hipEventRecord(p.ev_leaf, p.stream));
 		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_leaf, 0));
+		a_leafcolor<<<leaf_grid, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.dense, V);
 		u32 G = grid_for(pp.cap[leafL], 1, 8192);
-		a_encode_size<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app);
+		a_encode_size<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app, B.etmp, B.rest);
 		a_scan_app<<<1, kScanThreads, 0, p.cstream>>>(leafL, B.pm, B.app, B.appo);
-		a_encode_write<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap);
+		a_encode_write<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap,
+		                                                 B.etmp);
+		a_encode_rest<<<std::min(G, 1024u), kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo,
+		                                                                 p.cleaf_cap, B.rest);
 		a_setnode<<<1, kSetNodeThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
 		HD_TRY(hipEventRecord(p.ev_color, p.cstream));
 	}
@@ -3417,8 +3692,8 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		u32 G = grid_for(pp.cap[l], kAT, 4096);
 		if (l + 1 < (int)L)
 			a_build<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G);
-		a_dedup<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head);
-		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.acc);
+		a_dedup<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.blist);
+		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.blist, B.acc);
 	}
 	a_bottom<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, l_from, B.G, B.roots, B.acc, p.cnode_cap);
 	if (colors) // the next pass (and any host read) follows both streams

@@ -99,7 +99,7 @@ struct Pool {
 	// color side stream of a sync-free pass: the leaf encode and SetNode run there, beside the node
 	// levels (joined before the next pass)
 	hipStream_t cstream = nullptr;
-	hipEvent_t ev_leaf = nullptr, ev_color = nullptr;
+	hipEvent_t ev_top = nullptr, ev_leaf = nullptr, ev_color = nullptr;
 	Scratch scratch;
 	// statistics of the last edit (host)
 	u64 last_entries = 0, last_upserts = 0, last_jobs = 0;

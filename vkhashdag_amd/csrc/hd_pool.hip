@@ -177,6 +177,8 @@ int hd_pool_create(const hd_config *cfg, hd_pool **out) {
 		return fail(e);
 	if ((e = hipStreamCreateWithFlags(&p.cstream, hipStreamNonBlocking)) != hipSuccess)
 		return fail(e);
+	if ((e = hipEventCreateWithFlags(&p.ev_top, hipEventDisableTiming)) != hipSuccess)
+		return fail(e);
 	if ((e = hipEventCreateWithFlags(&p.ev_leaf, hipEventDisableTiming)) != hipSuccess)
 		return fail(e);
 	if ((e = hipEventCreateWithFlags(&p.ev_color, hipEventDisableTiming)) != hipSuccess)
@@ -226,7 +228,7 @@ int hd_pool_destroy(hd_pool *h) {
 		(void)hipStreamDestroy(p.stream);
 	if (p.cstream)
 		(void)hipStreamDestroy(p.cstream);
-	for (hipEvent_t ev : {p.ev_leaf, p.ev_color})
+	for (hipEvent_t ev : {p.ev_top, p.ev_leaf, p.ev_color})
 		if (ev)
 			(void)hipEventDestroy(ev);
 	delete h;
