@@ -836,6 +836,8 @@ struct ALevel {
 	u32 *status, *pend, *res, *slot, *lpos, *bucket;
 	u32 *content, *csize;
 	u32 *ores, *ccnt, *cbase, *own; // color results, SetNode passes
+	uint4 *pev; // 8 per entry (levels color leaf .. L - 2, color leaves rebuilt in LDS): a pushed
+	            // child's {voxel offset in its job, colors, wb, 0}, valid where pmask bit 16 + slot is set
 	u32 cap, list_cap;
 };
 
@@ -904,7 +906,10 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 				c.sf |= kSfActive | (cl == leafL ? (kSfWriter | kSfNewJob) : 0u);
 		}
 	}
-	if (c.push) { // one bump of the shared event counter per wavefront (the active pushing lanes)
+	if (c.push && A.pev) { // the job rebuilds its colors from its pairs' records
+		const u32 d = cl - leafL, msk = (1u << d) - 1u, sh = 3u * (P.c.voxel_level - cl);
+		stg(A.pev, t, make_uint4(morton3(cx & msk, cy & msk, cz & msk) << sh, c.col.colors, c.col.wb, 0u));
+	} else if (c.push) { // one bump of the shared event counter per wavefront (the active pushing lanes)
 		u32 d = cl - leafL, msk = (1u << d) - 1u, sh = 3u * (P.c.voxel_level - cl);
 		const u64 m = __ballot(1);
 		const u32 lane = __lane_id(), leader = (u32)__builtin_ctzll(m);
@@ -922,8 +927,8 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 	if (A.okids)
 		A.okids[t] = c.octree;
 	bool proceed = c.t == kProceed;
-	// bit 0: proceeds, bit 1: a terminal child whose pointer changed
-	u32 flag = (proceed ? 1u : 0u) | (!proceed && kid != old ? 2u : 0u);
+	// bit 0: proceeds, bit 1: a terminal child whose pointer changed, bit 2: pushed a color
+	u32 flag = (proceed ? 1u : 0u) | (!proceed && kid != old ? 2u : 0u) | (c.push ? 4u : 0u);
 	return PairOut{flag, proceed && cnt > 1 ? cnt : 0u, make_uint4(c.octree, c.sf, single, cnt), kid};
 }
 __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &A, u64 t, PushEvent *__restrict__ ev,
@@ -932,8 +937,8 @@ __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &
 }
 
 // Writes the child entry of a proceeding pair (k_emit). m8: proceed flags of the pair's node in
-// bits 0-7 and its changed terminal children in bits 8-15 (valid on slot 0), idx / lo: the child's
-// entry index and list offset at level l + 1.
+// bits 0-7, its changed terminal children in bits 8-15 and its pushed children in bits 16-23 (valid
+// on slot 0), idx / lo: the child's entry index and list offset at level l + 1.
 __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A, const ALevel &N,
                                     u32 *__restrict__ pm, u64 t, const Entry &E, u32 kid, u32 m8, const PairOut &o,
                                     u32 idx, u32 lo) {
@@ -1021,7 +1026,7 @@ __device__ inline Pair expand_emit_range(const EditParams &P, u32 l, const ALeve
 			if ((t & 7u) == 0)
 				for (u32 k = 0; k < 8; ++k) {
 					u32 f = sflag[threadIdx.x + k];
-					m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k));
+					m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k)) | (((f >> 2) & 1u) << (16 + k));
 				}
 			emit_pair(P, l, A, N, pm, t, m8, o, base.a + off.a, base.b + off.b);
 		}
@@ -1132,7 +1137,7 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 				E = cur[lane >> 3];
 				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap);
 			}
-			const u64 bp = __ballot(o.flag & 1u), bt = __ballot((o.flag >> 1) & 1u);
+			const u64 bp = __ballot(o.flag & 1u), bt = __ballot((o.flag >> 1) & 1u), bu = __ballot((o.flag >> 2) & 1u);
 			const u64 below = (1ull << lane) - 1ull;
 			u32 lc = o.lcnt; // inclusive wave scan of the list lengths
 #pragma unroll
@@ -1143,7 +1148,9 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			}
 			const Pair tot{(u32)__popcll(bp), __shfl(lc, 63)};
 			if (in) {
-				const u32 m8 = (lane & 7u) == 0 ? (u32)((bp >> lane) & 0xFFu) | ((u32)((bt >> lane) & 0xFFu) << 8) : 0u;
+				const u32 m8 = (lane & 7u) == 0 ? (u32)((bp >> lane) & 0xFFu) | ((u32)((bt >> lane) & 0xFFu) << 8) |
+				                                      ((u32)((bu >> lane) & 0xFFu) << 16)
+				                                : 0u;
 				const u32 idx = (u32)__popcll(bp & below);
 				Entry ne = emit_pair_e(P, l, A, N, pm, lane, E, o.kid, m8, o, idx, lc - o.lcnt);
 				if ((o.flag & 1u) && idx < kTopEnt)
@@ -1185,7 +1192,7 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 				if ((t & 7u) == 0)
 					for (u32 k = 0; k < 8; ++k) {
 						u32 f = sflag[threadIdx.x + k];
-						m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k));
+						m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k)) | (((f >> 2) & 1u) << (16 + k));
 					}
 				const u32 idx = base.a + off.a;
 				Entry ne = emit_pair_e(P, l, A, N, pm, t, E, o.kid, m8, o, idx, base.b + off.b);
@@ -1280,6 +1287,74 @@ __device__ inline u64 writer_chunk(const Entry *__restrict__ ent, u32 n, u32 c0,
 constexpr u32 kJobChunk = 64;
 
 constexpr u32 kDenseThreads = 256, kDenseRun = 16, kDenseHdrWords = 8192;
+// The job's current colors in voxel order (DAGColorPool's leaf decode): a uniform fill, or the
+// chunk's headers staged in LDS (hdr, kDenseHdrWords) and runs of kDenseRun voxels decoded per
+// thread by seeking the run's header and walking forward. out may be global or LDS memory; every
+// thread of the block calls this.
+__device__ inline void decode_job(const Entry &E, const u32 *__restrict__ cleaves, u32 *hdr, Color *out, u32 V) {
+	if (ctag(E.octree) != kTagLeaf) {
+		const Color fill = get_fill(E.octree);
+		for (u32 v = threadIdx.x; v < V; v += blockDim.x)
+			out[v] = fill;
+		return;
+	}
+	const u32 idx = cdata(E.octree), M = cleaves[idx + 1u], B = cleaves[idx + 2u];
+	if (2u * (M + B) > kDenseHdrWords) { // a chunk too large to stage: decode each voxel in place
+		for (u32 v = threadIdx.x; v < V; v += blockDim.x)
+			out[v] = chunk_voxel(cleaves, idx, v);
+		return;
+	}
+	const u32 mo = idx + 4u, wo = mo + 2u * M + 2u * B;
+	__syncthreads(); // the previous job's readers are done with hdr
+	for (u32 k = threadIdx.x; k < 2u * (M + B); k += blockDim.x)
+		hdr[k] = cleaves[mo + k];
+	__syncthreads();
+	const u32 *mac = hdr, *hd = hdr + 2u * M;
+	for (u32 v0 = threadIdx.x * kDenseRun; v0 < V; v0 += blockDim.x * kDenseRun) {
+		u32 m = v0 >> 14, h = 0, lb = 0, ws = 0;
+		auto seek = [&](u32 vid) { // last header of macro m with voxoff <= vid
+			const u32 fb = mac[2u * m];
+			lb = m + 1u < M ? mac[2u * (m + 1u)] : B;
+			ws = mac[2u * m + 1u];
+			u32 lo = fb, cnt = lb - fb;
+			while (cnt > 0) {
+				const u32 step = cnt >> 1;
+				if ((hd[2u * (lo + step) + 1u] >> 18) <= vid)
+					lo += step + 1u, cnt -= step + 1u;
+				else
+					cnt = step;
+			}
+			h = lo > fb ? lo - 1u : fb; // a macro's first header starts at voxel 0
+		};
+		if (m < M)
+			seek(v0 & 0x3FFFu);
+		for (u32 k = 0; k < kDenseRun && v0 + k < V; ++k) {
+			const u32 v = v0 + k, vid = v & 0x3FFFu;
+			if ((v >> 14) != m) { // a new macro block starts
+				m = v >> 14;
+				if (m < M)
+					seek(vid);
+			}
+			Color c = Color::none();
+			if (m < M) {
+				while (h + 1u < lb && (hd[2u * (h + 1u) + 1u] >> 18) <= vid)
+					++h;
+				const u32 hc = hd[2u * h], hp = hd[2u * h + 1u], bpw = (hp >> 16) & 3u;
+				if (bpw == 0)
+					c = Color{hc, 1u << 16};
+				else {
+					const u32 bit = ws + (hp & 0xFFFFu) + (vid - (hp >> 18)) * bpw, o = bit & 31u;
+					u32 w = cleaves[wo + (bit >> 5)] >> o;
+					if (o + bpw > 32u)
+						w |= cleaves[wo + (bit >> 5) + 1u] << (32u - o);
+					w &= (1u << bpw) - 1u;
+					c = Color{hc, w | (bpw << 8) | (1u << 16)};
+				}
+			}
+			out[v] = c;
+		}
+	}
+}
 __global__ void __launch_bounds__(kDenseThreads) a_dense_init(const ALevel *__restrict__ lv, u32 leafL,
                                                               const u32 *__restrict__ pm, u32 log2v,
                                                               const u32 *__restrict__ cleaves,
@@ -1293,68 +1368,7 @@ __global__ void __launch_bounds__(kDenseThreads) a_dense_init(const ALevel *__re
 		const u32 job = c0 + (u32)__builtin_ctzll(m) * gridDim.x;
 		const Entry E = ent[job];
 		Color *out = dense + (u64)job * V;
-		if (ctag(E.octree) != kTagLeaf) {
-			const Color fill = get_fill(E.octree);
-			for (u32 v = threadIdx.x; v < V; v += blockDim.x)
-				out[v] = fill;
-			continue;
-		}
-		const u32 idx = cdata(E.octree), M = cleaves[idx + 1u], B = cleaves[idx + 2u];
-		if (2u * (M + B) > kDenseHdrWords) { // a chunk too large to stage: decode each voxel in place
-			for (u32 v = threadIdx.x; v < V; v += blockDim.x)
-				out[v] = chunk_voxel(cleaves, idx, v);
-			continue;
-		}
-		const u32 mo = idx + 4u, wo = mo + 2u * M + 2u * B;
-		__syncthreads(); // the previous job's readers are done with hdr
-		for (u32 k = threadIdx.x; k < 2u * (M + B); k += blockDim.x)
-			hdr[k] = cleaves[mo + k];
-		__syncthreads();
-		const u32 *mac = hdr, *hd = hdr + 2u * M;
-		for (u32 v0 = threadIdx.x * kDenseRun; v0 < V; v0 += blockDim.x * kDenseRun) {
-			u32 m = v0 >> 14, h = 0, lb = 0, ws = 0;
-			auto seek = [&](u32 vid) { // last header of macro m with voxoff <= vid
-				const u32 fb = mac[2u * m];
-				lb = m + 1u < M ? mac[2u * (m + 1u)] : B;
-				ws = mac[2u * m + 1u];
-				u32 lo = fb, cnt = lb - fb;
-				while (cnt > 0) {
-					const u32 step = cnt >> 1;
-					if ((hd[2u * (lo + step) + 1u] >> 18) <= vid)
-						lo += step + 1u, cnt -= step + 1u;
-					else
-						cnt = step;
-				}
-				h = lo > fb ? lo - 1u : fb; // a macro's first header starts at voxel 0
-			};
-			if (m < M)
-				seek(v0 & 0x3FFFu);
-			for (u32 k = 0; k < kDenseRun && v0 + k < V; ++k) {
-				const u32 v = v0 + k, vid = v & 0x3FFFu;
-				if ((v >> 14) != m) { // a new macro block starts
-					m = v >> 14;
-					if (m < M)
-						seek(vid);
-				}
-				Color c = Color::none();
-				if (m < M) {
-					while (h + 1u < lb && (hd[2u * (h + 1u) + 1u] >> 18) <= vid)
-						++h;
-					const u32 hc = hd[2u * h], hp = hd[2u * h + 1u], bpw = (hp >> 16) & 3u;
-					if (bpw == 0)
-						c = Color{hc, 1u << 16};
-					else {
-						const u32 bit = ws + (hp & 0xFFFFu) + (vid - (hp >> 18)) * bpw, o = bit & 31u;
-						u32 w = cleaves[wo + (bit >> 5)] >> o;
-						if (o + bpw > 32u)
-							w |= cleaves[wo + (bit >> 5) + 1u] << (32u - o);
-						w &= (1u << bpw) - 1u;
-						c = Color{hc, w | (bpw << 8) | (1u << 16)};
-					}
-				}
-				out[v] = c;
-			}
-		}
+		decode_job(E, cleaves, hdr, out, V);
 	}
 }
 // Pushes cover terminal subtrees and leaf edits cover proceeding leaves: disjoint voxel ranges,
@@ -1575,7 +1589,7 @@ __device__ inline Built build_node(const EditParams &P, u32 l, const JoinIn &q, 
 	const u32 pmw = q.pmw, pmk = pmw & 0xFFu;
 	const uint4 k0 = q.k0, k1 = q.k1;
 	const u32 kv[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
-	bool changed = (pmw >> 8) != 0;
+	bool changed = ((pmw >> 8) & 0xFFu) != 0;
 	u32 nm = 0, nk[8];
 #pragma unroll
 	for (u32 i = 0; i < 8; ++i) {
@@ -1708,7 +1722,6 @@ __device__ inline u32 walk_short(const EditParams &P, bool leaf, const ALevel &A
 	u32 appended = 0;
 	bool any_fail = false;
 	const u64 bb = (u64)b << P.c.wbpb();
-#pragma unroll
 	for (u32 r = 0; r < kG; ++r) { // append_to_bucket in rank order: positions and page-tail zeros
 		if (r >= k)
 			break;
@@ -2008,6 +2021,268 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const
 		}
 	}
 }
+// Colored passes whose color leaves hold at most kJobV voxels (so at most two node levels lie
+// between a job and its 4^3 leaves). a_job_ranges, one thread per writer job: its contiguous entry
+// ranges at those levels (from first_child / pmask) in jr[2 j], and the header of its current
+// chunk {M, B, block size, 1} (or zeros for a uniform color) in jr[2 j + 1].
+constexpr u32 kJobV = 4096;
+__global__ void a_job_ranges(const ALevel *__restrict__ lv, const u32 *__restrict__ pm, u32 leafL, u32 L,
+                             const u32 *__restrict__ cleaves, uint4 *__restrict__ jr) {
+	const Entry *ent = lv[leafL].ent;
+	const u32 n = pm[kPmHdr + leafL];
+	for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+		if (!(ldg(&ent[j].sf, 0) & kSfWriter))
+			continue;
+		const u32 oct = ldg(&ent[j].octree, 0);
+		uint4 hdr = make_uint4(0, 0, 0, 0);
+		if (ctag(oct) == kTagLeaf)
+			hdr = make_uint4(ldg(cleaves, (u64)cdata(oct) + 1u), ldg(cleaves, (u64)cdata(oct) + 2u),
+			                 ldg(cleaves, (u64)cdata(oct)), 1u);
+		u32 lo = j, hi = j + 1u, r[4] = {0, 0, 0, 0};
+#pragma unroll
+		for (u32 k = 0; k < 2; ++k) {
+			const u32 l = leafL + k;
+			if (l + 1u < L && hi > lo) { // an empty range stays empty
+				const u32 *fc = lv[l].first_child, *pmk = lv[l].pmask;
+				const u32 nlo = ldg(fc, lo);
+				const u32 nhi = ldg(fc, hi - 1u) + (u32)__builtin_popcount(ldg(pmk, hi - 1u) & 0xFFu);
+				lo = nlo;
+				hi = max(nhi, nlo);
+			} else if (l + 1u < L)
+				lo = hi = 0;
+			r[2 * k] = lo;
+			r[2 * k + 1] = hi;
+		}
+		jr[2ull * j] = make_uint4(r[0], r[1], r[2], r[3]);
+		jr[2ull * j + 1u] = hdr;
+	}
+}
+
+// One workgroup per writer job rebuilds the job's colors in LDS and encodes them as a_encode_size
+// does (SetLeaf's size decision, the staged chunk; a chunk too large to stage leaves the colors in
+// dense for a_encode_rest). The colors pushed onto its terminal subtrees (pev records of the pairs
+// whose pmask push bit is set) and the colors its leaves' edits assigned (lcol / lmask) are written
+// first and marked covered (pushes cover terminal subtrees, leaves proceeding ones: disjoint); the
+// job's current colors (DAGColorPool's leaf decode, headers staged in LDS) fill the rest. Every
+// load of a phase is independent of the others in it: a job costs about three memory round trips.
+constexpr u32 kJobHdr = 4096; // staged header words (larger chunks are decoded voxel by voxel)
+struct JobLds {
+	Color col[kJobV];
+	union {
+		u32 hdr[kJobHdr];
+		u32 stmp[kEncTmp];
+	};
+	u32 cov[kJobV / 32];
+	uint4 fill[kJobV / 64 + 8]; // pushes: {voxel offset, colors, wb, voxel count}
+	u32 nfill;
+	u32 lbase[kJobV / 64];      // per leaf of the job: its first voxel, or kNull without assigned colors
+	u64 lmask[kJobV / 64];
+};
+static_assert(kEncTmp <= kJobHdr, "the chunk stage shares the header stage");
+__global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, const ALevel *__restrict__ lv,
+                                                             u32 *__restrict__ pm, const uint4 *__restrict__ jrange,
+                                                             const Color *__restrict__ lcol,
+                                                             const u64 *__restrict__ lmask, Color *__restrict__ dense,
+                                                             u32 V, u32 *__restrict__ info, u32 *__restrict__ app,
+                                                             u32 *__restrict__ etmp, u32 *__restrict__ rest) {
+	__shared__ JobLds S;
+	__shared__ EncScan sh[kEncThreads];
+	__shared__ u64 s_wm;
+	const u32 leafL = P.c.color_leaf, L = P.c.L, vl = P.c.voxel_level;
+	const u32 *cleaves = P.cleaves;
+	const Entry *ent = lv[leafL].ent;
+	const u32 n = pm[kPmHdr + leafL];
+	const u32 below = L - 1u - leafL; // node levels from the job down to the leaves (0, 1 or 2)
+	for (u32 c0 = blockIdx.x; c0 < n; c0 += gridDim.x * kJobChunk)
+	for (u64 m = writer_chunk(ent, n, c0, &s_wm, [&](u32 e) { app[e] = 0; }); m; m &= m - 1ull) {
+		const u32 e = c0 + (u32)__builtin_ctzll(m) * gridDim.x;
+		__syncthreads(); // the previous job's readers are done with S
+		for (u32 w = threadIdx.x; w < V / 32u; w += kEncThreads)
+			S.cov[w] = 0u;
+		if (threadIdx.x == 0)
+			S.nfill = 0;
+		const Entry E = ent[e];
+		const uint4 jr = jrange[2ull * e], jh = jrange[2ull * e + 1u];
+		__syncthreads();
+		const bool chunk = ctag(E.octree) == kTagLeaf;
+		const u32 idx = cdata(E.octree), M = jh.x, B0 = jh.y, block = jh.z;
+		// pushes (the job's own 8 pairs, then the pairs of its entries one level down) and the
+		// leaves' records, each in one round trip
+		{
+			const u32 np0 = below ? 8u : 0u, np1 = below > 1u ? 8u * (jr.y - jr.x) : 0u;
+			for (u32 t = threadIdx.x; t < np0 + np1; t += kEncThreads) {
+				const u32 l = t < np0 ? leafL : leafL + 1u;
+				const u32 pe = t < np0 ? e : jr.x + ((t - np0) >> 3), i = t & 7u;
+				const u32 pmw = ldg(lv[l].pmask, pe);
+				const uint4 q = ldg(lv[l].pev, 8ull * pe + i); // read ahead: valid where the push bit is set
+				if ((pmw >> (16u + i)) & 1u)
+					S.fill[atomicAdd(&S.nfill, 1u)] = make_uint4(q.x, q.y, q.z, 1u << (3u * (vl - (l + 1u))));
+			}
+		}
+		const ALevel &LA = lv[L - 1u];
+		const u32 llo = below == 0u ? e : (below == 1u ? jr.x : jr.z);
+		const u32 nl = below == 0u ? 1u : (below == 1u ? jr.y - jr.x : jr.w - jr.z);
+		if (threadIdx.x < nl) {
+			const u32 le = llo + threadIdx.x, m2 = (1u << P.leaf_bits >> 2) - 1u;
+			Entry LE{};
+			LE.sf = ldg(&LA.ent[le].sf, 0);
+			LE.job = ldg(&LA.ent[le].job, 0);
+			LE.x = ldg(&LA.ent[le].x, 0);
+			LE.y = ldg(&LA.ent[le].y, 0);
+			LE.z = ldg(&LA.ent[le].z, 0);
+			const u64 am = ldg(lmask, le);
+			// a leaf's lane is the Morton index of its voxel inside the 4^3 leaf
+			S.lbase[threadIdx.x] = leaf_writes_colors(P, lv, LE) ? morton3(LE.x & m2, LE.y & m2, LE.z & m2) << 6 : kNull;
+			S.lmask[threadIdx.x] = am;
+		}
+		const bool staged_hdr = chunk && 2u * (M + B0) <= kJobHdr;
+		if (staged_hdr)
+			for (u32 k = threadIdx.x; k < 2u * (M + B0); k += kEncThreads)
+				S.hdr[k] = cleaves[idx + 4u + k];
+		constexpr u32 kPer = kJobV / kEncThreads;
+		Color c[kPer]; // the leaves' colors, read ahead of their masks
+#pragma unroll
+		for (u32 r = 0; r < kPer; ++r) {
+			const u32 k = threadIdx.x + r * kEncThreads;
+			if ((k >> 6) < nl)
+				c[r] = ldg(lcol, 64ull * llo + k);
+		}
+		__syncthreads();
+		// the pushed and assigned colors (loads of all of a thread's voxels first), marked covered
+		for (u32 f = 0; f < S.nfill; ++f) {
+			const uint4 q = S.fill[f];
+			for (u32 v = threadIdx.x; v < q.w; v += kEncThreads)
+				S.col[q.x + v] = Color{q.y, q.z};
+			if (q.w >= 32u)
+				for (u32 w = (q.x >> 5) + threadIdx.x; w < (q.x + q.w) >> 5; w += kEncThreads)
+					S.cov[w] = 0xFFFFFFFFu;
+			else if (threadIdx.x == 0)
+				atomicOr(&S.cov[q.x >> 5], ((1u << q.w) - 1u) << (q.x & 31u));
+		}
+#pragma unroll
+		for (u32 r = 0; r < kPer; ++r) {
+			const u32 k = threadIdx.x + r * kEncThreads, li = k >> 6, lane = k & 63u;
+			if (li < nl && S.lbase[li] != kNull && ((S.lmask[li] >> lane) & 1ull)) {
+				const u32 at = S.lbase[li] | lane;
+				S.col[at] = c[r];
+				atomicOr(&S.cov[at >> 5], 1u << (at & 31u));
+			}
+		}
+		__syncthreads();
+		// the current colors of the voxels not covered above
+		if (!chunk || M == 0u || B0 == 0u) {
+			const Color fill = chunk ? Color::none() : get_fill(E.octree);
+			for (u32 v = threadIdx.x; v < V; v += kEncThreads)
+				if (!((S.cov[v >> 5] >> (v & 31u)) & 1u))
+					S.col[v] = fill;
+		} else if (!staged_hdr) {
+			for (u32 v = threadIdx.x; v < V; v += kEncThreads)
+				if (!((S.cov[v >> 5] >> (v & 31u)) & 1u))
+					S.col[v] = chunk_voxel(cleaves, idx, v);
+		} else {
+			const u32 *mac = S.hdr, *hd = S.hdr + 2u * M, wo = idx + 4u + 2u * M + 2u * B0;
+			for (u32 v0 = threadIdx.x * kDenseRun; v0 < V; v0 += kEncThreads * kDenseRun) {
+				// V <= 4096 < 2^14: one macro block; the last header with voxoff <= v0, then forward
+				const u32 ws = mac[1];
+				u32 lo = mac[0], cnt = (M > 1u ? mac[2] : B0) - lo;
+				const u32 lb = lo + cnt;
+				const u32 fb = lo;
+				while (cnt > 0) {
+					const u32 step = cnt >> 1;
+					if ((hd[2u * (lo + step) + 1u] >> 18) <= v0)
+						lo += step + 1u, cnt -= step + 1u;
+					else
+						cnt = step;
+				}
+				u32 h = lo > fb ? lo - 1u : fb;
+				for (u32 k = 0; k < kDenseRun && v0 + k < V; ++k) {
+					const u32 v = v0 + k;
+					while (h + 1u < lb && (hd[2u * (h + 1u) + 1u] >> 18) <= v)
+						++h;
+					if ((S.cov[v >> 5] >> (v & 31u)) & 1u)
+						continue;
+					Color c;
+					const u32 hc = hd[2u * h], hp = hd[2u * h + 1u], bpw = (hp >> 16) & 3u;
+					if (bpw == 0)
+						c = Color{hc, 1u << 16};
+					else {
+						const u32 bit = ws + (hp & 0xFFFFu) + (v - (hp >> 18)) * bpw, o = bit & 31u;
+						u32 w = cleaves[wo + (bit >> 5)] >> o;
+						if (o + bpw > 32u)
+							w |= cleaves[wo + (bit >> 5) + 1u] << (32u - o);
+						w &= (1u << bpw) - 1u;
+						c = Color{hc, w | (bpw << 8) | (1u << 16)};
+					}
+					S.col[v] = c;
+				}
+			}
+		}
+		__syncthreads();
+		// encode: a_encode_size's register path over the LDS colors
+		EncRegs R;
+		enc_load(S.col, V, R);
+		EncScan tot;
+		EncScan run = block_excl_scan(R.x, sh, tot);
+		const u32 B = tot.heads, W = (tot.bits + 31u) >> 5, ds = 6u + 2u * B + W;
+		const bool staged = ds <= kEncTmp; // block-uniform
+		if (staged) {
+			const u32 bo = 6u, wo = bo + 2u * B;
+			for (u32 w = threadIdx.x; w < W; w += kEncThreads)
+				S.stmp[wo + w] = 0u;
+			if (threadIdx.x == 0) {
+				S.stmp[0] = 0u;
+				S.stmp[1] = 1u;
+				S.stmp[2] = B;
+				S.stmp[3] = W;
+				S.stmp[4] = 0u;
+				S.stmp[5] = 0u;
+			}
+			__syncthreads();
+#pragma unroll
+			for (u32 k = 0; k < kEncPer; ++k)
+				if (k < R.n) {
+					const u32 v = R.v0 + k, bpw = (R.w[k] >> 8) & 0xFFu;
+					if ((R.heads >> k) & 1u) {
+						S.stmp[bo + 2u * run.heads] = R.c[k];
+						S.stmp[bo + 2u * run.heads + 1u] = (v << 18) | (bpw << 16) | run.bits;
+						run.heads += 1u;
+					}
+					if (bpw) {
+						const u32 w = R.w[k] & ((1u << bpw) - 1u), o = run.bits & 31u;
+						atomicOr(&S.stmp[wo + (run.bits >> 5)], w << o);
+						if (o + bpw > 32u)
+							atomicOr(&S.stmp[wo + (run.bits >> 5) + 1u], w >> (32u - o));
+					}
+					run.bits += bpw;
+				}
+			__syncthreads();
+			for (u32 w = threadIdx.x; w < ds; w += kEncThreads)
+				etmp[(u64)e * kEncTmp + w] = S.stmp[w];
+		} else { // a_encode_rest encodes it from the dense colors
+			Color *out = dense + (u64)e * V;
+			for (u32 v = threadIdx.x; v < V; v += kEncThreads)
+				out[v] = S.col[v];
+		}
+		if (threadIdx.x == 0) {
+			u32 as = (ds & 1u) ? ds + 1u : ds;
+			u32 inplace = 0;
+			if (!P.c.keep_history && chunk) { // DAGColorPool::SetLeaf (:172-204)
+				if (ds <= block)
+					inplace = 1, as = 0;
+				else
+					as = max(block << 1, as);
+			}
+			info[4 * e] = ds;
+			info[4 * e + 1] = as;
+			info[4 * e + 2] = inplace | (staged ? 2u : 0u);
+			info[4 * e + 3] = tot.heads;
+			app[e] = as;
+			if (!staged)
+				rest[atomicAdd(&pm[kPmRest], 1u)] = e;
+		}
+	}
+}
+
 // single block: exclusive prefix of the append sizes in entry (DFS) order
 // Exclusive scan of the jobs' encoded sizes, one workgroup: 1024 threads x 8 consecutive jobs per
 // tile (a running sum in registers, a wave scan, the waves' totals through LDS).
@@ -3374,6 +3649,12 @@ struct PassPlan {
 	u64 ev_cap = 0, bytes = 0;
 };
 
+// Color leaves of at most kJobV voxels (and no deeper than the node leaves) are rebuilt in LDS by
+// a_job_encode: pushes go to per-pair records instead of the event list.
+static bool lds_jobs(const DevConfig &c) {
+	return c.color_leaf + 1u <= c.L && 3u * (c.voxel_level - c.color_leaf) <= 12u;
+}
+
 static u64 edit_level_bound(const hd_edit_desc &d, u32 voxel_level, u32 l) {
 	const u32 s = voxel_level - l;
 	const i64 N = (i64)1 << l, S = (i64)1 << s; // nodes per axis, node size
@@ -3440,10 +3721,12 @@ static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
 		pp.list_cap[l] = pp.count > 1 ? std::min(2 * S[l], lim) : 0;
 	}
 	pp.ev_cap = 0;
+	u64 pev_bytes = 0;
 	if (pp.stateful)
 		for (u32 l = leafL; l + 1 < L; ++l)
-			pp.ev_cap += 8 * pp.cap[l];
+			(lds_jobs(p.dc) ? pev_bytes : pp.ev_cap) += 8 * pp.cap[l];
 	pp.ev_cap = std::min<u64>(pp.ev_cap, lim);
+	pev_bytes *= sizeof(uint4);
 	// bytes: entries + per-level arrays, expand temporaries, lists, events, dense colors
 	u64 b = 0, maxpairs = 0;
 	for (u32 l = 0; l < L; ++l) {
@@ -3454,9 +3737,9 @@ static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
 			maxpairs = std::max(maxpairs, 8 * c);
 		}
 	}
-	b += maxpairs * 24 + pp.ev_cap * sizeof(PushEvent);
+	b += maxpairs * 24 + pp.ev_cap * sizeof(PushEvent) + pev_bytes;
 	if (pp.stateful)
-		b += pp.cap[leafL] * ((u64)sizeof(Color) << (3 * (vl - leafL))) + pp.cap[leafL] * (28 + 4 * (u64)kEncTmp) +
+		b += pp.cap[leafL] * ((u64)sizeof(Color) << (3 * (vl - leafL))) + pp.cap[leafL] * (60 + 4 * (u64)kEncTmp) +
 		     pp.cap[L - 1] * (64 * sizeof(Color) + 8);
 	pp.bytes = b;
 	if (b > kAsyncBudget || (pp.stateful && vl - leafL > 7))
@@ -3468,7 +3751,7 @@ struct AsyncBufs {
 	ALevel *d_lv = nullptr;
 	u32 *pm = nullptr, *roots = nullptr, *acc = nullptr, *bcnt = nullptr, *head = nullptr, *blist = nullptr;
 	u32 *flag = nullptr, *lcnt = nullptr, *info = nullptr, *app = nullptr, *appo = nullptr, *etmp = nullptr, *rest = nullptr;
-	uint4 *cstate = nullptr;
+	uint4 *cstate = nullptr, *jrange = nullptr;
 	Pair *bsum = nullptr;
 	PushEvent *ev = nullptr;
 	Color *dense = nullptr, *lcol = nullptr;
@@ -3526,6 +3809,8 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		HD_RET(buf(p, (pre + "content").c_str(), l, c * (l + 1 < L ? 9 : 2), &A.content));
 		if (l + 1 < L)
 			HD_RET(buf(p, (pre + "csize").c_str(), l, c, &A.csize));
+		if (stateful && lds_jobs(p.dc) && l >= leafL && l + 1 < L)
+			HD_RET(buf(p, (pre + "pev").c_str(), l, 8 * c, &A.pev));
 		if (stateful && l <= leafL) {
 			HD_RET(buf(p, (pre + "ores").c_str(), l, c, &A.ores));
 			HD_RET(buf(p, (pre + "ccnt").c_str(), l, c, &A.ccnt));
@@ -3551,6 +3836,7 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		HD_RET(p.scratch.get<u32>(pre + "appo", jobs, &B.appo));
 		HD_RET(p.scratch.get<u32>(pre + "etmp", jobs * kEncTmp, &B.etmp));
 		HD_RET(p.scratch.get<u32>(pre + "rest", jobs, &B.rest));
+		HD_RET(p.scratch.get<uint4>(pre + "jrange", 2 * jobs, &B.jrange));
 		const u64 leaves = std::max<u64>(cap[L - 1], 1);
 		HD_RET(p.scratch.get<Color>(pre + "lcol", leaves * 64, &B.lcol));
 		HD_RET(p.scratch.get<u64>(pre + "lmask", leaves, &B.lmask));
@@ -3659,7 +3945,8 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		a_emit<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.bsum);
 	}
 	const bool colors = pp.stateful && pp.cap[leafL] > 0;
-	if (colors) { // the jobs' dense colors on the color stream, beside the leaves
+	const bool ljobs = lds_jobs(p.dc);
+	if (colors && !ljobs) { // the jobs' dense colors on the color stream, beside the leaves
 		HD_TRY(hipEventRecord(p.ev_top, p.stream));
 		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_top, 0));
 		a_dense_init<<<grid_for(pp.cap[leafL], 1, 8192), kDenseThreads, 0, p.cstream>>>(B.d_lv, leafL, B.pm,
@@ -3677,9 +3964,18 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	if (colors) { // the leaf encode and SetNode on the color stream, beside the node levels
 		HD_TRY(hipEventRecord(p.ev_leaf, p.stream));
 		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_leaf, 0));
-		a_leafcolor<<<leaf_grid, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.dense, V);
 		u32 G = grid_for(pp.cap[leafL], 1, 8192);
-		a_encode_size<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app, B.etmp, B.rest);
+		if (ljobs) {
+			a_job_ranges<<<grid_for(pp.cap[leafL], 256, 4096), 256, 0, p.cstream>>>(B.d_lv, B.pm, leafL, L, p.cleaves,
+			                                                                       B.jrange);
+			a_job_encode<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.jrange, B.lcol, B.lmask, B.dense, V,
+			                                               B.info, B.app, B.etmp, B.rest);
+		}
+		else {
+			a_leafcolor<<<leaf_grid, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.dense, V);
+			a_encode_size<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app, B.etmp,
+			                                                B.rest);
+		}
 		a_scan_app<<<1, kScanThreads, 0, p.cstream>>>(leafL, B.pm, B.app, B.appo);
 		a_encode_write<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap,
 		                                                 B.etmp);
@@ -3908,7 +4204,7 @@ static int frame_create(Pool &p, const hd_frame_config &cfg, u32 root, u32 croot
 			pp.list_cap[l] = cfg.max_edits > 1 ? std::min(2 * S, lim) : 0;
 		}
 		pp.ev_cap = 0;
-		if (colors)
+		if (colors && !lds_jobs(p.dc))
 			for (u32 l = p.dc.color_leaf; l + 1 < L; ++l)
 				pp.ev_cap += 8 * pp.cap[l];
 		pp.ev_cap = std::min<u64>(pp.ev_cap, lim);
