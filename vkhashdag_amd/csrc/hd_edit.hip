@@ -1438,6 +1438,19 @@ __device__ inline u32 child_result(const ALevel &C, const GroupTable &G, u32 c) 
 	const u32 v = !pend ? st : (r != kNull ? r : (gf ? pt : gr));
 	return c < C.cap ? v : kNull; // c >= cap is unreachable with sound bounds (the pass reports it)
 }
+// child_result and whether it is a node its group appended in this pass. Appends go past the
+// bucket's used words, where no earlier node could point, so a node holding such a child is new:
+// the find among the bucket's existing nodes would fail and is skipped.
+__device__ inline uint2 child_result_fresh(const ALevel &C, const GroupTable &G, u32 c) {
+	const u32 cc = min(c, C.cap - 1u);
+	const u32 pend = ldg(C.pend, cc), st = ldg(C.status, cc), r = ldg(C.res, cc), s = ldg(C.slot, cc);
+	const u32 pt = ldg(&C.ent[cc].ptr, 0);
+	const u32 si = s & G.mask;
+	const u32 gf = ldg(G.gfail, si), gr = ldg(G.gres, si);
+	const bool fresh = pend && r == kNull && !gf;
+	const u32 v = !pend ? st : (r != kNull ? r : (gf ? pt : gr));
+	return c < C.cap ? make_uint2(v, fresh ? 1u : 0u) : make_uint2(kNull, 0u);
+}
 
 // edit_leaf (k_leaf) + the find among the bucket's existing leaves: one wavefront per leaf. A
 // leaf inside a color job records the colors its edits assign (lcol, by lane) and which voxels
@@ -1566,12 +1579,16 @@ __device__ inline JoinIn load_join(const ALevel &A, u32 e) {
 // The results of an entry's proceeding children, by slot: they are entries fc .. fc + cnt - 1 of
 // the next level in slot order; all lookups are issued before any result is used.
 template <typename ChildFn>
-__device__ inline void children_by_slot(const JoinIn &q, ChildFn child, u32 (&cs)[8]) {
+__device__ inline void children_by_slot(const JoinIn &q, ChildFn child, u32 (&cs)[8], bool &fresh) {
 	const u32 pmk = q.pmw & 0xFFu, cnt = (u32)__builtin_popcount(pmk);
 	u32 cr[8];
+	fresh = false;
 #pragma unroll
-	for (u32 j = 0; j < 8; ++j)
-		cr[j] = cnt ? child(q.fc + min(j, cnt - 1u)) : kNull;
+	for (u32 j = 0; j < 8; ++j) {
+		const uint2 r = cnt ? child(q.fc + min(j, cnt - 1u)) : make_uint2(kNull, 0u);
+		cr[j] = r.x;
+		fresh |= r.y != 0u;
+	}
 #pragma unroll
 	for (u32 i = 0; i < 8; ++i) {
 		const u32 rank = (u32)__builtin_popcount(pmk & ((1u << i) - 1u));
@@ -1620,8 +1637,10 @@ __device__ inline Built join_entry(const EditParams &P, u32 l, const ALevel &A, 
                                    u32 e) {
 	const JoinIn q = load_join(A, e);
 	u32 cs[8];
-	children_by_slot(q, [&](u32 c) { return child_result(C, G, c); }, cs);
+	bool fresh;
+	children_by_slot(q, [&](u32 c) { return child_result_fresh(C, G, c); }, cs, fresh);
 	Built o = build_node(P, l, q, cs);
+	o.need = fresh; // (build_node leaves need false)
 	if (!o.k) {
 		stg(A.pend, e, 0u);
 		stg(A.status, e, o.fin);
@@ -1633,8 +1652,9 @@ __device__ inline Built join_entry(const EditParams &P, u32 l, const ALevel &A, 
 __device__ inline bool find_entry(const EditParams &P, u32 l, const ALevel &A, u32 e, const Built &o, u32 *stage,
                                   u32 *top, u32 cap) {
 	u32 bw;
-	const u32 f = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, headers_by_value(P.c, l), bw,
-	                                    stage, top, cap);
+	const u32 f = o.need ? kNull // a fresh child: a new node
+	                     : find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, headers_by_value(P.c, l),
+	                                             bw, stage, top, cap);
 	if (f == kFindRetry)
 		return false;
 #pragma unroll
@@ -2644,6 +2664,7 @@ struct BottomLds {
 		__align__(16) u32 stage[kStage]; // build: the scanned buckets' words
 	};
 	unsigned char gfail[kSTab];
+	unsigned char cfresh[kSmallLevel][8]; // by parent and slot: the child's result is its fresh append
 	u32 gcount, stage_top;
 };
 // the table clear writes the stage as uint4: tab, bkey, bmin, bhead (kNull) then bn (0)
@@ -2857,7 +2878,7 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 		const u32 cur = l & 1;
 		u32 fin = kNull, optr = kNull, b = 0, bw = 0, h = 0, k = 0, pidx = 0;
 		u32 c[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-		bool need = false;
+		bool need = false, fresh_out = false;
 		if (act) {
 			if (leaf) {
 				pidx = (ldg(&A.ent[lane].parent, 0) << 3) | (ldg(&A.ent[lane].sf, 0) & 7u);
@@ -2877,19 +2898,27 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 				const JoinIn q = have_pre ? pre : load_join(A, lane);
 				pidx = q.pidx;
 				u32 cs[8];
+				bool fresh = false;
 				if (l < l_from) {
 #pragma unroll
-					for (u32 i = 0; i < 8; ++i)
+					for (u32 i = 0; i < 8; ++i) {
 						cs[i] = S.content[lane][i];
+						fresh |= ((q.pmw >> i) & 1u) && S.cfresh[lane][i];
+					}
 				} else
-					children_by_slot(q, [&](u32 ch) { return child_result(s_lv[l + 1], G, ch); }, cs);
+					children_by_slot(q, [&](u32 ch) { return child_result_fresh(s_lv[l + 1], G, ch); }, cs, fresh);
 				Built o = build_node(P, (u32)l, q, cs);
 				fin = o.fin;
 				if (o.k) {
-					fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k, headers_by_value(P.c, (u32)l),
-					                            bw, S.stage, &S.stage_top, kStage);
-					if (fin == kFindRetry) // the stage is full: scan in global memory
-						fin = find_in_global(P.words, P.c, (u64)o.b << P.c.wbpb(), bw, o.c, o.k);
+					if (fresh) { // a fresh child: a new node, no find
+						fin = kNull;
+						bw = P.bucket_words[o.b];
+					} else {
+						fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k,
+						                            headers_by_value(P.c, (u32)l), bw, S.stage, &S.stage_top, kStage);
+						if (fin == kFindRetry) // the stage is full: scan in global memory
+							fin = find_in_global(P.words, P.c, (u64)o.b << P.c.wbpb(), bw, o.c, o.k);
+					}
 					if (fin == kNull) {
 						need = true;
 #pragma unroll
@@ -2930,8 +2959,10 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 				full = true;
 			else
 				++appended;
-			if (same)
+			if (same) {
 				fin = f == kNull ? optr : f;
+				fresh_out = f != kNull;
+			}
 			if (need && b == lb)
 				bw = nbw; // later groups of this bucket append after it
 			pending &= ~members;
@@ -2944,8 +2975,10 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 		}
 		if (act) {
 			S.res[cur][lane] = fin;
-			if (l > 0 && (pidx >> 3) < kSmallLevel)
+			if (l > 0 && (pidx >> 3) < kSmallLevel) {
 				S.content[pidx >> 3][pidx & 7u] = fin;
+				S.cfresh[pidx >> 3][pidx & 7u] = fresh_out;
+			}
 		}
 		S.stage_top = 0; // every lane writes the same value; the next level allocates from 0
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -3034,22 +3067,27 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 			if (todo) {
 				const JoinIn q = (have_pre && e0 == 0) ? pre : load_join(A, e);
 				u32 cs[8];
+				bool fresh = false;
 				if (l < l_from) { // the children's results sit in this entry's content row, by slot
 #pragma unroll
-					for (u32 i = 0; i < 8; ++i)
+					for (u32 i = 0; i < 8; ++i) {
 						cs[i] = S.content[e][i];
+						fresh |= ((q.pmw >> i) & 1u) && S.cfresh[e][i];
+					}
 				} else
-					children_by_slot(q, [&](u32 c) { return child_result(s_lv[l + 1], G, c); }, cs);
+					children_by_slot(q, [&](u32 c) { return child_result_fresh(s_lv[l + 1], G, c); }, cs, fresh);
 				o = build_node(P, (u32)l, q, cs);
+				o.need = fresh;
 				optr = q.ptr;
 				pidx = q.pidx;
 				if (o.k)
 					bwv = P.bucket_words[o.b];
 			}
-			// a bucket larger than the whole stage is scanned in global memory
+			// a bucket larger than the whole stage is scanned in global memory; an entry with a
+			// fresh child skips the find
 			const bool huge = ((bwv + 3u) & ~3u) > kStage;
 			for (;;) {
-				const u32 len = todo && o.k && !huge ? (bwv + 3u) & ~3u : 0u;
+				const u32 len = todo && o.k && !huge && !o.need ? (bwv + 3u) & ~3u : 0u;
 				Pair tot;
 				const Pair off = block_excl_scan2(Pair{len, 0u}, s_scan, tot);
 				const bool fits = off.a + len <= kStage;
@@ -3094,7 +3132,8 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 					bool need = false;
 					if (o.k) {
 						const u64 base = (u64)o.b << wbpb;
-						fin = huge     ? find_in_global(P.words, P.c, base, bwv, o.c, o.k)
+						fin = o.need   ? kNull
+						      : huge   ? find_in_global(P.words, P.c, base, bwv, o.c, o.k)
 						      : len == 0 ? kNull
 						      : hv       ? find_in_words_hv(S.stage + off.a, base, bwv, o.c, o.k)
 						                 : find_in_words(S.stage + off.a, P.c, base, bwv, o.c, o.k);
@@ -3184,8 +3223,10 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 			S.res[cur][e] = r;
 			if (l > 0) {
 				const u32 ps = S.pslot[e];
-				if ((ps >> 3) < kSmallLevel)
+				if ((ps >> 3) < kSmallLevel) {
 					S.content[ps >> 3][ps & 7u] = r;
+					S.cfresh[ps >> 3][ps & 7u] = sl != kNull && !S.gfail[sl]; // a group's result is its append
+				}
 			}
 		}
 		__syncthreads();
