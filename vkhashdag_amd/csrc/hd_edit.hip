@@ -68,7 +68,7 @@ struct EditParams {
 
 __device__ inline u32 color_child(const u32 *cnodes, u32 p, u32 i) { // DAGColorPool::GetChild (:139-143)
 	u32 t = ctag(p);
-	return t == kTagNode ? cnodes[(u64)cdata(p) * 8u + i] : (t == kTagColor ? p : kColorNull);
+	return t == kTagNode ? ldg(cnodes, (u64)cdata(p) * 8u + i) : (t == kTagColor ? p : kColorNull);
 }
 
 struct Classified {
@@ -859,7 +859,7 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 		u32 w[9];
 #pragma unroll
 		for (u32 k = 0; k < 9; ++k)
-			w[k] = P.words[(u64)E.ptr + k];
+			w[k] = ldg(P.words, (u64)E.ptr + k);
 		const u32 m = w[0];
 		if ((m >> i) & 1u) {
 			const u32 r = (u32)__builtin_popcount(m & ((1u << i) - 1u));
@@ -923,9 +923,9 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 			atomicOr(&pm[kPmOverflow], 1u);
 	}
 	const u32 kid = c.t == kClear ? kNull : (c.t == kFill ? P.filled[cl] : old);
-	A.kids[t] = kid; // the old child for proceeding and untouched slots
+	stg(A.kids, t, kid); // the old child for proceeding and untouched slots
 	if (A.okids)
-		A.okids[t] = c.octree;
+		stg(A.okids, t, c.octree);
 	bool proceed = c.t == kProceed;
 	// bit 0: proceeds, bit 1: a terminal child whose pointer changed, bit 2: pushed a color
 	u32 flag = (proceed ? 1u : 0u) | (!proceed && kid != old ? 2u : 0u) | (c.push ? 4u : 0u);
@@ -944,8 +944,8 @@ __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A,
                                     u32 idx, u32 lo) {
 	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
 	if (i == 0) {
-		A.first_child[e] = idx;
-		A.pmask[e] = m8;
+		stg(A.first_child, e, idx);
+		stg(A.pmask, e, m8);
 	}
 	Entry out{};
 	if (!(o.flag & 1u))
@@ -979,7 +979,7 @@ __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A,
 		lo = o.cs.z;
 	out = Entry{cx, cy, cz, kid, e, sf & ~kSfNewJob, o.cs.x, job, lo, ln};
 	if (idx < N.cap)
-		N.ent[idx] = out;
+		stg(N.ent, idx, out);
 	else
 		atomicOr(&pm[kPmOverflow], 1u);
 	return out;
@@ -1395,15 +1395,15 @@ __device__ inline u32 wave_find_leaf(const u32 *__restrict__ W, const u32 *__res
 	const u32 wpb = c.words_per_bucket();
 	u32 x0 = 0, x1 = 0;
 	if (2u * lane + 2u <= wpb) {
-		x0 = W[base + 2u * lane];
-		x1 = W[base + 2u * lane + 1u];
+		x0 = ldg(W, base + 2u * lane);
+		x1 = ldg(W, base + 2u * lane + 1u);
 	}
-	const u32 bw = bucket_words[bucket];
+	const u32 bw = ldg(bucket_words, bucket);
 	for (u32 i0 = 0; i0 < bw; i0 += 128u) {
 		u32 j = i0 + 2u * lane;
 		if (i0) {
-			x0 = j + 2u <= bw ? W[base + j] : 0u;
-			x1 = j + 2u <= bw ? W[base + j + 1u] : 0u;
+			x0 = j + 2u <= bw ? ldg(W, base + j) : 0u;
+			x1 = j + 2u <= bw ? ldg(W, base + j + 1u) : 0u;
 		}
 		bool m = j + 2u <= bw && x0 == n0 && x1 == n1;
 		unsigned long long b = __ballot(m);
@@ -1468,15 +1468,15 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 	// the wave's leaf index is uniform: keep it (and what is read through it) in scalar registers
 	const u32 e0 = __builtin_amdgcn_readfirstlane((u32)(((u64)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
 	for (u32 e = e0; e < n; e += waves) {
-		Entry E = A.ent[e];
+		Entry E = ldg(A.ent, e);
 		E.x = __builtin_amdgcn_readfirstlane(E.x), E.y = __builtin_amdgcn_readfirstlane(E.y);
 		E.z = __builtin_amdgcn_readfirstlane(E.z), E.ptr = __builtin_amdgcn_readfirstlane(E.ptr);
 		E.sf = __builtin_amdgcn_readfirstlane(E.sf), E.job = __builtin_amdgcn_readfirstlane(E.job);
 		E.loff = __builtin_amdgcn_readfirstlane(E.loff), E.llen = __builtin_amdgcn_readfirstlane(E.llen);
 		u32 w0 = 0, w1 = 0;
 		if (E.ptr != kNull) {
-			w0 = P.words[E.ptr];
-			w1 = P.words[E.ptr + 1u];
+			w0 = ldg(P.words, E.ptr);
+			w1 = ldg(P.words, E.ptr + 1ull);
 		}
 		bool v = ((lane < 32u ? (w0 >> lane) : (w1 >> (lane - 32u))) & 1u) != 0;
 		u32 vx = (E.x << 2) | ((lane >> 2) & 2u) | (lane & 1u);
@@ -1487,7 +1487,7 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		bool asg = false;
 		bool nv = v;
 		for (u32 k = 0; k < E.llen; ++k) {
-			DevEditor ed = P.editor(E.llen == 1 ? E.loff : A.lists[E.loff + k]);
+			DevEditor ed = P.editor(E.llen == 1 ? E.loff : ldg(A.lists, (u64)E.loff + k));
 			if (!ed.stateful())
 				nv = ed.edit_voxel(vx, vy, vz, nv);
 			else
@@ -1505,8 +1505,8 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		bool pending = (n0 != w0 || n1 != w1) && m != 0ull; // wavefront-uniform
 		if (!pending) {
 			if (lane == 0) {
-				A.pend[e] = 0;
-				A.status[e] = (n0 != w0 || n1 != w1) ? kNull : E.ptr;
+				stg(A.pend, e, 0u);
+				stg(A.status, e, (n0 != w0 || n1 != w1) ? kNull : E.ptr);
 			}
 			continue;
 		}
@@ -1514,11 +1514,11 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		u32 b = P.c.level_base[level] + (h & ((1u << P.c.bucket_bits[level]) - 1u));
 		u32 f = wave_find_leaf(P.words, P.bucket_words, P.c, b, n0, n1, lane);
 		if (lane == 0) {
-			A.pend[e] = 1;
-			A.content[2 * (u64)e] = n0;
-			A.content[2 * (u64)e + 1] = n1;
-			A.bucket[e] = b;
-			A.res[e] = f;
+			stg(A.pend, e, 1u);
+			stg(A.content, 2 * (u64)e, n0);
+			stg(A.content, 2 * (u64)e + 1, n1);
+			stg(A.bucket, e, b);
+			stg(A.res, e, f);
 		}
 	}
 }
