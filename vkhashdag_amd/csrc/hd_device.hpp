@@ -400,13 +400,13 @@ __device__ inline u32 find_in_bucket_from(const u32 *__restrict__ words, const D
 	const u64 base = (u64)bucket << c.wbpb();
 	if (leaf) {
 		for (u32 i = from; i + 2 <= bw; i += 2)
-			if (words[base + i] == node[0] && words[base + i + 1] == node[1])
+			if (ldg(words, base + i) == node[0] && ldg(words, base + i + 1) == node[1])
 				return (u32)(base + i);
 		return kNull;
 	}
 	u32 i = from;
 	while (i < bw) {
-		u32 w0 = words[base + i];
+		u32 w0 = ldg(words, base + i);
 		u32 sz = inner_words(w0);
 		if (sz == 0) {
 			i = ((i >> c.wbpp) + 1u) << c.wbpp;
@@ -415,7 +415,7 @@ __device__ inline u32 find_in_bucket_from(const u32 *__restrict__ words, const D
 		if (sz == n && w0 == node[0]) {
 			bool eq = true;
 			for (u32 k = 1; k < n; ++k)
-				eq = eq && words[base + i + k] == node[k];
+				eq = eq && ldg(words, base + i + k) == node[k];
 			if (eq)
 				return (u32)(base + i);
 		}
@@ -438,7 +438,7 @@ __device__ inline u32 find_in_bucket_win(const u32 *__restrict__ words, const u3
 		const uint4 *q = reinterpret_cast<const uint4 *>(words + base);
 #pragma unroll
 		for (u32 k = 0; k < 4; ++k) {
-			uint4 t = q[k];
+			uint4 t = ldg(q, k);
 			W[4 * k] = t.x;
 			W[4 * k + 1] = t.y;
 			W[4 * k + 2] = t.z;
@@ -447,13 +447,13 @@ __device__ inline u32 find_in_bucket_win(const u32 *__restrict__ words, const u3
 	} else {
 #pragma unroll
 		for (u32 k = 0; k < 16; ++k)
-			W[k] = words[base + k]; // a bucket this small sits inside the padded pool
+			W[k] = ldg(words, base + k); // a bucket this small sits inside the padded pool
 	}
-	const u32 bw = bucket_words[bucket];
+	const u32 bw = ldg(bucket_words, bucket);
 	bw_out = bw;
 	auto get = [&](u32 i) -> u32 {
 		if (i >= 16u)
-			return words[base + i];
+			return ldg(words, base + i);
 		u32 v = 0;
 #pragma unroll
 		for (u32 k = 0; k < 16; ++k)
@@ -568,8 +568,8 @@ __device__ inline u32 find_in_bucket_staged(const u32 *__restrict__ words, const
 	uint4 v[16];
 #pragma unroll
 	for (u32 k = 0; k < 16; ++k)
-		v[k] = q[k];
-	const u32 bw = bucket_words[bucket];
+		v[k] = ldg(q, k);
+	const u32 bw = ldg(bucket_words, bucket);
 	bw_out = bw;
 	if (bw == 0u)
 		return kNull;
@@ -624,7 +624,7 @@ __device__ inline u32 find_in_bucket_staged(const u32 *__restrict__ words, const
 		uint4 t[16];
 #pragma unroll
 		for (u32 k = 0; k < 16; ++k)
-			t[k] = q[(i >> 2) + k];
+			t[k] = ldg(q, (i >> 2) + k);
 #pragma unroll
 		for (u32 k = 0; k < 16; ++k)
 			if (i + 4u * k < len)
@@ -648,14 +648,14 @@ __device__ inline u32 append_to_bucket(u32 *__restrict__ words, const DevConfig 
 	if (off + n > c.words_per_page()) {
 		u64 pg = ((u64)bucket << c.wbpb()) + ((u64)slot << c.wbpp);
 		for (u32 k = off; k < c.words_per_page(); ++k)
-			words[pg + k] = 0u;
+			stg(words, pg + k, 0u);
 		++slot;
 		off = 0;
 	}
 	u32 rel = (slot << c.wbpp) | off;
 	u64 at = ((u64)bucket << c.wbpb()) + rel;
 	for (u32 k = 0; k < n; ++k)
-		words[at + k] = node[k];
+		stg(words, at + k, node[k]);
 	bw = rel + n;
 	return (u32)at;
 }
@@ -668,7 +668,7 @@ __device__ inline u32 append_regs(u32 *__restrict__ words, const DevConfig &c, u
 	if (off + n > c.words_per_page()) {
 		u64 pg = ((u64)bucket << c.wbpb()) + ((u64)slot << c.wbpp);
 		for (u32 k = off; k < c.words_per_page(); ++k)
-			words[pg + k] = 0u;
+			stg(words, pg + k, 0u);
 		++slot;
 		off = 0;
 	}
@@ -677,7 +677,7 @@ __device__ inline u32 append_regs(u32 *__restrict__ words, const DevConfig &c, u
 #pragma unroll
 	for (u32 k = 0; k < 9; ++k)
 		if (k < n)
-			words[at + k] = node[k];
+			stg(words, at + k, node[k]);
 	bw = rel + n;
 	return (u32)at;
 }

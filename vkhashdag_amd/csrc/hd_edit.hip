@@ -1794,7 +1794,7 @@ __device__ inline u32 walk_short(const EditParams &P, bool leaf, const ALevel &A
 				if (x0 + d < k && pos[x0 + d] != kNull && j < nz[x0 + d])
 					stg(P.words, (u64)pos[x0 + d] + j, w[d][j]);
 	}
-	P.bucket_words[b] = bw;
+	stg(P.bucket_words, b, bw);
 	if (any_fail)
 		atomicOr(&P.meta[kMetaOverflow], 1u);
 	return appended;
@@ -1808,7 +1808,7 @@ __device__ inline u32 walk_entry(const EditParams &P, u32 l, const ALevel &A, co
 	const bool leaf = l == P.c.L - 1;
 	const u32 stride = leaf ? 2u : 9u;
 	const u32 b = A.bucket[e], k = bcnt[b];
-	const u32 bw0 = P.bucket_words[b];
+	const u32 bw0 = ldg(P.bucket_words, b);
 	u32 appended = 0;
 	if (k <= 4u)
 		appended = walk_short<4>(P, leaf, A, G, b, k, bw0, blist);
@@ -1837,7 +1837,7 @@ __device__ inline u32 walk_entry(const EditParams &P, u32 l, const ALevel &A, co
 			G.gfail[bs] = f == kNull ? 1u : 0u;
 			last = best;
 		}
-		P.bucket_words[b] = bw;
+		stg(P.bucket_words, b, bw);
 		if (any_fail)
 			atomicOr(&P.meta[kMetaOverflow], 1u);
 	}
@@ -2748,7 +2748,7 @@ __device__ inline void small_walk_bucket(const EditParams &P, bool leaf, BottomL
 			last = best;
 		}
 	}
-	P.bucket_words[b] = bw;
+	stg(P.bucket_words, b, bw);
 	if (appended)
 		atomicAdd(&acc[kAccUpserts], appended);
 	if (any_fail)
@@ -2891,7 +2891,7 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 					b = ldg(A.bucket, lane);
 					k = 2;
 					h = murmur_leaf(c[0], c[1]);
-					bw = P.bucket_words[b];
+					bw = ldg(P.bucket_words, b);
 					optr = ldg(&A.ent[lane].ptr, 0);
 				}
 			} else {
@@ -2912,7 +2912,7 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 				if (o.k) {
 					if (fresh) { // a fresh child: a new node, no find
 						fin = kNull;
-						bw = P.bucket_words[o.b];
+						bw = ldg(P.bucket_words, o.b);
 					} else {
 						fin = find_in_bucket_staged(P.words, P.bucket_words, P.c, o.b, o.c, o.k,
 						                            headers_by_value(P.c, (u32)l), bw, S.stage, &S.stage_top, kStage);
@@ -2951,7 +2951,7 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 			u32 f = kNull, nbw = bw;
 			if (lane == ld) {
 				f = append_regs(P.words, P.c, lb, nbw, c, k);
-				P.bucket_words[lb] = nbw;
+				stg(P.bucket_words, lb, nbw);
 			}
 			f = __builtin_amdgcn_readlane(f, ld);
 			nbw = __builtin_amdgcn_readlane(nbw, ld);
@@ -3047,7 +3047,7 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 						S.csz[e] = 2;
 						S.hash[e] = murmur_leaf(c0, c1);
 						S.bucket[e] = b;
-						S.bw[e] = P.bucket_words[b];
+						S.bw[e] = ldg(P.bucket_words, b);
 						optr = ldg(&A.ent[e].ptr, 0);
 					}
 					S.res[cur][e] = fin;
@@ -3081,7 +3081,7 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 				optr = q.ptr;
 				pidx = q.pidx;
 				if (o.k)
-					bwv = P.bucket_words[o.b];
+					bwv = ldg(P.bucket_words, o.b);
 			}
 			// a bucket larger than the whole stage is scanned in global memory; an entry with a
 			// fresh child skips the find
