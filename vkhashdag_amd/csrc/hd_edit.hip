@@ -2649,8 +2649,14 @@ constexpr u32 kStage = 5 * kSTab; // bucket staging words of the build (shares t
 struct BottomLds {
 	// per entry; [2] = this level / its child level, whose results the build resolves
 	u32 res[2][kSmallLevel];  // final pointer, or kNull while a group result is pending
-	u32 slot[2][kSmallLevel]; // dedup slot of an entry with new content, else kNull
-	u32 ptr[2][kSmallLevel];  // old pointer (kept when the bucket is full)
+	union {
+		struct {
+			u32 slot[2][kSmallLevel]; // dedup slot of an entry with new content, else kNull
+			u32 ptr[2][kSmallLevel];  // old pointer (kept when the bucket is full)
+		};
+		// the wave-0 levels: every entry's join inputs (3 uint4 each), loaded by the whole block
+		__align__(16) u32 jin[4 * kSmallLevel];
+	};
 	u32 content[kSmallLevel][9];
 	u32 csz[kSmallLevel], hash[kSmallLevel], bucket[kSmallLevel], bw[kSmallLevel], bmap[kSmallLevel];
 	u32 glist[kSmallLevel], gnext[kSmallLevel]; // groups (their slots) and the per-bucket group lists
@@ -2869,8 +2875,10 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 // first lane appending it and every member taking the result (its old pointer when the bucket is
 // full); each result goes to the parent's content row at its slot.
 __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, const ALevel *s_lv, const u32 *s_n,
-                                   const GroupTable &G, BottomLds &S, u32 *__restrict__ acc, JoinIn pre, bool have_pre) {
+                                   const GroupTable &G, BottomLds &S, u32 *__restrict__ acc, JoinIn pre, bool have_pre,
+                                   bool preloaded) {
 	const u32 lane = threadIdx.x, L = P.c.L;
+	u32 jbase = 0; // preloaded: the level's first entry in S.jin
 	for (int l = l_top; l >= 0; --l) {
 		const u32 n = s_n[l];
 		const ALevel A = s_lv[l];
@@ -2895,7 +2903,15 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 					optr = ldg(&A.ent[lane].ptr, 0);
 				}
 			} else {
-				const JoinIn q = have_pre ? pre : load_join(A, lane);
+				JoinIn q;
+				if (preloaded) {
+					const uint4 *J = reinterpret_cast<const uint4 *>(S.jin) + 3u * (jbase + lane);
+					const uint4 a = J[0];
+					q.ptr = a.x, q.pmw = a.y, q.fc = a.z, q.pidx = a.w;
+					q.k0 = J[1];
+					q.k1 = J[2];
+				} else
+					q = have_pre ? pre : load_join(A, lane);
 				pidx = q.pidx;
 				u32 cs[8];
 				bool fresh = false;
@@ -2933,9 +2949,11 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 			}
 		}
 		// the next level's entries: their loads overlap this level's groups
-		have_pre = l > 0 && lane < s_n[l - 1];
+		have_pre = !preloaded && l > 0 && lane < s_n[l - 1];
 		if (have_pre)
 			pre = load_join(s_lv[l - 1], lane);
+		if (!leaf)
+			jbase += n;
 		u64 pending = __ballot(need);
 		u32 appended = 0;
 		bool full = false;
@@ -3016,8 +3034,27 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 		for (int i = 0; i <= l; ++i)
 			tiny = tiny && s_n[i] <= 64u;
 		if (tiny) {
+			// every wave-0 level's join inputs come from the top-down only: the whole block loads
+			// them at once, so wave 0 waits for memory once per level (its bucket counters)
+			u32 items = 0;
+			for (int i = 0; i <= l; ++i)
+				items += (u32)i + 1u < L ? s_n[i] : 0u;
+			const bool preloaded = 3u * items <= kSmallLevel; // 4 kSmallLevel words, 12 per entry
+			if (preloaded)
+				for (u32 it = threadIdx.x; it < items; it += blockDim.x) {
+					int li = (u32)l + 1u < L ? l : l - 1; // levels l, l - 1, .. 0 (no leaf level)
+					u32 base = 0;
+					while (it >= base + s_n[li])
+						base += s_n[li--];
+					const JoinIn q = load_join(s_lv[li], it - base);
+					uint4 *J = reinterpret_cast<uint4 *>(S.jin) + 3u * it;
+					J[0] = make_uint4(q.ptr, q.pmw, q.fc, q.pidx);
+					J[1] = q.k0;
+					J[2] = q.k1;
+				}
+			__syncthreads();
 			if (threadIdx.x < 64)
-				tiny_levels(P, l, l_from, s_lv, s_n, G, S, acc, pre, have_pre);
+				tiny_levels(P, l, l_from, s_lv, s_n, G, S, acc, pre, have_pre, preloaded);
 			__syncthreads();
 			break;
 		}
