@@ -52,6 +52,17 @@ def test_batched_random_edits_equal_sequential(seed, res, n):
     assert_equivalent(dev, ora, cfg)
 
 
+def test_batched_root_color_leaf_equals_sequential():
+    # a color leaf of 32^3 voxels (more than a workgroup rebuilds in LDS): the dense per-voxel path
+    capi = _capi()
+    cfg = small_cfg(32, 6, 5, 2, 5)
+    e = random_edits(30, 32, 13, True)
+    dev, ora = capi.DevicePool(cfg), O.OraclePool(cfg)
+    dev.edit(e, capi.MODE_BATCHED)
+    ora.edit(e)
+    assert_equivalent(dev, ora, cfg)
+
+
 def test_batched_terrain_mixed_batch_equals_sequential():
     capi = _capi()
     res = 128
