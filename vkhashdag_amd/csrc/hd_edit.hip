@@ -4039,7 +4039,12 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	const u64 small_level = std::min<u32>(g_tuning.small_level, kSmallLevel);
 	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= small_level)
 		++l_from;
-	if (colors) { // the leaf encode and SetNode on the color stream, beside the node levels
+	// the color leaves (job rebuild and encode) and SetNode on the color stream, beside the node
+	// levels; they start once the node levels have passed level color_after (L: right after the
+	// leaves)
+	bool colors_queued = !colors;
+	auto queue_colors = [&]() -> int {
+		colors_queued = true;
 		HD_TRY(hipEventRecord(p.ev_leaf, p.stream));
 		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_leaf, 0));
 		u32 G = grid_for(pp.cap[leafL], 1, 8192);
@@ -4061,14 +4066,22 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		                                                                 p.cleaf_cap, B.rest);
 		a_setnode<<<1, kSetNodeThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
 		HD_TRY(hipEventRecord(p.ev_color, p.cstream));
-	}
+		return HD_OK;
+	};
+	const int color_after = (int)L - 1; // after the leaf level's walk (measured best)
+	if (color_after >= (int)L)
+		HD_RET(queue_colors());
 	for (int l = (int)L - 1; l > l_from; --l) {
 		u32 G = grid_for(pp.cap[l], kAT, 4096);
 		if (l + 1 < (int)L)
 			a_build<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G);
 		a_dedup<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.blist);
 		a_walk<<<G, kAT, 0, p.stream>>>(P, (u32)l, B.d_lv, B.pm, B.G, B.bcnt, B.head, B.blist, B.acc);
+		if (!colors_queued && l <= color_after)
+			HD_RET(queue_colors());
 	}
+	if (!colors_queued)
+		HD_RET(queue_colors());
 	a_bottom<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, l_from, B.G, B.roots, B.acc, p.cnode_cap);
 	if (colors) // the next pass (and any host read) follows both streams
 		HD_TRY(hipStreamWaitEvent(p.stream, p.ev_color, 0));
