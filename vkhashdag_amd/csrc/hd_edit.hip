@@ -851,8 +851,11 @@ struct PairOut {
 
 // classification of one (node, child) pair: the body of k_expand. The old node is read whole
 // (9 words, the pool is padded) so its mask and child arrive in one memory round trip.
+// bits: where the classification records which list edits touch the child (bit k for list entry k;
+// at least (llen + 31) / 32 words, private to the thread), for emit_pair_e; or null.
 __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel &A, u64 t, const Entry &E,
-                                        PushEvent *__restrict__ ev, u32 *__restrict__ pm, u32 ev_cap) {
+                                        PushEvent *__restrict__ ev, u32 *__restrict__ pm, u32 ev_cap,
+                                        u32 *bits = nullptr) {
 	const u32 i = (u32)(t & 7u);
 	u32 old = kNull;
 	if (E.ptr != kNull) {
@@ -882,6 +885,9 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 	else {
 		cnt = 0;
 		bool active = false;
+		if (bits)
+			for (u32 w = 0; w < (E.llen + 31u) >> 5; ++w)
+				bits[w] = 0u;
 		for (u32 k0 = 0; k0 < E.llen; k0 += kListBatch) { // list words of a batch in one round trip
 			u32 ix[kListBatch];
 #pragma unroll
@@ -895,6 +901,8 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 						++cnt;
 						single = ix[j];
 						active |= ed.stateful();
+						if (bits)
+							bits[(k0 + j) >> 5] |= 1u << ((k0 + j) & 31u);
 					}
 				}
 		}
@@ -939,9 +947,10 @@ __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &
 // Writes the child entry of a proceeding pair (k_emit). m8: proceed flags of the pair's node in
 // bits 0-7, its changed terminal children in bits 8-15 and its pushed children in bits 16-23 (valid
 // on slot 0), idx / lo: the child's entry index and list offset at level l + 1.
+// bits: expand_pair_e's record of the list edits that touch the child, or null to test them again.
 __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A, const ALevel &N,
                                     u32 *__restrict__ pm, u64 t, const Entry &E, u32 kid, u32 m8, const PairOut &o,
-                                    u32 idx, u32 lo) {
+                                    u32 idx, u32 lo, const u32 *bits = nullptr) {
 	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
 	if (i == 0) {
 		stg(A.first_child, e, idx);
@@ -965,10 +974,14 @@ __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A,
 #pragma unroll
 				for (u32 j = 0; j < kListBatch; ++j)
 					ix[j] = k0 + j < E.llen ? ldg(A.lists, (u64)E.loff + k0 + j) : 0u;
+				static_assert(32 % kListBatch == 0, "a batch's bits sit in one word");
+				if (bits)
+					keep = (bits[k0 >> 5] >> (k0 & 31u)) & ((1u << kListBatch) - 1u);
+				else
 #pragma unroll
-				for (u32 j = 0; j < kListBatch; ++j)
-					if (k0 + j < E.llen && P.editor(ix[j]).geom(l + 1, cx, cy, cz) != kNotAffected)
-						keep |= 1u << j;
+					for (u32 j = 0; j < kListBatch; ++j)
+						if (k0 + j < E.llen && P.editor(ix[j]).geom(l + 1, cx, cy, cz) != kNotAffected)
+							keep |= 1u << j;
 #pragma unroll
 				for (u32 j = 0; j < kListBatch; ++j)
 					if ((keep >> j) & 1u)
@@ -1088,6 +1101,8 @@ __device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__
 // (every one small enough for one block), with the current level's entries and count in LDS.
 constexpr u32 kTopEnt = kTopPairs / 8;
 constexpr u32 kTopEdits = 1024; // a pass with at most this many edits reads them from LDS in a_top
+// kLists: the pass has more than one edit (nodes may carry edit lists).
+template <bool kLists>
 __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__restrict__ lv, u32 *__restrict__ roots,
                                              u32 *__restrict__ pm, u32 pm_words, u32 l_end, PushEvent *ev, u32 ev_cap,
                                              u32 *__restrict__ gen_ctr) {
@@ -1097,6 +1112,7 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 	__shared__ u32 scount;
 	__shared__ hd_edit_desc sedit[kTopEdits];
 	__shared__ u32 sfilled[HD_MAX_LEVELS];
+	__shared__ u32 sbits[kLists ? kAT : 1][kTopEdits / 32]; // per thread: the list edits touching its child
 	for (u32 i = threadIdx.x; i < pm_words; i += kAT)
 		pm[i] = 0;
 	// the pass's edits and the filled chain are read by every classification: keep them in LDS
@@ -1133,9 +1149,11 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			const bool in = lane < 8u * n;
 			PairOut o{0, 0, make_uint4(0, 0, 0, 0), kNull};
 			Entry E{};
+			u32 *bits = nullptr;
 			if (in) {
 				E = cur[lane >> 3];
-				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap);
+				bits = kLists && E.llen <= kTopEdits ? sbits[lane] : nullptr;
+				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap, bits);
 			}
 			const u64 bp = __ballot(o.flag & 1u), bt = __ballot((o.flag >> 1) & 1u), bu = __ballot((o.flag >> 2) & 1u);
 			const u64 below = (1ull << lane) - 1ull;
@@ -1152,7 +1170,7 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 				                                      ((u32)((bu >> lane) & 0xFFu) << 16)
 				                                : 0u;
 				const u32 idx = (u32)__popcll(bp & below);
-				Entry ne = emit_pair_e(P, l, A, N, pm, lane, E, o.kid, m8, o, idx, lc - o.lcnt);
+				Entry ne = emit_pair_e(P, l, A, N, pm, lane, E, o.kid, m8, o, idx, lc - o.lcnt, bits);
 				if ((o.flag & 1u) && idx < kTopEnt)
 					nxt[idx] = ne;
 			}
@@ -1180,9 +1198,11 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			const bool in = t < (u64)n * 8u;
 			PairOut o{0, 0, make_uint4(0, 0, 0, 0), kNull};
 			Entry E{};
+			u32 *bits = nullptr;
 			if (in) {
 				E = cur[t >> 3];
-				o = expand_pair_e(P, l, A, t, E, ev, pm, ev_cap);
+				bits = kLists && E.llen <= kTopEdits ? sbits[threadIdx.x] : nullptr;
+				o = expand_pair_e(P, l, A, t, E, ev, pm, ev_cap, bits);
 			}
 			sflag[threadIdx.x] = o.flag;
 			Pair tile;
@@ -1195,7 +1215,7 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 						m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k)) | (((f >> 2) & 1u) << (16 + k));
 					}
 				const u32 idx = base.a + off.a;
-				Entry ne = emit_pair_e(P, l, A, N, pm, t, E, o.kid, m8, o, idx, base.b + off.b);
+				Entry ne = emit_pair_e(P, l, A, N, pm, t, E, o.kid, m8, o, idx, base.b + off.b, bits);
 				if ((o.flag & 1u) && idx < kTopEnt)
 					nxt[idx] = ne;
 			}
@@ -4015,8 +4035,8 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		++top_end;
 	if (reserve)
 		HD_RET(gen_reserve(p, B, 1));
-	a_top<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.roots, B.pm, kPmHdr + 2 * L, top_end, B.ev, (u32)pp.ev_cap,
-	                               B.gen_ctr);
+	const auto top = pp.count > 1 ? a_top<true> : a_top<false>;
+	top<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.roots, B.pm, kPmHdr + 2 * L, top_end, B.ev, (u32)pp.ev_cap, B.gen_ctr);
 	for (u32 l = top_end; l + 1 < L; ++l) {
 		u32 G = grid_for(8 * pp.cap[l] * lf, 2048, kMaxEmitBlocks);
 		a_count<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.ev, (u32)pp.ev_cap, B.bsum);
