@@ -24,6 +24,13 @@ __device__ inline float fmx(float a, float b) { return a < b ? b : a; } // std::
 __device__ inline float fmn(float a, float b) { return b < a ? b : a; }
 __device__ inline u32 f2u(float f) { return __float_as_uint(f); }
 __device__ inline float u2f(u32 u) { return __uint_as_float(u); }
+// Bit i of the result = (byte i of x != 0), i < 4: bit 7 of each byte is set by the byte's top bit
+// or by a carry out of its low 7 bits (no carry crosses bytes); the multiply by 2^0+2^7+2^14+2^21
+// moves bits 7, 15, 23, 31 to 28..31 without overlapping partial products.
+__device__ inline u32 nonzero_bytes4(u32 x) {
+	const u32 t = (((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+	return (t * 0x00204081u) >> 28u;
+}
 
 struct MarchOut {
 	bool hit;
@@ -88,18 +95,15 @@ __device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_l
 		if (++iter > kMaxIter)
 			return out;
 		if (child_bits == 0u) {
-			if (scale > leaf_scale) {
-				child_bits = W[parent];
+			if (scale >= leaf_scale) { // one load path for inner nodes and leaves (lanes diverge less)
+				const bool at_leaf = scale == leaf_scale;
+				// word 1 exists for both (a leaf's second mask word, an inner node's first child)
+				const u32 w0 = W[parent], w1 = W[parent + 1u];
+				child_bits = w0;
 				if (kCount)
-					reads += 1;
-			} else if (scale == leaf_scale) { // DAG_GetLeafFirstChildBits (trace.frag:56-67)
-				u32 l0 = W[parent], l1 = W[parent + 1u];
-				if (kCount)
-					reads += 2;
-				child_bits = ((l0 & 0x000000FFu) ? 0x01u : 0u) | ((l0 & 0x0000FF00u) ? 0x02u : 0u) |
-				             ((l0 & 0x00FF0000u) ? 0x04u : 0u) | ((l0 & 0xFF000000u) ? 0x08u : 0u) |
-				             ((l1 & 0x000000FFu) ? 0x10u : 0u) | ((l1 & 0x0000FF00u) ? 0x20u : 0u) |
-				             ((l1 & 0x00FF0000u) ? 0x40u : 0u) | ((l1 & 0xFF000000u) ? 0x80u : 0u);
+					reads += at_leaf ? 2 : 1;
+				if (at_leaf) // DAG_GetLeafFirstChildBits (trace.frag:56-67): bit i = byte i != 0
+					child_bits = nonzero_bytes4(w0) | (nonzero_bytes4(w1) << 4u);
 			} else
 				child_bits = parent;
 		}
@@ -116,8 +120,12 @@ __device__ inline MarchOut march(const u32 *__restrict__ W, u32 root, u32 leaf_l
 			h = tc_max;
 			if (kCount)
 				reads += 1;
-			parent = scale > leaf_scale ? W[parent + 1u + (u32)__builtin_popcount(child_bits & (child_mask - 1u))]
-			                            : (W[parent + (child_shift >> 2u)] >> ((child_shift & 3u) << 3u)) & 0xFFu;
+			// one load for both node kinds: an inner node's child pointer, or the leaf word holding the
+			// child's byte
+			const bool inner = scale > leaf_scale;
+			const u32 w = W[inner ? parent + 1u + (u32)__builtin_popcount(child_bits & (child_mask - 1u))
+			                      : parent + (child_shift >> 2u)];
+			parent = inner ? w : (w >> ((child_shift & 3u) << 3u)) & 0xFFu;
 			idx = 0u;
 			--scale;
 			scale_exp2 = half;
