@@ -7,8 +7,11 @@
 // (the full notice is reproduced at trace.frag:25-50); after Laine & Karras, "Efficient Sparse
 // Voxel Octrees" (I3D 2010).
 //
-// Launch: 256 threads = a 16x16 pixel tile; the per-scale parent stack lives in LDS laid out
-// [scale][thread] so a wavefront's stack accesses hit 64 consecutive banks.
+// Launch: one workgroup per pixel tile (8x8 or 16x16, from the tuning); the per-scale parent stack
+// lives in LDS laid out [scale][thread] so a wavefront's stack accesses hit 64 consecutive banks.
+// The march is bound by VALU issue (about 90 vector instructions per wave iteration, with the
+// descend / advance / pop paths serialised under divergence), not by memory: fewer instructions
+// per iteration and a shorter grid tail are what speed it up.
 #include "hd_internal.hpp"
 
 #include <cmath>
@@ -344,7 +347,10 @@ template <bool kCount, u32 kT> __global__ void __launch_bounds__(kT *kT) k_trace
 			hi = mid;
 	}
 	const hd_view V = A.views[lo];
-	u32 t = tile - A.tile_start[lo];
+	// Tiles go out in reverse raster order, bottom row first: a view's lower rows usually see the
+	// near ground, whose marches are the longest, so they start first and the short sky tiles fill
+	// the end of the grid (longest-first order shortens its tail).
+	u32 t = A.tile_start[lo + 1] - 1u - tile;
 	u32 tiles_x = (V.width + kT - 1) / kT;
 	u32 px = (t % tiles_x) * kT + (threadIdx.x % kT), py = (t / tiles_x) * kT + (threadIdx.x / kT);
 	if (px >= V.width || py >= V.height)
