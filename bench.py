@@ -50,8 +50,9 @@ def main():
     ap.add_argument("--plain-prefix", type=int, default=512, help="edits of the batch also run plain, for the ratio")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--trace-only", action="store_true",
-                    help="terrain + one edit batch, then --frames timed frames and a copy of --copy-mib MiB "
-                         "(the byte-count calibration of the HBM counters); for rocprofv3 runs")
+                    help="terrain + the edit batches of the warmup and timed steps (the scene of the last "
+                         "timed frame), then --frames frames and a copy of --copy-mib MiB (the byte-count "
+                         "calibration of the HBM counters); for rocprofv3 runs")
     ap.add_argument("--frames", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=3000)
@@ -111,7 +112,7 @@ def main():
         return pool.timer_end()
 
     if args.trace_only:
-        return trace_only(args, pool, batches[0], render, npx)
+        return trace_only(args, pool, batches, render, npx)
 
     edit_ms, trace_ms, step_ms = [], [], []
     for s in range(args.warmup + args.steps):
@@ -292,11 +293,12 @@ def batched_section(args, cfg, heights, res):
     return out
 
 
-def trace_only(args, pool, batch, render, npx):
+def trace_only(args, pool, batches, render, npx):
     """Profiling driver: the edited terrain, --frames frames of k_trace, then one device copy of
     --copy-mib MiB (a kernel of known traffic: reads and writes exactly that many bytes)."""
     from vkhashdag_amd import capi
-    pool.edit(batch)
+    for batch in batches:
+        pool.edit(batch)
     ms = [render() for _ in range(args.frames)]
     nbytes = args.copy_mib << 20
     src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
