@@ -84,3 +84,24 @@ def test_gc_empty_pool():
     dev.gc()
     ora.gc()
     assert_same_pool(dev, ora, "empty gc")
+
+
+def test_gc_twice_leaves_the_pool_unchanged():
+    """A second GC finds every live node already packed in pointer order in its bucket: pool words,
+    bucket counters and roots stay as they are (so repeating GC on a compacted pool, as the
+    benchmark's steady-state GC does, redoes exactly the first call's work)."""
+    capi = _capi()
+    cfg = small_cfg(64, 6, 5, 2, 3)
+    dev, ora = capi.DevicePool(cfg), O.OraclePool(cfg)
+    e = random_edits(30, 64, 7, True)
+    dev.edit(e)
+    ora.edit(e)
+    dev.gc()
+    ora.gc()
+    w1, r1 = dev.full_words().copy(), dev.root
+    dev.gc()
+    ora.gc()
+    assert dev.root == r1
+    assert np.array_equal(dev.full_words(), w1)
+    assert_same_pool(dev, ora, "second gc")
+

@@ -201,6 +201,21 @@ def test_gc_reference_multiworker_canonical():
     assert np.array_equal(r.canonical(), before)
 
 
+@pytest.mark.parametrize("seed", [7, 8])
+def test_gc_idempotent(seed):
+    """GC of a compacted pool changes nothing: leaves are already packed in pointer order and inner
+    nodes re-hash to the buckets and slots they occupy (a wrong processing order or padding rule
+    would move them on the second pass)."""
+    cfg = small_cfg(64, 6, 5, 2, 3)
+    o = O.OraclePool(cfg)
+    o.edit(random_edits(30, 64, seed, True))
+    o.gc()
+    w1, r1 = o.full_words().copy(), o.root
+    o.gc()
+    assert o.root == r1
+    assert np.array_equal(o.full_words(), w1)
+
+
 # ------------------------------------------------------------------ mathematics
 @pytest.mark.parametrize("colored", [True, False])
 def test_voxels_and_colors_equal_brute_force(colored):
