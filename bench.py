@@ -233,6 +233,9 @@ def main():
             "traffic": profiled_traffic(args),
             "traffic_source": TRAFFIC_PROFILE if profiled_traffic(args) else None,
             "algorithmic_bytes_per_launch": bytes_frame,
+            # what limits the kernel instead: vector-instruction issue, from the SQ counters
+            "valu_issue_frac": profiled_valu_issue(args),
+            "valu_issue_source": VALU_PROFILE if profiled_valu_issue(args) else None,
         },
         "cpu_baseline": cpu,
         "gc_ms": gc_ms,
@@ -379,6 +382,24 @@ def profiled_traffic(args):
     try:
         for line in open(os.path.join(ROOT, TRAFFIC_PROFILE)):
             if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
+
+
+VALU_PROFILE = "profiles/k_trace_4096_terrain_1080p_instructions.md"
+
+
+def profiled_valu_issue(args):
+    """Fraction of the vector-instruction issue rate k_trace reaches, from the committed rocprofv3
+    SQ_INSTS_VALU run (wave instructions x 4 cycles over 1024 SIMDs at 2.4 GHz for the kernel-trace
+    average duration); only for the workload it was taken on."""
+    if (args.res, args.width, args.height) != (4096, 1920, 1080):
+        return None
+    try:
+        for line in open(os.path.join(ROOT, VALU_PROFILE)):
+            if line.startswith("valu_issue_frac:"):
                 return float(line.split(":")[1])
     except OSError:
         return None

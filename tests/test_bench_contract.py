@@ -36,3 +36,20 @@ def test_profiled_traffic_matches_the_summary():
     assert abs((durs[1] + durs[2]) / 2 - avg) < 0.1
     # another workload has no profiled traffic
     assert bench.profiled_traffic(types.SimpleNamespace(res=1024, width=1920, height=1080)) is None
+
+
+def test_profiled_valu_issue_matches_the_counters():
+    import bench
+    args = types.SimpleNamespace(res=4096, width=1920, height=1080)
+    f = bench.profiled_valu_issue(args)
+    text = open(os.path.join(ROOT, bench.VALU_PROFILE)).read()
+    trace = open(os.path.join(ROOT, bench.TRAFFIC_PROFILE)).read()
+    # recompute from the table: mean SQ_INSTS_VALU of frames 2-3 over the issue capacity of the
+    # kernel-trace average (1024 SIMDs, 4 cycles per wave64 instruction, 2.4 GHz)
+    row = re.search(r"\| SQ_INSTS_VALU \| ([^\n]+) \|", text).group(1).split("|")
+    valu = (float(row[1]) + float(row[2])) / 2
+    avg_us = float(re.search(r"trace_avg_us: ([0-9.]+)", trace).group(1))
+    cap = 256 * 4 * avg_us * 1e-6 * 2.4e9 / 4
+    assert f is not None and 0.0 < f <= 1.0
+    assert abs(valu / cap - f) < 0.01
+    assert bench.profiled_valu_issue(types.SimpleNamespace(res=1024, width=1920, height=1080)) is None
