@@ -144,9 +144,11 @@ enum {
 	HD_MODE_PLAIN = 0,  /* one level-synchronous pass per edit, edits applied in order: the result
 	                       (pool words, bucket counters, color arrays, roots) equals NodePoolBase::Edit
 	                       applied to each edit in turn (NodePool.hpp:405-417) */
-	HD_MODE_BATCHED = 1 /* all edits in one level-synchronous pass with per-node ordered edit lists;
-	                       the reachable DAG and the colors of occupied voxels equal sequential
-	                       application */
+	HD_MODE_BATCHED = 1 /* dependency-ordered passes: each edit goes to the pass after every earlier
+	                       edit whose voxels it may share, and one level-synchronous pass applies all
+	                       edits of a pass with per-node ordered edit lists (32 passes for 4096 mixed
+	                       edits on the 4096^3 terrain); the reachable DAG and the colors of occupied
+	                       voxels equal sequential application */
 };
 
 /* Apply n edits to (root, color_root); writes the new roots. Replaces NodePoolBase::Edit and
@@ -195,7 +197,9 @@ int hd_trace_beam_async(hd_pool *pool, uint32_t root, uint32_t color_root, const
                         uint32_t n_tiles, uint32_t n_beam, float *d_beam, uint32_t *rgba);
 /* Host helper: beam_start[0..n_views] prefix of beam texels (ceil(W/8) x ceil(H/8) per beam view). */
 int hd_trace_beam_sizes(const hd_view *views, uint32_t n_views, uint32_t *beam_start);
-/* Host helper: tile_start[0..n_views] prefix of 16x16 tiles per view. */
+/* Host helper: tile_start[0..n_views] prefix of pixel tiles per view; the tile edge is the current
+ * hd_tuning.trace_tile (8 or 16), so compute the prefix after any hd_tuning_set and before the
+ * hd_trace_async / hd_trace_beam_async call it is passed to. */
 int hd_trace_tiles(const hd_view *views, uint32_t n_views, uint32_t *tile_start);
 /* HIP-event timing of the work queued on the pool's stream between the two calls. */
 int hd_timer_begin(hd_pool *pool);

@@ -2,8 +2,10 @@
 // TEST INFRASTRUCTURE ONLY: loaded by tests/ and bench.py's cpu_baseline leg, never by the product.
 #include "hd_oracle.hpp"
 
+#include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <thread>
 
 using namespace hdo;
 
@@ -255,30 +257,21 @@ void hdo_march_rays_words(const u32 *words, u64 nwords, u32 L, u32 root, float p
 // view: pos[3], look[3], side[3], up[3], proj_factor; flags bit0 = shadow.
 // rgba: W*H u32 (RGBA8, round-to-nearest), rec: W*H*12 u32 (hdo_march_rays_words layout;
 // rec[11] = shadow hit flag).
-void hdo_render_arrays(const u32 *words, u64 nwords, u32 L, u32 root, const u32 *cnodes, u64 ncnodes,
-                       const u32 *cleaves, u64 ncleaves, u32 color_root, u32 color_leaf_level, const float *view,
-                       u32 W, u32 H, u32 flags, u32 *rgba, u32 *rec) {
-	WordView v{words, nwords};
-	ColorPool cp;
-	cp.nodes.resize(ncnodes);
-	for (u64 i = 0; i < ncnodes; ++i)
-		std::copy(cnodes + 8 * i, cnodes + 8 * i + 8, cp.nodes[i].begin());
-	cp.leaves.assign(cleaves, cleaves + ncleaves);
+} // extern "C"
+
+namespace {
+// One frame over rows [y0, y1) (see hdo_render_arrays).
+template <typename WordFn>
+void render_rows(const WordFn &v, u32 L, u32 root, const ColorPool &cp, u32 color_root, u32 color_leaf_level,
+                 const float *view, u32 W, u32 H, u32 flags, const std::vector<float> &B, u32 y0, u32 y1, u32 *rgba,
+                 u32 *rec) {
 	V3 pos{view[0], view[1], view[2]}, look{view[3], view[4], view[5]}, side{view[6], view[7], view[8]},
 	    up{view[9], view[10], view[11]};
 	float proj = view[12];
 	const V3 light = normalize3(V3{4.0f, 5.0f, 3.0f});
-	// flags bit 2: the beam prepass (beam.frag) and beam-started primary rays (trace.frag:384-390)
 	const bool beam = (flags & 4u) != 0;
 	const u32 Wb = (W + kBeamBlock - 1) / kBeamBlock, Hb = (H + kBeamBlock - 1) / kBeamBlock;
-	std::vector<float> B;
-	if (beam) {
-		B.resize((size_t)Wb * Hb);
-		for (u32 by = 0; by < Hb; ++by)
-			for (u32 bx = 0; bx < Wb; ++bx)
-				B[(size_t)by * Wb + bx] = beam_value(v, root, L, proj, pos, look, side, up, bx, by, Wb, Hb);
-	}
-	for (u32 py = 0; py < H; ++py)
+	for (u32 py = y0; py < y1; ++py)
 		for (u32 px = 0; px < W; ++px) {
 			float u = (float(px) + 0.5f) / float(W), w = (float(py) + 0.5f) / float(H);
 			float cx = u * 2.0f - 1.0f, cy = w * 2.0f - 1.0f;
@@ -317,6 +310,113 @@ void hdo_render_arrays(const u32 *words, u64 nwords, u32 L, u32 root, const u32 
 				o[11] = shadow ? 1u : 0u;
 			}
 		}
+}
+
+// A frame split by rows over `threads` host threads (each pixel is independent, so the image does
+// not depend on the split).
+template <typename WordFn>
+void render_frame(const WordFn &v, u32 L, u32 root, const ColorPool &cp, u32 color_root, u32 color_leaf_level,
+                  const float *view, u32 W, u32 H, u32 flags, u32 *rgba, u32 *rec, u32 threads) {
+	V3 pos{view[0], view[1], view[2]}, look{view[3], view[4], view[5]}, side{view[6], view[7], view[8]},
+	    up{view[9], view[10], view[11]};
+	const bool beam = (flags & 4u) != 0;
+	const u32 Wb = (W + kBeamBlock - 1) / kBeamBlock, Hb = (H + kBeamBlock - 1) / kBeamBlock;
+	std::vector<float> B;
+	if (beam) {
+		B.resize((size_t)Wb * Hb);
+		for (u32 by = 0; by < Hb; ++by)
+			for (u32 bx = 0; bx < Wb; ++bx)
+				B[(size_t)by * Wb + bx] = beam_value(v, root, L, view[12], pos, look, side, up, bx, by, Wb, Hb);
+	}
+	threads = std::max(1u, std::min(threads, H));
+	if (threads == 1) {
+		render_rows(v, L, root, cp, color_root, color_leaf_level, view, W, H, flags, B, 0, H, rgba, rec);
+		return;
+	}
+	std::vector<std::thread> pool;
+	std::atomic<u32> next{0};
+	for (u32 t = 0; t < threads; ++t)
+		pool.emplace_back([&] {
+			for (u32 y; (y = next.fetch_add(8)) < H;)
+				render_rows(v, L, root, cp, color_root, color_leaf_level, view, W, H, flags, B, y, std::min(H, y + 8),
+				            rgba, rec);
+		});
+	for (auto &t : pool)
+		t.join();
+}
+} // namespace
+
+extern "C" {
+
+// Full frame as the device renders it: Camera_GenRay (trace.frag:366-370, 375), DAG_RayMarch,
+// optional shadow ray toward normalize(4,5,3) from the hit position (LOD off), Color_Fetch and
+// the diffuse shading of trace.frag:395-397 (shadowed pixels keep the 0.5 ambient term).
+// view: pos[3], look[3], side[3], up[3], proj_factor; flags bit0 = shadow, bit2 = beam prepass
+// (beam.frag) with beam-started primary rays (trace.frag:384-390).
+// rgba: W*H u32 (RGBA8, round-to-nearest), rec: W*H*12 u32 (hdo_march_rays_words layout;
+// rec[11] = shadow hit flag).
+void hdo_render_arrays(const u32 *words, u64 nwords, u32 L, u32 root, const u32 *cnodes, u64 ncnodes,
+                       const u32 *cleaves, u64 ncleaves, u32 color_root, u32 color_leaf_level, const float *view,
+                       u32 W, u32 H, u32 flags, u32 *rgba, u32 *rec) {
+	WordView v{words, nwords};
+	ColorPool cp;
+	cp.nodes.resize(ncnodes);
+	for (u64 i = 0; i < ncnodes; ++i)
+		std::copy(cnodes + 8 * i, cnodes + 8 * i + 8, cp.nodes[i].begin());
+	cp.leaves.assign(cleaves, cleaves + ncleaves);
+	render_frame(v, L, root, cp, color_root, color_leaf_level, view, W, H, flags, rgba, rec, 1);
+}
+// The same frame straight from an oracle pool (no flat copy of the node pool), over `threads` threads.
+void hdo_render_pool(void *p, u32 root, u32 color_root, const float *view, u32 W, u32 H, u32 flags, u32 *rgba,
+                     u32 *rec, u32 threads) {
+	auto *o = static_cast<OPool *>(p);
+	auto fn = [o](u64 i) { return o->pool.word(i); };
+	render_frame(fn, o->pool.cfg.node_levels(), root, o->color, color_root, o->color.leaf_level, view, W, H, flags,
+	             rgba, rec, threads);
+}
+
+// Used words of every bucket packed in bucket order (hd_pool_read_live_words' layout); returns the count.
+u64 hdo_live_words(void *p, u32 *out, u64 cap) {
+	auto *o = static_cast<OPool *>(p);
+	const auto &pool = o->pool;
+	const auto &cfg = pool.cfg;
+	u64 at = 0;
+	for (u64 b = 0; b < cfg.total_buckets(); ++b) {
+		u64 base = b << cfg.word_bits_per_bucket();
+		for (u32 i = 0; i < pool.bucket_words[b]; ++i, ++at)
+			if (at < cap)
+				out[at] = pool.word(base + i);
+	}
+	return at;
+}
+
+// Occupancy of n voxels (x, y, z) of the DAG below root: descend node by node (NodeCoord child
+// index x | y << 1 | z << 2, NodeCoord.hpp:18-30; leaf bit index NodeCoord.hpp:32-43).
+void hdo_voxel_query(void *p, u32 root, const u32 *vox, u64 n, uint8_t *out) {
+	auto *o = static_cast<OPool *>(p);
+	const auto &pool = o->pool;
+	const u32 L = pool.cfg.node_levels();
+	for (u64 k = 0; k < n; ++k) {
+		const u32 x = vox[3 * k], y = vox[3 * k + 1], z = vox[3 * k + 2];
+		u32 ptr = root;
+		bool occ = false;
+		for (u32 l = 0; ptr != kNull; ++l) {
+			if (l + 1 == L) { // leaf: 4^3 voxels, bit = (hi-bits xyz) << 3 | (lo-bits xyz)
+				const u32 lx = x & 3u, ly = y & 3u, lz = z & 3u;
+				const u32 bit = (((lx >> 1) | ((ly >> 1) << 1) | ((lz >> 1) << 2)) << 3) |
+				                ((lx & 1u) | ((ly & 1u) << 1) | ((lz & 1u) << 2));
+				occ = (pool.word((u64)ptr + (bit >> 5)) >> (bit & 31u)) & 1u;
+				break;
+			}
+			const u32 sh = L - l; // child size at level l + 1 is 2^(L - l) voxels
+			const u32 ci = ((x >> sh) & 1u) | (((y >> sh) & 1u) << 1) | (((z >> sh) & 1u) << 2);
+			const u32 m = pool.word(ptr) & 0xFFu;
+			if (!((m >> ci) & 1u))
+				break;
+			ptr = pool.word((u64)ptr + 1 + (u32)std::popcount(m & ((1u << ci) - 1u)));
+		}
+		out[k] = occ ? 1 : 0;
+	}
 }
 
 // Brute-force DDA on a dense grid (res^3 bytes): out = vox[3] + hit flag per ray.

@@ -79,6 +79,9 @@ def olib():
         _sig(L, "hdo_dda", None, [u8p, U32, f32p, f32p, U64, u32p])
         _sig(L, "hdo_beam_image", None, [u32p, U64, U32, U32, f32p, U32, U32, f32p])
         _sig(L, "hdo_load_live", None, [VP, u32p, u32p, U64, u32p, U32, u32p, U64, u32p, U64])
+        _sig(L, "hdo_render_pool", None, [VP, U32, U32, f32p, U32, U32, U32, u32p, VP, U32])
+        _sig(L, "hdo_live_words", U64, [VP, VP, U64])
+        _sig(L, "hdo_voxel_query", None, [VP, U32, u32p, U64, u8p])
         _olib = L
     return _olib
 
@@ -207,6 +210,42 @@ class OraclePool(_PoolBase):
 
     def overflows(self):
         return int(olib().hdo_overflows(self.h))
+
+    def live_words(self):
+        """(bucket_words, used words of every bucket packed in bucket order), in one call."""
+        n = int(olib().hdo_live_words(self.h, None, 0))
+        out = np.zeros(max(1, n), np.uint32)
+        olib().hdo_live_words(self.h, out.ctypes.data, n)
+        return self.bucket_words(), out[:n]
+
+    def render(self, view, W, H, shadow=True, beam=False, threads=None, root=None, color_root=None):
+        """The frame hdo_render_arrays computes, read straight from this pool, over host threads."""
+        rgba = np.zeros(W * H, np.uint32)
+        rec = np.zeros((W * H, 12), np.uint32)
+        flags = (1 if shadow else 0) | (4 if beam else 0)
+        threads = threads or min(16, len(os.sched_getaffinity(0)))
+        olib().hdo_render_pool(self.h, self.root if root is None else root,
+                               self.color_root if color_root is None else color_root,
+                               np.ascontiguousarray(view, np.float32), W, H, flags, rgba, rec.ctypes.data, threads)
+        return rgba, rec
+
+    def canonical(self, roots=None):
+        """Canonical serialisation of the DAG below roots, read straight from this pool."""
+        roots = np.ascontiguousarray([self.root] if roots is None else roots, np.uint32)
+        cap = 1 << 20
+        while True:
+            out = np.zeros(cap, np.uint32)
+            n = int(olib().hdo_canonical(self.h, roots, len(roots), out, cap))
+            if n <= cap:
+                return out[:n]
+            cap = n
+
+    def voxel_query(self, vox, root=None):
+        """Occupancy (0/1) of voxels (x, y, z) of the DAG below root."""
+        vox = np.ascontiguousarray(vox, np.uint32).reshape(-1, 3)
+        out = np.zeros(len(vox), np.uint8)
+        olib().hdo_voxel_query(self.h, self.root if root is None else root, vox, len(vox), out)
+        return out
 
     def bucket_words(self):
         out = np.zeros(self.cfg.total_buckets, np.uint32)

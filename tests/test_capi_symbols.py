@@ -64,3 +64,53 @@ def test_tuning_set_get_and_validation():
         assert capi.get_tuning()["trace_tile"] == 8  # a rejected set changes nothing
     finally:
         capi.set_tuning(**old)
+
+
+def _ckpt_header(n_live, total_buckets=None, flags=0, n_roots=0):
+    """A checkpoint header (hd_pool.hip's CkptHeader) for a 64^3 pool of 2^6 buckets per level."""
+    import sys
+    sys.path.insert(0, ROOT)
+    from vkhashdag_amd import capi
+
+    class Hdr(ctypes.Structure):
+        _fields_ = [("magic", ctypes.c_char * 8), ("version", ctypes.c_uint32), ("n_roots", ctypes.c_uint32),
+                    ("cfg", capi.hd_config), ("total_buckets", ctypes.c_uint64), ("n_live", ctypes.c_uint64),
+                    ("color_nodes", ctypes.c_uint64), ("color_leaf_words", ctypes.c_uint64),
+                    ("overflow", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+    h = Hdr()
+    h.magic, h.version, h.n_roots = b"HDAGCKP1", 1, n_roots
+    h.cfg.word_bits_per_page, h.cfg.page_bits_per_bucket, h.cfg.node_levels = 5, 2, 5
+    for i in range(5):
+        h.cfg.bucket_bits[i] = 6
+    h.cfg.color_leaf_level, h.cfg.color_node_capacity, h.cfg.color_leaf_capacity = 2, 1 << 10, 1 << 12
+    h.total_buckets = 5 * 64 if total_buckets is None else total_buckets
+    h.n_live, h.flags = n_live, flags
+    return bytes(h), h.total_buckets
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+def test_checkpoint_header_counts_checked_before_the_gpu(tmp_path):
+    # a corrupt n_live, a bucket count that does not fit the config, a body of the wrong size and
+    # unknown flags are errors (not exceptions or huge allocations), found before any device call
+    lib = ctypes.CDLL(LIB)
+    lib.hd_last_error.restype = ctypes.c_char_p
+    h, n = ctypes.c_void_p(), ctypes.c_uint32(0)
+    roots = (ctypes.c_uint32 * 4)()
+    f = tmp_path / "ck"
+    hdr, nb = _ckpt_header(n_live=1 << 62)
+    f.write_bytes(hdr + bytes(4 * nb))
+    assert lib.hd_pool_load(str(f).encode(), ctypes.byref(h), roots, 4, ctypes.byref(n)) != 0
+    assert b"header counts" in lib.hd_last_error()
+    hdr, nb = _ckpt_header(n_live=0, total_buckets=7)
+    f.write_bytes(hdr + bytes(4 * 7))
+    assert lib.hd_pool_load(str(f).encode(), ctypes.byref(h), roots, 4, ctypes.byref(n)) != 0
+    assert b"header counts" in lib.hd_last_error()
+    hdr, nb = _ckpt_header(n_live=16)
+    f.write_bytes(hdr + bytes(4 * nb + 4 * 15))  # one live word short
+    assert lib.hd_pool_load(str(f).encode(), ctypes.byref(h), roots, 4, ctypes.byref(n)) != 0
+    assert b"truncated" in lib.hd_last_error()
+    hdr, nb = _ckpt_header(n_live=0, flags=6)
+    f.write_bytes(hdr + bytes(4 * nb))
+    assert lib.hd_pool_load(str(f).encode(), ctypes.byref(h), roots, 4, ctypes.byref(n)) != 0
+    assert b"flags" in lib.hd_last_error()
+    assert not h.value

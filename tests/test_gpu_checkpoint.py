@@ -70,3 +70,44 @@ def test_load_rejects_bad_files(tmp_path):
     b, extra = capi.DevicePool.load(good)
     assert extra == []
     assert_same_device_pools(a, b, "small pool")
+
+
+def test_save_load_then_gc_keeps_the_filled_chain(tmp_path):
+    # the filled chain is a GC root once created: a loaded pool knows it was created, so a GC right
+    # after the load compacts exactly as on the original
+    capi = _capi()
+    res = 64
+    cfg = small_cfg(res, 6, 5, 2, 3)
+    a = capi.DevicePool(cfg)
+    a.edit(W.edits([W.edit(W.AABB_FILL, (0, 0, 0), (64, 64, 40), color=0x3355AA),
+                    W.edit(W.SPHERE_DIG, (30, 30, 38), r2=60)]))
+    assert len(a.filled()) == cfg.node_levels
+    path = tmp_path / "pool.hdag"
+    a.save(path)
+    b, _ = capi.DevicePool.load(path)
+    assert_same_device_pools(a, b, "after load")
+    a.gc()
+    b.gc()
+    assert_same_device_pools(a, b, "GC right after the load")
+    assert np.array_equal(a.filled(), b.filled())
+
+
+def test_interop_rejects_bucket_counters_over_capacity():
+    capi = _capi()
+    cfg = small_cfg(64, 6, 5, 2, 3)
+    a = capi.DevicePool(cfg)
+    a.edit(W.edits([W.edit(W.SPHERE_FILL, (32, 32, 32), r2=100, color=0x123456)]))
+    bw, live = a.live_words()
+    before = (a.bucket_words().copy(), a.live_words()[1].copy())
+    bad = bw.copy()
+    b = int(np.nonzero(bw)[0][0])
+    bad[b] = cfg.words_per_bucket + 1
+    with pytest.raises(capi.HDError, match="holds"):
+        a.write_live_words(bad, np.zeros(int(bad.sum()), np.uint32))
+    with pytest.raises(capi.HDError, match="holds"):
+        capi.check(capi.lib().hd_pool_write_bucket_words(a.h, b, 1, bad[b:b + 1].ctypes.data))
+    with pytest.raises(capi.HDError, match="counters sum"):
+        a.write_live_words(bw, live[:-1])
+    # nothing was written by the rejected calls
+    assert np.array_equal(a.bucket_words(), before[0])
+    assert np.array_equal(a.live_words()[1], before[1])

@@ -480,3 +480,149 @@ def test_beam_prepass_is_conservative():
         # the beam image: a finite block value never exceeds the plain hit distance of its pixels
         B = O.beam_image(words, cfg.node_levels, ora.root, view, Wd, Ht)
         assert np.all(B[np.isfinite(B)] >= 0.0)
+
+
+# ------------------------------------------------------------------ shadow rays, LOD cut, shading
+def _shading_scene(lod, seed=0, Wd=160, Ht=96):
+    res = 128
+    h = W.value_noise_heightmap(res, 4, 9, 20, 90)
+    cfg = small_cfg(res, 8, 9, 2, 3)
+    p = O.OraclePool(cfg, heights=h)
+    p.edit(W.edits([W.terrain_edit(res, seed=3, skin=6)] + list(W.terrain_sphere_edits(10, res, h, 3, 20, seed=8))))
+    # cameras on the light's side (+x, +y, above), so the frame holds lit and shadowed pixels
+    theta, phi = (0.8, 0.35, 1.25)[seed], (0.7, 0.5, 0.9)[seed]
+    c = np.array([64.0, 64.0, 40.0])
+    pos = c + 150 * np.array([np.cos(theta) * np.cos(phi), np.sin(theta) * np.cos(phi), np.sin(phi)])
+    look, side, up = W.look_side_up(c - pos, aspect=Wd / Ht)
+    proj = Ht / (2.0 * np.tan(np.pi / 6)) if lod else float("inf")
+    view = W.view_block(pos / res, look, side, up, proj)
+    words = p.full_words()
+    rgba, rec = O.render(words, cfg.node_levels, p.root, p.color_nodes(), p.color_leaves(), p.color_root,
+                         cfg.color_leaf_level, view, Wd, Ht, shadow=True)
+    return p, cfg, words, view, rgba, rec
+
+
+def _light():
+    light = np.array([4, 5, 3], np.float32)
+    return (light / np.sqrt(np.float32((light * light).sum(dtype=np.float32)))).astype(np.float32)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_shadow_flags_match_dda(seed):
+    # The shadow ray (LOD off) starts at the hit position trace.frag:248-258 computes -- epsilon off
+    # the hit face -- toward normalize(4, 5, 3), the shading light of trace.frag:396. A brute-force
+    # DDA on the decoded grid from the same point gives the same occluded / lit answer.
+    p, cfg, words, view, rgba, rec = _shading_scene(False, seed)
+    hit = (rec[:, 3] >> 31).astype(bool)
+    o = np.ascontiguousarray(rec[hit, 4:7].view(np.float32))
+    d = np.ascontiguousarray(np.broadcast_to(_light(), o.shape))
+    grid = O.decode_voxels(words, cfg.node_levels, p.root)
+    dd = O.dda(grid, o, d)
+    shadow = rec[hit, 11].astype(bool)
+    assert 0.05 < shadow.mean() < 0.95, shadow.mean()
+    agree = shadow == (dd[:, 3] > 0)
+    # fp32 march vs fp64 DDA can only differ on rays grazing a voxel edge
+    assert agree.mean() >= 0.998, (agree.mean(), np.nonzero(~agree)[0][:8])
+    # a wrong light (a transposed component) disagrees on many pixels: the check has teeth
+    bad = O.dda(grid, o, np.ascontiguousarray(np.broadcast_to(_light()[[0, 2, 1]], o.shape)))
+    assert (shadow == (bad[:, 3] > 0)).mean() < agree.mean() - 0.01
+
+
+def _slab(o, d, lo, hi):
+    """t range of the ray o + t d inside the box [lo, hi] (float64)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        a = (lo - o) / d
+        b = (hi - o) / d
+    t0 = np.nanmax(np.minimum(a, b), axis=1)
+    t1 = np.nanmin(np.maximum(a, b), axis=1)
+    return t0, t1
+
+
+def test_lod_stop_cube_properties():
+    # With the LOD cut-off on (trace.frag:148), the march stops at a node cube the ray enters: that
+    # cube holds at least one voxel of the decoded grid, the ray enters it no later than the first
+    # voxel a brute-force DDA hits, and the stop rule held there: cube size x proj_factor is below
+    # the ray's exit distance from the cube.
+    # a small frame: a voxel projects to ~0.4 pixel, so the cut stops at 2- and 4-voxel nodes
+    Wd, Ht = 96, 56
+    p, cfg, words, view, rgba, rec = _shading_scene(True, 0, Wd, Ht)
+    res = cfg.resolution
+    hit = (rec[:, 3] >> 31).astype(bool)
+    size = (rec[:, 3] & 0x7FFFFFFF).astype(np.int64)
+    assert (size[hit] > 1).mean() > 0.05, "the LOD cut never engaged"
+    grid = O.decode_voxels(words, cfg.node_levels, p.root)
+    S = np.concatenate([[0], np.cumsum(np.cumsum(np.cumsum(grid.astype(np.int64), 0), 1), 2).ravel()])
+    cs = np.zeros((res + 1,) * 3, np.int64)
+    cs[1:, 1:, 1:] = S[1:].reshape(res, res, res)
+
+    def occupied(v, s):  # voxels of the grid ([z, y, x]) inside the cube [v, v + s)
+        x0, y0, z0 = v[:, 0], v[:, 1], v[:, 2]
+        x1, y1, z1 = x0 + s, y0 + s, z0 + s
+        return (cs[z1, y1, x1] - cs[z0, y1, x1] - cs[z1, y0, x1] - cs[z1, y1, x0] + cs[z0, y0, x1]
+                + cs[z0, y1, x0] + cs[z1, y0, x0] - cs[z0, y0, x0])
+
+    v = rec[hit, :3].astype(np.int64)
+    s = size[hit]
+    assert np.all(occupied(v, s) > 0)
+    # rays as Camera_GenRay builds them (float32), in world units [0,1]^3
+    px, py = np.meshgrid(np.arange(Wd, dtype=np.float32), np.arange(Ht, dtype=np.float32))
+    cx = ((px + np.float32(0.5)) / np.float32(Wd)) * np.float32(2) - np.float32(1)
+    cy = ((py + np.float32(0.5)) / np.float32(Ht)) * np.float32(2) - np.float32(1)
+    look, side, up = view[3:6], view[6:9], view[9:12]
+    d = np.stack([look[i] - side[i] * cx - up[i] * cy for i in range(3)], -1).reshape(-1, 3).astype(np.float32)
+    d = d / np.sqrt((d * d).sum(1, keepdims=True, dtype=np.float32)).astype(np.float32)
+    o = np.broadcast_to(view[:3], d.shape).astype(np.float32)
+    t0, t1 = _slab(o[hit].astype(np.float64), d[hit].astype(np.float64), v / res, (v + s[:, None]) / res)
+    assert np.all(t0 <= t1 + 1e-6)
+    dd = O.dda(grid, np.ascontiguousarray(o[hit]), np.ascontiguousarray(d[hit]))
+    g = dd[:, 3] > 0
+    tv0, _ = _slab(o[hit][g].astype(np.float64), d[hit][g].astype(np.float64), dd[g, :3] / res,
+                   (dd[g, :3] + 1) / res)
+    assert np.all(t0[g] <= tv0 + 1e-5)
+    # stop rule: scale_exp2 * proj_factor < tc_max (+ bias 0), scale in the [1,2] cube = size / res
+    proj = np.float64(view[12])
+    assert np.all((s / res) * proj < t1 * (1 + 1e-5))
+
+
+@pytest.mark.parametrize("lod", [False, True])
+def test_shading_equals_numpy_formula(lod):
+    # trace.frag:395-396 restated in numpy from the hit records: diffuse = max(dot(n, L), 0) * .5 + .5
+    # with L = normalize(4, 5, 3) (0.5 where the shadow ray is blocked), times Color_Fetch of the hit
+    # voxel, clamped and rounded to RGBA8; misses are black with alpha 255.
+    p, cfg, words, view, rgba, rec = _shading_scene(lod, 1)
+    hit = (rec[:, 3] >> 31).astype(bool)
+    L = _light()
+    n = rec[:, 7:10].view(np.float32)
+    nd = (n[:, 0] * L[0] + n[:, 1] * L[1]).astype(np.float32) + n[:, 2] * L[2]
+    diffuse = np.where(rec[:, 11] > 0, np.float32(0.5),
+                       np.maximum(nd, np.float32(0)) * np.float32(0.5) + np.float32(0.5)).astype(np.float32)
+    col = np.zeros((len(rec), 3), np.float32)
+    col[hit] = O.color_fetch(p.color_nodes(), p.color_leaves(), p.color_root, cfg.voxel_level, cfg.color_leaf_level,
+                             np.ascontiguousarray(rec[hit, :3]))
+    c = np.clip(diffuse[:, None] * col, 0, 1) * np.float32(255) + np.float32(0.5)
+    q = np.where(hit[:, None], np.floor(c), 0).astype(np.uint32)
+    want = q[:, 0] | (q[:, 1] << 8) | (q[:, 2] << 16) | np.uint32(255 << 24)
+    assert hit.mean() > 0.2 and (rec[hit, 11] > 0).any()
+    assert np.array_equal(rgba, want.astype(np.uint32))
+
+
+def test_voxel_query_and_pool_render_equal_array_checkers():
+    # the pool-handle checkers used at full size (voxel queries, threaded frames, bulk live words,
+    # canonical from the pool) equal the array checkers on a small scene
+    res = 64
+    cfg = small_cfg(res, 6, 5, 2, 3)
+    p = O.OraclePool(cfg)
+    p.edit(random_edits(24, res, 77, True))
+    words = p.full_words()
+    grid = O.decode_voxels(words, cfg.node_levels, p.root)
+    vox = np.argwhere(np.ones((res, res, res), bool))[:, ::-1].astype(np.uint32)
+    assert np.array_equal(p.voxel_query(vox), grid.reshape(-1))
+    view = W.orbit_view((32, 32, 32), 70, 2, 80, 60, res=res)
+    a = O.render(words, cfg.node_levels, p.root, p.color_nodes(), p.color_leaves(), p.color_root,
+                 cfg.color_leaf_level, view, 80, 60)
+    b = p.render(view, 80, 60, threads=4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    bw = p.bucket_words()
+    parts = [p.read_words(int(i) * cfg.words_per_bucket, int(bw[i])) for i in np.nonzero(bw)[0]]
+    assert np.array_equal(p.live_words()[1], np.concatenate(parts))
+    assert np.array_equal(p.canonical(), O.canonical_words(words, cfg.node_levels, np.array([p.root], np.uint32)))

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhashdag_amd.so")
-SOURCES = ["hd_pool.hip", "hd_edit.hip", "hd_trace.hip", "hd_gc.hip", "hd_batch.hip", "hd_serve.hip"]
+SOURCES = ["hd_pool.hip", "hd_edit.hip", "hd_trace.hip", "hd_gc.hip", "hd_serve.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HD_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Xclang",
@@ -32,7 +32,10 @@ def _compile(src: str) -> str:
 
 
 def build(verbose: bool = True) -> str:
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    missing = [s for s in SOURCES if not os.path.exists(os.path.join(CSRC, s))]
+    if missing:
+        raise FileNotFoundError(f"HIP sources missing from {CSRC}: {missing}")
+    srcs = list(SOURCES)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

@@ -7,6 +7,7 @@
 
 #include <cstdarg>
 #include <mutex>
+#include <stdexcept>
 
 
 namespace hd {
@@ -305,10 +306,23 @@ int hd_pool_read_bucket_words(hd_pool *h, uint64_t first, uint64_t count, uint32
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
 }
+// A bucket counter above the bucket's capacity would make appends and finds run into the next
+// bucket (or past the pool for the last one): every counter is checked before anything is copied.
+static int check_bucket_counters(const Pool &p, const uint32_t *bw, uint64_t first, uint64_t count,
+                                 const char *who) {
+	const u64 wpb = 1ull << p.dc.wbpb();
+	for (u64 i = 0; i < count; ++i)
+		if (bw[i] > wpb)
+			return set_error("%s: bucket %llu holds %u words (> %llu)", who, (unsigned long long)(first + i), bw[i],
+			                 (unsigned long long)wpb),
+			       HD_ERR_ARG;
+	return HD_OK;
+}
 int hd_pool_write_bucket_words(hd_pool *h, uint64_t first, uint64_t count, const uint32_t *in) {
 	Pool &p = h->p;
 	if (first + count > p.total_buckets)
 		return set_error("write beyond buckets"), HD_ERR_ARG;
+	HD_RET(check_bucket_counters(p, in, first, count, "hd_pool_write_bucket_words"));
 	HD_TRY(hipMemcpyAsync(p.bucket_words + first, in, count * 4, hipMemcpyHostToDevice, p.stream));
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
@@ -437,14 +451,20 @@ extern "C" int hd_pool_read_live_words(hd_pool *h, uint32_t *out, uint64_t cap, 
 }
 // Inverse: set every bucket's counter and used words (host interop / checkpoint load).
 extern "C" int hd_pool_write_live_words(hd_pool *h, const uint32_t *bucket_words, const uint32_t *live, uint64_t n) {
+	if (!h || !bucket_words || (!live && n))
+		return set_error("hd_pool_write_live_words: null argument"), HD_ERR_ARG;
 	Pool &p = h->p;
+	HD_RET(check_bucket_counters(p, bucket_words, 0, p.total_buckets, "hd_pool_write_live_words"));
+	u64 sum = 0;
+	for (u64 b = 0; b < p.total_buckets; ++b)
+		sum += bucket_words[b];
+	if (sum != n)
+		return set_error("hd_pool_write_live_words: %llu words given, counters sum to %llu", (unsigned long long)n,
+		                 (unsigned long long)sum),
+		       HD_ERR_ARG;
 	HD_TRY(hipMemcpyAsync(p.bucket_words, bucket_words, p.total_buckets * 4, hipMemcpyHostToDevice, p.stream));
 	u64 *off, total;
 	HD_RET(live_offsets(p, &off, &total));
-	if (total != n)
-		return set_error("hd_pool_write_live_words: %llu words given, counters sum to %llu", (unsigned long long)n,
-		                 (unsigned long long)total),
-		       HD_ERR_ARG;
 	u32 *tmp;
 	HD_RET(p.scratch.get<u32>("live_tmp", total + 1, &tmp));
 	HD_TRY(hipMemcpyAsync(tmp, live, total * 4, hipMemcpyHostToDevice, p.stream));
@@ -467,8 +487,9 @@ struct CkptHeader {
 	uint32_t version, n_roots;
 	hd_config cfg;
 	uint64_t total_buckets, n_live, color_nodes, color_leaf_words;
-	uint32_t overflow, pad_;
+	uint32_t overflow, flags; // flags bit 0: the filled-node chain had been created (it is a GC root)
 };
+constexpr uint32_t kCkptFilled = 1;
 constexpr char kCkptMagic[8] = {'H', 'D', 'A', 'G', 'C', 'K', 'P', '1'};
 
 struct File {
@@ -505,6 +526,7 @@ extern "C" int hd_pool_save(hd_pool *h, const char *path, const uint32_t *roots,
 	hd.color_nodes = meta[kMetaColorNodes];
 	hd.color_leaf_words = meta[kMetaColorLeaves];
 	hd.overflow = meta[kMetaOverflow];
+	hd.flags = p.filled.empty() ? 0u : kCkptFilled;
 	HD_RET(hd_pool_read_live_words(h, nullptr, 0, &hd.n_live));
 	std::vector<u32> bw(p.total_buckets), live(hd.n_live ? hd.n_live : 1), cn(8 * hd.color_nodes + 1),
 	    cl(hd.color_leaf_words + 1);
@@ -530,9 +552,17 @@ extern "C" int hd_pool_save(hd_pool *h, const char *path, const uint32_t *roots,
 	return HD_OK;
 }
 
+static int pool_load(const char *path, hd_pool **out, uint32_t *roots, uint32_t cap, uint32_t *n_roots);
 extern "C" int hd_pool_load(const char *path, hd_pool **out, uint32_t *roots, uint32_t cap, uint32_t *n_roots) {
 	if (!path || !out || !n_roots || (!roots && cap))
 		return set_error("hd_pool_load: null argument"), HD_ERR_ARG;
+	try {
+		return pool_load(path, out, roots, cap, n_roots);
+	} catch (const std::exception &e) {
+		return set_error("hd_pool_load: %s", e.what()), HD_ERR_ARG;
+	}
+}
+static int pool_load(const char *path, hd_pool **out, uint32_t *roots, uint32_t cap, uint32_t *n_roots) {
 	File F;
 	F.f = fopen(path, "rb");
 	if (!F.f)
@@ -545,6 +575,33 @@ extern "C" int hd_pool_load(const char *path, hd_pool **out, uint32_t *roots, ui
 		return set_error("hd_pool_load: %u roots, room for %u", hd.n_roots, cap), HD_ERR_ARG;
 	if (hd.color_nodes > hd.cfg.color_node_capacity || hd.color_leaf_words > hd.cfg.color_leaf_capacity)
 		return set_error("hd_pool_load: color counts exceed the saved capacities"), HD_ERR_ARG;
+	if (hd.flags & ~kCkptFilled)
+		return set_error("hd_pool_load: unknown header flags 0x%x", hd.flags), HD_ERR_ARG;
+	// counts against the config and the file size, before any device call or large allocation
+	if (hd.cfg.node_levels == 0 || hd.cfg.node_levels > HD_MAX_LEVELS || hd.cfg.word_bits_per_page > 31 ||
+	    hd.cfg.page_bits_per_bucket > 31)
+		return set_error("hd_pool_load: invalid config in the header"), HD_ERR_ARG;
+	u64 buckets = 0;
+	for (u32 l = 0; l < hd.cfg.node_levels; ++l) {
+		if (hd.cfg.bucket_bits[l] > 32)
+			return set_error("hd_pool_load: invalid bucket bits in the header"), HD_ERR_ARG;
+		buckets += 1ull << hd.cfg.bucket_bits[l];
+	}
+	const u64 wbpb = (u64)hd.cfg.word_bits_per_page + hd.cfg.page_bits_per_bucket;
+	if (buckets != hd.total_buckets || wbpb > 32 || hd.n_live > (buckets << wbpb))
+		return set_error("hd_pool_load: header counts do not fit the config"), HD_ERR_ARG;
+	const u64 body = hd.total_buckets * 4 + hd.n_live * 4 + hd.color_nodes * 32 + hd.color_leaf_words * 4 +
+	                 (u64)hd.n_roots * 4;
+	const long here = ftell(F.f);
+	if (here < 0 || fseek(F.f, 0, SEEK_END) != 0)
+		return set_error("hd_pool_load: cannot size %s", path), HD_ERR_ARG;
+	const long end = ftell(F.f);
+	if (end < 0 || fseek(F.f, here, SEEK_SET) != 0)
+		return set_error("hd_pool_load: cannot size %s", path), HD_ERR_ARG;
+	if ((u64)(end - here) != body)
+		return set_error("hd_pool_load: %s is truncated or oversized: %lld body bytes, the header describes %llu", path,
+		                 (long long)(end - here), (unsigned long long)body),
+		       HD_ERR_ARG;
 	hd_pool *h = nullptr;
 	HD_RET(hd_pool_create(&hd.cfg, &h));
 	struct Guard {
@@ -580,6 +637,9 @@ extern "C" int hd_pool_load(const char *path, hd_pool **out, uint32_t *roots, ui
 	meta[kMetaColorLeaves] = (u32)hd.color_leaf_words;
 	meta[kMetaOverflow] = hd.overflow;
 	HD_TRY(hipMemcpy(p.meta, meta, sizeof(meta), hipMemcpyHostToDevice));
+	// the saved pool holds the chain's nodes, so this finds them and appends nothing
+	if (hd.flags & kCkptFilled)
+		HD_RET(make_filled(p));
 	*n_roots = hd.n_roots;
 	*out = h;
 	guard.h = nullptr;

@@ -3,15 +3,42 @@
 //
 // The march keeps trace.frag's fp32 operation order (the library is built with
 // -ffp-contract=off and correctly rounded division / sqrt) so hits are bit-identical to the
-// CPU transliteration. Copyright (c) 2009-2011, NVIDIA Corporation, BSD-3 for the ESVO march
-// (the full notice is reproduced at trace.frag:25-50); after Laine & Karras, "Efficient Sparse
-// Voxel Octrees" (I3D 2010).
+// CPU transliteration. The ESVO march follows Laine & Karras, "Efficient Sparse Voxel Octrees"
+// (I3D 2010), as restated in trace.frag, which carries the BSD-3 notice reproduced below; it
+// applies to `march` in this file.
 //
 // Launch: one workgroup per pixel tile (8x8 or 16x16, from the tuning); the per-scale parent stack
 // lives in LDS laid out [scale][thread] so a wavefront's stack accesses hit 64 consecutive banks.
 // The march is bound by VALU issue (about 90 vector instructions per wave iteration, with the
 // descend / advance / pop paths serialised under divergence), not by memory: fewer instructions
 // per iteration and a shorter grid tail are what speed it up.
+/*
+ *  Copyright (c) 2009-2011, NVIDIA Corporation
+ *  All rights reserved.
+ *
+ *  Redistribution and use in source and binary forms, with or without
+ *  modification, are permitted provided that the following conditions are met:
+ *      * Redistributions of source code must retain the above copyright
+ *        notice, this list of conditions and the following disclaimer.
+ *      * Redistributions in binary form must reproduce the above copyright
+ *        notice, this list of conditions and the following disclaimer in the
+ *        documentation and/or other materials provided with the distribution.
+ *      * Neither the name of NVIDIA Corporation nor the
+ *        names of its contributors may be used to endorse or promote products
+ *        derived from this software without specific prior written permission.
+ *
+ *  THIS SOFTWARE IS PROVIDED BY THE COPYRIGHT HOLDERS AND CONTRIBUTORS "AS IS" AND
+ *  ANY EXPRESS OR IMPLIED WARRANTIES, INCLUDING, BUT NOT LIMITED TO, THE IMPLIED
+ *  WARRANTIES OF MERCHANTABILITY AND FITNESS FOR A PARTICULAR PURPOSE ARE
+ *  DISCLAIMED. IN NO EVENT SHALL <COPYRIGHT HOLDER> BE LIABLE FOR ANY
+ *  DIRECT, INDIRECT, INCIDENTAL, SPECIAL, EXEMPLARY, OR CONSEQUENTIAL DAMAGES
+ *  (INCLUDING, BUT NOT LIMITED TO, PROCUREMENT OF SUBSTITUTE GOODS OR SERVICES;
+ *  LOSS OF USE, DATA, OR PROFITS; OR BUSINESS INTERRUPTION) HOWEVER CAUSED AND
+ *  ON ANY THEORY OF LIABILITY, WHETHER IN CONTRACT, STRICT LIABILITY, OR TORT
+ *  (INCLUDING NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY OUT OF THE USE OF THIS
+ *  SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF SUCH DAMAGE.
+ */
+
 #include "hd_internal.hpp"
 
 #include <cmath>
