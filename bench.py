@@ -42,7 +42,7 @@ def main():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--mode", choices=["plain", "batched"], default="plain")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed runs of the CPU baseline (after a warm-up)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tuning", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--no-batched", action="store_true")
@@ -114,20 +114,27 @@ def main():
     if args.trace_only:
         return trace_only(args, pool, batches, render, npx)
 
-    edit_ms, trace_ms, step_ms = [], [], []
+    edit_ms, edit_gpu_ms, trace_ms, step_ms = [], [], [], []
+    state1 = None
     for s in range(args.warmup + args.steps):
         pool.sync()
         t0 = time.perf_counter()
+        pool.timer_begin()
         pool.edit(batches[s], mode)
+        ems = pool.timer_end()  # HIP events around the step's edit work on the pool's stream
         t1 = time.perf_counter()
         kms = render()
         pool.sync()
         t2 = time.perf_counter()
         if s >= args.warmup:
             edit_ms.append((t1 - t0) * 1e3)
+            edit_gpu_ms.append(ems)
             trace_ms.append(kms)
             step_ms.append((t2 - t0) * 1e3)
-        log(f"step {s}: edit {1e3 * (t1 - t0):.1f} ms ({vox[s] / 1e6:.1f} M voxels), trace kernel {kms:.3f} ms")
+        if s == 0 and state0 is not None:
+            state1 = pool.state()  # after the timed region of this step: the CPU leg's result is checked against it
+        log(f"step {s}: edit {1e3 * (t1 - t0):.1f} ms ({ems:.1f} ms on the device, {vox[s] / 1e6:.1f} M voxels), "
+            f"trace kernel {kms:.3f} ms")
 
     # the same frame with the beam prepass (beam.frag): coarse 8x8-block march, then the trace
     vb_ = capi.make_view(view_block, args.width, args.height, capi.TRACE_SHADOW | capi.TRACE_BEAM)
@@ -183,15 +190,20 @@ def main():
 
     cpu = None
     if state0 is not None:
-        cpu = cpu_baseline(cfg, heights, state0, batches[0], args.cpu_seconds, res)
-        if batched is not None:
-            b3 = cpu_baseline(cfg, heights, batched.pop("_state0"), batched.pop("_batch"), args.cpu_seconds, res)
-            batched["cpu_reference_voxels_per_s"] = b3["value"]
-            batched["cpu_reference_sample"] = b3["sample"]
-            batched["batched_vs_cpu"] = batched["batched_voxels_per_s"] / b3["value"]
+        cpu = cpu_baseline(cfg, heights, state0, batches[0], res, args.cpu_runs, state1,
+                           "the first batch of 256 sphere edits")
+        if cpu is not None:
+            cpu["gpu_plain_vs_cpu"] = value / cpu["value"]
+        if batched is not None and batched.get("_state0") is not None:
+            b3 = cpu_baseline(cfg, heights, batched.pop("_state0"), batched.pop("_batch"), res, args.cpu_runs,
+                              batched.pop("_state1"), "the 4096 mixed edits")
+            if b3 is not None:
+                batched["cpu_reference"] = b3
+                batched["cpu_reference_voxels_per_s"] = b3["value"]
+                batched["batched_vs_cpu"] = batched["batched_voxels_per_s"] / b3["value"]
     if batched is not None:
-        batched.pop("_state0", None)
-        batched.pop("_batch", None)
+        for k in ("_state0", "_batch", "_state1"):
+            batched.pop(k, None)
 
     out = {
         "metric": METRIC,
@@ -223,7 +235,8 @@ def main():
         "trace_ms": trace_avg_ms,
         "trace_beam_ms": beam_ms,
         "rays_per_frame": rays,
-        "roofline": {
+        "roofline": edit_roofline(args, edit_gpu_ms),
+        "roofline_trace": {
             "kernel": "k_trace (DAG_RayMarch + shadow + Color_Fetch)",
             "bound": "hbm",
             "achieved": achieved,
@@ -275,11 +288,21 @@ def batched_section(args, cfg, heights, res):
     out["_state0"] = None if args.no_cpu else p.state()
     out["_batch"] = batch
     t0 = time.perf_counter()
+    p.timer_begin()
     p.edit(batch, capi.MODE_BATCHED)
+    tb_gpu = p.timer_end() / 1e3
     tb = time.perf_counter() - t0
     st = p.stats()
-    out.update(batched_s=tb, batched_voxels_per_s=vox / tb, groups=int(st.last_groups),
+    out["_state1"] = None if args.no_cpu else p.state()
+    out.update(batched_s=tb, batched_device_s=tb_gpu, batched_voxels_per_s=vox / tb, groups=int(st.last_groups),
                overflow=int(st.overflow))
+    eb = edit_bytes_profile(args)
+    if eb is not None and "batched_bytes" in eb:
+        achieved = eb["batched_bytes"] / tb_gpu / 1e9
+        out["roofline"] = {"kernel": "batched edit passes (node levels + color stream)", "bound": "hbm",
+                           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": eb["batched_bytes"],
+                           "algorithmic_source": EDIT_BYTES_PROFILE}
     log(f"batched: {args.batch_edits} edits in {tb * 1e3:.1f} ms ({st.last_groups} passes), "
         f"{vox / tb / 1e9:.2f} G voxels/s")
     p.close()
@@ -371,6 +394,57 @@ def serve_run(args, scfg, heights, res, reqs, rate):
     return out
 
 
+EDIT_BYTES_PROFILE = "profiles/edit_algorithmic_bytes_4096_terrain.md"
+EDIT_TRAFFIC_PROFILE = "profiles/edit_plain_4096_terrain_256_spheres_hbm.md"
+
+
+def _profile_values(path):
+    """`key: value` lines of a committed profile summary (values parsed as JSON)."""
+    out = {}
+    try:
+        for line in open(os.path.join(ROOT, path)):
+            if ":" in line and not line.startswith(("#", "|", " ")):
+                k, v = line.split(":", 1)
+                try:
+                    out[k.strip()] = json.loads(v)
+                except ValueError:
+                    pass
+    except OSError:
+        return None
+    return out
+
+
+def edit_bytes_profile(args):
+    """B_edit of the benchmark's workloads, counted by the oracle's instrumented edit recursion
+    (oracle/edit_bytes.py wrote the table); only for the workload it was counted on."""
+    if (args.res, args.edits) != (4096, 256):
+        return None
+    return _profile_values(EDIT_BYTES_PROFILE)
+
+
+def edit_roofline(args, edit_gpu_ms):
+    """The plain edit path against HBM: B_edit of the timed steps (existing node words read, words
+    hashed, scanned in target buckets, appended, counter words) over their HIP-event-timed device
+    time. traffic = the step's HBM bytes from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs over every
+    edit kernel (scaled by a copy of known size), per 256-edit step."""
+    eb = edit_bytes_profile(args)
+    if eb is None or args.mode != "plain" or len(eb.get("plain_step_bytes", [])) < args.warmup + args.steps:
+        return None
+    b = eb["plain_step_bytes"][args.warmup:args.warmup + args.steps]
+    per_step_bytes = float(np.mean(b))
+    t = float(np.mean(edit_gpu_ms)) / 1e3
+    achieved = per_step_bytes / t / 1e9
+    tr = _profile_values(EDIT_TRAFFIC_PROFILE) or {}
+    return {"kernel": f"plain edit pass x {args.edits} per step (a_top, a_count/a_emit, a_leaf, a_build/a_dedup/"
+                      f"a_walk, a_bottom + the color stream)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": tr.get("traffic_bytes_per_step"),
+            "traffic_source": EDIT_TRAFFIC_PROFILE if tr.get("traffic_bytes_per_step") else None,
+            "algorithmic_bytes_per_step": per_step_bytes, "algorithmic_source": EDIT_BYTES_PROFILE,
+            "device_ms_per_step": t * 1e3,
+            "note": "latency-bound: ~10 dependent launches per edit, each a few memory round trips"}
+
+
 TRAFFIC_PROFILE = "profiles/k_trace_4096_terrain_1080p.md"
 
 
@@ -411,30 +485,66 @@ def C_ref(v):
     return ctypes.byref(v)
 
 
-def cpu_baseline(cfg, heights, state0, batch, budget_s, res):
-    """The reference's ThreadedEdit (libfork busy_pool over the host cores) from the same state."""
+def host_cpu():
+    """The host's CPU: model name, online CPUs, the CPUs this process may run on, and its share: the
+    GPU box sets OMP_NUM_THREADS to the cores one GPU's share holds (16), fewer than it may see."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS") or 0) or affinity
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity, "workers": min(affinity, share)}
+
+
+def cpu_baseline(cfg, heights, state0, batch, res, runs, check_state, what):
+    """The reference's NodePoolThreadedEdit::ThreadedEdit (oracle/_ref: the VkHashDAG headers compiled
+    with a libfork busy_pool over the host's cores, VBREditorWrapper for colored edits, max_task_level
+    = the color leaf level as at main.cpp:216) on the same edits from the device pool's exact state.
+    steady_clock around each ThreadedEdit call (main.cpp:152-157), summed over the edits; one warm-up
+    run, then the median of `runs` runs, each from the same state. The reference's final DAG is
+    checked against the device's (canonically: the threaded reference appends in a nondeterministic
+    order)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import _oracle as O  # the checker (oracle/_ref)
+    import _oracle as O  # the checker (oracle/_ref; oracle/liboracle for the canonical compare)
     from vkhashdag_amd import workloads as W
 
     if not O.have_ref():
         log("oracle/_ref not built: no reference CPU baseline")
         return None
-    workers = min(16, os.cpu_count() or 1)
+    cpu = host_cpu()
+    workers = cpu["workers"]
     ref = O.RefPool(cfg, heights=heights, workers=workers)
-    ref.load_state(state0)
-    done, secs = 0, 0.0
-    while done < len(batch) and secs < budget_s:
-        chunk = batch[done:done + 8]
-        secs += ref.edit_timed(chunk, threaded=True)
-        done += len(chunk)
-        log(f"cpu baseline: {done} edits in {secs:.1f}s")
-    vox = W.edited_voxels(batch[:done], res)
-    return {"value": vox / secs, "unit": "edited voxels/s", "cores": workers, "kind": "reference",
-            "sample": f"first {done} edits of the first batch ({vox / 1e6:.1f} M voxels), "
-                      f"NodePoolThreadedEdit::ThreadedEdit with VBREditorWrapper, libfork busy_pool({workers}), "
-                      f"max_task_level = color leaf level, from the device pool's state after the terrain build",
-            "seconds": secs}
+    try:
+        secs = []
+        for r in range(runs + 1):
+            ref.load_state(state0)
+            t = ref.edit_timed(batch, threaded=True)
+            if r:
+                secs.append(t)
+            log(f"cpu baseline ({what}): run {r}{' (warm-up)' if r == 0 else ''} {t * 1e3:.1f} ms")
+        same = None
+        if check_state is not None:
+            a, b = O.OraclePool(cfg), O.OraclePool(cfg)
+            a.load_state(ref.state())
+            b.load_state(check_state)
+            same = bool(np.array_equal(a.canonical(), b.canonical()))
+            log(f"cpu baseline ({what}): reference DAG {'equals' if same else 'DIFFERS FROM'} the device DAG")
+            del a, b
+    finally:
+        ref.close()  # joins the busy_pool's worker threads before the process exits
+    vox = W.edited_voxels(batch, res)
+    med = float(np.median(secs))
+    return {"value": vox / med, "unit": "edited voxels/s", "cores": workers, "kind": "reference",
+            "sample": f"{what} ({len(batch)} edits, {vox / 1e6:.1f} M voxels) from the device pool's state before "
+                      f"them: NodePoolThreadedEdit::ThreadedEdit with VBREditorWrapper, libfork busy_pool({workers}), "
+                      f"max_task_level = color leaf level; median of {runs} runs after a warm-up",
+            "seconds": med, "seconds_runs": secs, "cpu_model": cpu["model"], "nproc": cpu["nproc"],
+            "affinity_cpus": cpu["affinity"], "result_equals_device": same}
 
 
 if __name__ == "__main__":

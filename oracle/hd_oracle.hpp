@@ -38,6 +38,7 @@
 #include <span>
 #include <stdexcept>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 namespace hdo {
@@ -688,6 +689,62 @@ struct NodePool {
 	std::vector<u32> filled;
 	u64 overflows = 0; // upserts that fell back because the bucket was full
 
+	// Algorithmic node-pool traffic of edits, in words (the numerator of the edit path's roofline):
+	// words of existing nodes the recursion reads (edit_leaf / edit_node unpack), words hashed and
+	// words scanned in the target bucket by each upsert, words appended (incl. page padding) and one
+	// bucket-counter word per append. In batch mode an input node is counted once however many
+	// edits read it, and the appended nodes are those of the final DAG (see batch_traffic_end).
+	struct Traffic {
+		u64 visited = 0, hashed = 0, scanned = 0, written = 0, counters = 0, upserts = 0;
+		bool batch = false;
+		std::vector<u32> in_bw;         // batch mode: bucket counters of the input state
+		std::unordered_set<u32> seen;   // batch mode: input nodes already counted
+	};
+	Traffic traffic;
+	bool in_input(u32 ptr) const { // the node existed in the batch's input state
+		return (ptr & (cfg.words_per_bucket() - 1u)) < traffic.in_bw[ptr >> cfg.word_bits_per_bucket()];
+	}
+	void count_visit(u32 ptr, u32 words) {
+		if (ptr == kNull)
+			return;
+		if (traffic.batch && (!in_input(ptr) || !traffic.seen.insert(ptr).second))
+			return;
+		traffic.visited += words;
+	}
+	void traffic_begin(bool batch) {
+		traffic = Traffic{};
+		traffic.batch = batch;
+		if (batch)
+			traffic.in_bw = bucket_words;
+	}
+	// Batch mode: every node of the final DAG that is not in the input was appended once by the
+	// batch: its words hashed, the bucket's input words scanned (a miss reads them all), its words
+	// written and the counter bumped. The per-edit upsert counts of the sequential replay are dropped.
+	void batch_traffic_end(u32 root) {
+		traffic.hashed = traffic.scanned = traffic.written = traffic.counters = traffic.upserts = 0;
+		std::unordered_set<u32> done;
+		const u32 L = cfg.node_levels();
+		std::vector<std::pair<u32, u32>> stack;
+		if (root != kNull)
+			stack.push_back({root, 0});
+		while (!stack.empty()) {
+			auto [ptr, l] = stack.back();
+			stack.pop_back();
+			if (in_input(ptr) || !done.insert(ptr).second)
+				continue;
+			const u32 *p = read_node(ptr);
+			const u32 sz = l + 1 == L ? 2u : inner_words(p[0]);
+			traffic.hashed += sz;
+			traffic.scanned += traffic.in_bw[ptr >> cfg.word_bits_per_bucket()];
+			traffic.written += sz;
+			traffic.counters += 1;
+			traffic.upserts += 1;
+			if (l + 1 < L)
+				for (u32 k = 1; k < sz; ++k)
+					stack.push_back({p[k], l + 1});
+		}
+	}
+
 	explicit NodePool(Config c) : cfg(std::move(c)) {
 		if (!Config::validate(cfg))
 			throw std::invalid_argument("invalid config");
@@ -713,20 +770,23 @@ struct NodePool {
 	}
 
 	// find_node_in_span (NodePool.hpp:79-91)
-	static u32 find_in_span(bool leaf, u32 base, const u32 *w, u32 n, const u32 *node, u32 nsz) {
+	static u32 find_in_span(bool leaf, u32 base, const u32 *w, u32 n, const u32 *node, u32 nsz, u64 *scanned) {
 		u32 i = 0;
 		while (nsz <= n - i) {
 			u32 sz = leaf ? 2u : inner_words(w[i]);
 			if (sz == 0)
 				break;
-			if (sz == nsz && std::equal(node, node + nsz, w + i))
+			if (sz == nsz && std::equal(node, node + nsz, w + i)) {
+				*scanned += i + nsz;
 				return base + i;
+			}
 			i += sz;
 		}
+		*scanned += i;
 		return kNull;
 	}
 	// find_node (NodePool.hpp:93-132)
-	u32 find(bool leaf, u32 bucket, u32 bwords, u32 boff, const u32 *node, u32 nsz) const {
+	u32 find(bool leaf, u32 bucket, u32 bwords, u32 boff, const u32 *node, u32 nsz, u64 *scanned) const {
 		if (bwords <= boff)
 			return kNull;
 		u32 find_words = bwords - boff;
@@ -734,7 +794,7 @@ struct NodePool {
 		u32 poff = boff & (cfg.words_per_page() - 1u);
 		while (poff + find_words >= cfg.words_per_page()) {
 			u32 r = find_in_span(leaf, (pidx << cfg.wbpp) | poff, read_page(pidx) + poff, cfg.words_per_page() - poff,
-			                     node, nsz);
+			                     node, nsz, scanned);
 			if (r != kNull)
 				return r;
 			find_words -= cfg.words_per_page() - poff;
@@ -742,7 +802,8 @@ struct NodePool {
 			++pidx;
 		}
 		if (find_words) {
-			u32 r = find_in_span(leaf, (pidx << cfg.wbpp) | poff, read_page(pidx) + poff, find_words, node, nsz);
+			u32 r = find_in_span(leaf, (pidx << cfg.wbpp) | poff, read_page(pidx) + poff, find_words, node, nsz,
+			                     scanned);
 			if (r != kNull)
 				return r;
 		}
@@ -772,12 +833,16 @@ struct NodePool {
 	u32 upsert(u32 level, bool leaf, const u32 *node, u32 nsz, u32 fallback) {
 		u32 b = bucket_of(level, leaf, node, nsz);
 		u32 bw = bucket_words[b];
-		u32 f = find(leaf, b, bw, 0, node, nsz);
+		traffic.hashed += nsz;
+		traffic.upserts += 1;
+		u32 f = find(leaf, b, bw, 0, node, nsz, &traffic.scanned);
 		if (f != kNull)
 			return f;
 		auto [p, nw] = append(b, bw, node, nsz);
 		if (p != kNull) {
 			bucket_words[b] = nw;
+			traffic.written += nw - bw;
+			traffic.counters += 1;
 			return p;
 		}
 		++overflows;
@@ -905,6 +970,7 @@ struct NodePool {
 			leaf[0] = p[0];
 			leaf[1] = p[1];
 		}
+		count_visit(ptr, 2);
 		bool changed = false;
 		for (u32 i = 0; i < 64; ++i) {
 			bool v = (leaf[i >> 5] >> (i & 31u)) & 1u;
@@ -926,6 +992,7 @@ struct NodePool {
 		if (level == cfg.node_levels() - 1)
 			return edit_leaf(x, ptr, level, pos, st);
 		std::array<u32, 9> u = unpack(ptr);
+		count_visit(ptr, inner_words(u[0]));
 		std::array<State, 8> cs{};
 		bool changed = false;
 		for (u32 i = 0; i < 8; ++i) {

@@ -375,6 +375,20 @@ void hdo_render_pool(void *p, u32 root, u32 color_root, const float *view, u32 W
 	             rgba, rec, threads);
 }
 
+// Edit traffic counting (NodePool::Traffic): begin, apply edits with hdo_edit, end. out = words
+// [visited, hashed, scanned, written, counters, upserts]; batch != 0 counts the edits between
+// begin and end as one batch (input nodes once, appended nodes = the new nodes of the final DAG
+// below root).
+void hdo_traffic_begin(void *p, u32 batch) { static_cast<OPool *>(p)->pool.traffic_begin(batch != 0); }
+void hdo_traffic_end(void *p, u32 root, u64 *out) {
+	auto &pool = static_cast<OPool *>(p)->pool;
+	if (pool.traffic.batch)
+		pool.batch_traffic_end(root);
+	const auto &t = pool.traffic;
+	out[0] = t.visited, out[1] = t.hashed, out[2] = t.scanned, out[3] = t.written, out[4] = t.counters;
+	out[5] = t.upserts;
+}
+
 // Used words of every bucket packed in bucket order (hd_pool_read_live_words' layout); returns the count.
 u64 hdo_live_words(void *p, u32 *out, u64 cap) {
 	auto *o = static_cast<OPool *>(p);

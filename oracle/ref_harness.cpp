@@ -331,6 +331,18 @@ void hdr_bucket_words(void *p, u32 *out) {
 	auto *r = static_cast<Ref *>(p);
 	std::copy(r->pool.m_bucket_words.get(), r->pool.m_bucket_words.get() + r->pool.GetConfig().GetTotalBuckets(), out);
 }
+// Used words of every bucket packed in bucket order; returns the count (out may be null).
+u64 hdr_live_words(void *p, u32 *out, u64 cap) {
+	auto *r = static_cast<Ref *>(p);
+	const auto &cfg = r->pool.GetConfig();
+	const u64 nb = cfg.GetTotalBuckets(), wbpb = (u64)cfg.word_bits_per_page + cfg.page_bits_per_bucket;
+	u64 at = 0;
+	for (u64 b = 0; b < nb; ++b)
+		for (u32 i = 0; i < r->pool.m_bucket_words[b]; ++i, ++at)
+			if (out && at < cap)
+				out[at] = r->pool.word((b << wbpb) + i);
+	return at;
+}
 u32 hdr_filled(void *p, u32 *out) {
 	auto *r = static_cast<Ref *>(p);
 	for (size_t i = 0; i < r->pool.m_filled_node_pointers.size(); ++i)

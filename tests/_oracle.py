@@ -82,6 +82,8 @@ def olib():
         _sig(L, "hdo_render_pool", None, [VP, U32, U32, f32p, U32, U32, U32, u32p, VP, U32])
         _sig(L, "hdo_live_words", U64, [VP, VP, U64])
         _sig(L, "hdo_voxel_query", None, [VP, U32, u32p, U64, u8p])
+        _sig(L, "hdo_traffic_begin", None, [VP, U32])
+        _sig(L, "hdo_traffic_end", None, [VP, U32, u64p])
         _olib = L
     return _olib
 
@@ -112,6 +114,7 @@ def rlib():
         _sig(L, "hdr_hash_leaf", U32, [U32, U32])
         _sig(L, "hdr_vbr_write", U64, [VP, U32, u32p, U32, u32p, U64])
         _sig(L, "hdr_load_live", None, [VP, u32p, u32p, U64, u32p, U32, u32p, U64, u32p, U64])
+        _sig(L, "hdr_live_words", U64, [VP, VP, U64])
         _rlib = L
     return _rlib
 
@@ -240,6 +243,18 @@ class OraclePool(_PoolBase):
                 return out[:n]
             cap = n
 
+    TRAFFIC_FIELDS = ("visited", "hashed", "scanned", "written", "counters", "upserts")
+
+    def edit_traffic(self, e, batch=False):
+        """Applies edits like edit() and returns their algorithmic node-pool traffic in words
+        (NodePool::Traffic): per-edit sums, or batch counting (input nodes once, appended nodes
+        = the new nodes of the final DAG)."""
+        olib().hdo_traffic_begin(self.h, int(batch))
+        self.edit(e)
+        out = np.zeros(6, np.uint64)
+        olib().hdo_traffic_end(self.h, self.root, out)
+        return {k: int(v) for k, v in zip(self.TRAFFIC_FIELDS, out)}
+
     def voxel_query(self, vox, root=None):
         """Occupancy (0/1) of voxels (x, y, z) of the DAG below root."""
         vox = np.ascontiguousarray(vox, np.uint32).reshape(-1, 3)
@@ -293,10 +308,13 @@ class RefPool(_PoolBase):
         if heights is not None:
             L.hdr_set_heightmap(self.h, np.ascontiguousarray(heights, np.uint32), heights.shape[0])
 
-    def __del__(self):
+    def close(self):
+        """Destroys the pool and joins its worker threads."""
         if getattr(self, "h", None):
             rlib().hdr_pool_destroy(self.h)
             self.h = None
+
+    __del__ = close
 
     def edit(self, e, threaded=False, max_task_level=None):
         e = _edits(e)
@@ -346,6 +364,18 @@ class RefPool(_PoolBase):
         out = np.zeros(max(1, n), np.uint32)
         rlib().hdr_color_leaves(self.h, out)
         return out[:n]
+
+    def live_words(self):
+        n = int(rlib().hdr_live_words(self.h, None, 0))
+        out = np.zeros(max(1, n), np.uint32)
+        rlib().hdr_live_words(self.h, out.ctypes.data, n)
+        return self.bucket_words(), out[:n]
+
+    def state(self):
+        """Everything a pool needs to continue from this state (DevicePool.state()'s layout)."""
+        bw, live = self.live_words()
+        return dict(bucket_words=bw, live=live, filled=self.filled(), color_nodes=self.color_nodes(),
+                    color_leaves=self.color_leaves(), root=self.root, color_root=self.color_root)
 
     def traversal(self, o, d):
         o = np.ascontiguousarray(o, np.float32)

@@ -53,3 +53,26 @@ def test_profiled_valu_issue_matches_the_counters():
     assert f is not None and 0.0 < f <= 1.0
     assert abs(valu / cap - f) < 0.01
     assert bench.profiled_valu_issue(types.SimpleNamespace(res=1024, width=1920, height=1080)) is None
+
+
+def test_edit_roofline_from_the_algorithmic_bytes_table():
+    # the edit path's roofline numerator comes from the committed oracle count of the benchmark's
+    # steps (oracle/edit_bytes.py); the table's byte column equals 4 x the sum of its word columns
+    import bench
+    eb = bench._profile_values(bench.EDIT_BYTES_PROFILE)
+    steps = eb["plain_step_bytes"]
+    assert len(steps) >= 25 and all(b > 0 for b in steps)
+    text = open(os.path.join(ROOT, bench.EDIT_BYTES_PROFILE)).read()
+    rows = re.findall(r"^\| (\d+) \| (\d+) \| (\d+) \| (\d+) \| (\d+) \| (\d+) \| (\d+) \| (\d+) \| (\d+) \|$", text, re.M)
+    assert len(rows) == len(steps)
+    for r in rows:
+        v = [int(x) for x in r]
+        assert v[8] == 4 * (v[2] + v[3] + v[4] + v[5] + v[6]) == steps[v[0]]
+        assert v[1] == eb["plain_step_voxels"][v[0]]
+    args = types.SimpleNamespace(res=4096, edits=256, mode="plain", warmup=1, steps=3)
+    r = bench.edit_roofline(args, [40.0, 40.0, 40.0])
+    assert r["bound"] == "hbm" and r["unit"] == "GB/s"
+    assert abs(r["achieved"] - sum(steps[1:4]) / 3 / 0.040 / 1e9) < 1e-6
+    assert abs(r["frac"] - r["achieved"] / 8000.0) < 1e-12
+    assert bench.edit_roofline(types.SimpleNamespace(res=1024, edits=256, mode="plain", warmup=1, steps=3),
+                               [1.0]) is None

@@ -626,3 +626,35 @@ def test_voxel_query_and_pool_render_equal_array_checkers():
     parts = [p.read_words(int(i) * cfg.words_per_bucket, int(bw[i])) for i in np.nonzero(bw)[0]]
     assert np.array_equal(p.live_words()[1], np.concatenate(parts))
     assert np.array_equal(p.canonical(), O.canonical_words(words, cfg.node_levels, np.array([p.root], np.uint32)))
+
+
+def test_edit_traffic_known_answers():
+    # Closed forms on single-voxel edits (L node levels, the filled chain created beforehand):
+    #  * one voxel into an empty pool: no input node is read; one new node per level (the 2-word
+    #    leaf and L - 1 one-child inner nodes of 2 words), each hashed and written once, one
+    #    counter word each;
+    #  * the same edit again: the recursion reads the 2-word path node at every level and changes
+    #    nothing (no upsert);
+    #  * a second voxel of the same leaf as a batch of one: the path is read once and L new nodes
+    #    appended (leaf 2 words, inner nodes still one child).
+    cfg = small_cfg(64, 6, 5, 2, 3)
+    L = cfg.node_levels
+    p = O.OraclePool(cfg)
+    O.olib().hdo_make_filled(p.h)
+    one = W.edit(W.AABB_FILL, (9, 9, 9), (10, 10, 10))
+    t = p.edit_traffic(one)
+    assert (t["visited"], t["hashed"], t["written"], t["counters"], t["upserts"]) == (0, 2 * L, 2 * L, L, L)
+    assert t["scanned"] <= 9 * L  # only the filled chain's nodes can share a bucket
+    t = p.edit_traffic(one)
+    assert (t["visited"], t["hashed"], t["scanned"], t["written"], t["counters"], t["upserts"]) == (2 * L, 0, 0, 0, 0, 0)
+    t = p.edit_traffic(W.edit(W.AABB_FILL, (8, 9, 9), (9, 10, 10)), batch=True)
+    assert (t["visited"], t["hashed"], t["written"], t["counters"], t["upserts"]) == (2 * L, 2 * L, 2 * L, L, L)
+    # batch counting never exceeds the per-edit sums of the same edits from the same state
+    q1, q2 = O.OraclePool(cfg), O.OraclePool(cfg)
+    base = random_edits(6, 64, 3, True)
+    q1.edit(base)
+    q2.edit(base)
+    more = random_edits(12, 64, 4, True)
+    a, b = q1.edit_traffic(more), q2.edit_traffic(more, batch=True)
+    assert q1.root == q2.root
+    assert b["visited"] <= a["visited"] and b["written"] <= a["written"] and b["upserts"] <= a["upserts"]

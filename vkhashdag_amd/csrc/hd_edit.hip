@@ -3729,6 +3729,24 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 					for (i64 x = c0[0]; x <= c1[0]; ++x)
 						grid[((size_t)z * cells + y) * cells + x].push_back(k);
 	}
+	// A weighted (two R5G6B5 endpoints + weight) color that an edit fills at or above the color
+	// leaves is stored RGB8 by the color octree (DAGColorPool::FillNode), while a pass's per-voxel
+	// fold keeps it exact; the two agree only where the edit's own wrapper logic decides the node.
+	// So an edit with weighted colors gets a group of its own, after its group's other edits (they
+	// are voxel-disjoint from it, so they commute): every node it reaches is then decided exactly
+	// as NodePoolBase::Edit decides it. RGB8 colors are stored exactly either way.
+	std::vector<std::vector<u32>> split;
+	split.reserve(groups.size());
+	for (auto &g : groups) {
+		std::vector<u32> rest, solo;
+		for (u32 k : g)
+			(is_stateful(edits[k]) && edits[k].color.has_value && edits[k].color.bits_per_weight ? solo : rest).push_back(k);
+		if (!rest.empty())
+			split.push_back(std::move(rest));
+		for (u32 k : solo)
+			split.push_back({k});
+	}
+	groups.swap(split);
 }
 
 // ---------------------------------------------------------------- sync-free pass: host side
@@ -4043,6 +4061,11 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		a_emit<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.bsum);
 	}
 	const bool colors = pp.stateful && pp.cap[leafL] > 0;
+	// A colored pass that proceeds only above the color leaves (its fills and clears land at or above
+	// them) has no color leaf job, yet the octree above the leaves may change (a fill color at a
+	// proceeding node, cleared children): SetNode alone, once the top-down has classified them.
+	if (pp.stateful && !colors && leafL > 0)
+		a_setnode<<<1, kSetNodeThreads, 0, p.stream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
 	const bool ljobs = lds_jobs(p.dc);
 	if (colors && !ljobs) { // the jobs' dense colors on the color stream, beside the leaves
 		HD_TRY(hipEventRecord(p.ev_top, p.stream));
