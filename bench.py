@@ -58,6 +58,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=3000)
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="aggregate request arrivals per second")
     ap.add_argument("--serve-overload", type=float, default=10.0, help="rate multiple of the overload run")
+    ap.add_argument("--no-config4", action="store_true", help="skip the 16384^3 multi-view frame section")
+    ap.add_argument("--c4-frames", type=int, default=8, help="timed frames of the 16384^3 section")
+    ap.add_argument("--no-config5", action="store_true", help="skip the 32768^3 serving section")
+    ap.add_argument("--c5-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is designed, not built",
@@ -187,6 +191,8 @@ def main():
 
     batched = None if args.no_batched else batched_section(args, cfg, heights, res)
     serving = None if args.no_serve else serving_section(args, cfg, heights, res)
+    config4 = None if args.no_config4 else config4_section(args)
+    config5 = None if args.no_config5 else config5_section(args)
 
     cpu = None
     if state0 is not None:
@@ -256,6 +262,8 @@ def main():
         "gc_used_words": [int(used_before), int(used_after)],
         "batched": batched,
         "serving": serving,
+        "multiview_16384": config4,
+        "serving_32768": config5,
         "pool": {"used_words": int(st.used_words), "color_nodes": int(st.color_nodes),
                  "color_leaf_words": int(st.color_leaf_words), "overflow": int(st.overflow)},
     }
@@ -321,6 +329,154 @@ def batched_section(args, cfg, heights, res):
                plain_voxels_per_s=vox_prefix / tp, batched_prefix_voxels_per_s=vox_prefix / tbp,
                batched_vs_plain=tp / tbp)
     log(f"prefix {len(prefix)} edits: plain {tp * 1e3:.1f} ms, batched {tbp * 1e3:.1f} ms -> {tp / tbp:.1f}x")
+    return out
+
+
+def pool_bytes(cfg, st):
+    """Node-pool bytes per level (used words x 4 B) against the pool's capacity and 288 GB of HBM."""
+    from vkhashdag_amd import capi  # noqa: F401
+    hbm = 288e9
+    levels = []
+    for l, bits in enumerate(cfg.bucket_bits):
+        used = int(st.level_used_words[l]) * 4
+        cap = (1 << bits) * cfg.words_per_bucket * 4
+        levels.append({"level": l, "used_bytes": used, "capacity_bytes": cap, "used_of_capacity": used / cap})
+    used = sum(x["used_bytes"] for x in levels)
+    reserved = cfg.total_words * 4 + cfg.color_node_capacity * 32 + cfg.color_leaf_capacity * 4
+    return {"levels": levels, "node_used_bytes": used, "color_leaf_bytes": int(st.color_leaf_words) * 4,
+            "color_node_bytes": int(st.color_nodes) * 32, "reserved_bytes": reserved,
+            "used_of_hbm": (used + int(st.color_leaf_words) * 4 + int(st.color_nodes) * 32) / hbm,
+            "reserved_of_hbm": reserved / hbm}
+
+
+def config4_section(args):
+    """A 16384^3 terrain (the same value-noise recipe, 8 octaves, seed 3, z in [2048, 12288]) with VBR
+    colors, generated and built on the device. Each frame is one hipGraph: up to 4 passes of small
+    mixed edits (radius <= 64), a GC of the new root, then 16 views at uneven resolutions between
+    640x360 and 1920x1080 (primary + 1 shadow ray per hit) in one launch. Launch parameters come from
+    the autotuner's stored set. Node-pool bytes per level are reported against 288 GB."""
+    from vkhashdag_amd import capi
+    from vkhashdag_amd import workloads as W
+
+    res = 16384
+    cfg = W.config_for_resolution(res, bucket_bits=16, wbpp=9, pbpb=2, color_leaf_depth=4,
+                                  color_node_capacity=1 << 26, color_leaf_capacity=1 << 30)
+    pool = capi.DevicePool(cfg)
+    t0 = time.perf_counter()
+    heights = pool.value_noise_heightmap(8, 3, res // 8, (res * 3) // 4)
+    t_hm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pool.edit(W.terrain_edit(res, seed=3, skin=8))
+    pool.sync()
+    t_build = time.perf_counter() - t0
+    st = pool.stats()
+    log(f"16384^3 terrain: heightmap {t_hm:.2f}s, build {t_build:.2f}s, {st.used_words} words, "
+        f"{st.color_leaf_words} color leaf words, overflow {st.overflow}")
+    rng = np.random.default_rng(3)
+    views = []
+    for k in range(16):
+        w, h = int(rng.integers(640, 1921)), int(rng.integers(360, 1081))
+        vb = W.orbit_view((res / 2, res / 2, res * 0.3), 0.9 * res, 100 + k, w, h, res=res, elevation=(0.35, 0.6))
+        views.append((vb, w, h, capi.TRACE_SHADOW))
+    pixels = sum(w * h for _, w, h, _ in views)
+    out = {"workload": "16384^3 value-noise terrain (8 octaves, seed 3, z in [2048,12288], VBR colors), frames of "
+                       "24 mixed edits (r log-uniform 2..64) -> GC -> 16 views (640x360..1920x1080, primary + 1 "
+                       "shadow ray per hit), each frame one hipGraph",
+           "heightmap_s": t_hm, "terrain_build_s": t_build, "views": [[w, h] for _, w, h, _ in views],
+           "pixels_per_frame": pixels, "pool_after_build": pool_bytes(cfg, st)}
+    frame = capi.Frame(pool, max_edits=32, max_passes=4, max_radius=64, n_views=16, max_width=1920,
+                       max_height=1080, colors=True, gc=True)
+    ms, wall = [], []
+    n_frames = max(1, args.c4_frames)
+    for f in range(n_frames + 1):
+        e = W.mixed_batch(24, res, heights, 2, 64, seed=500 + f)
+        frame.set(e, views)
+        t0 = time.perf_counter()
+        pool.timer_begin()
+        frame.launch()
+        frame.wait()
+        fm = pool.timer_end()
+        if f:  # the first replay is a warm-up
+            ms.append(fm)
+            wall.append((time.perf_counter() - t0) * 1e3)
+    frame.close()
+    st = pool.stats()
+    # rays of one frame's views on the final DAG (untimed, instrumented trace)
+    vs = (capi.hd_view * len(views))()
+    off = 0
+    for i, (vb, w, h, fl) in enumerate(views):
+        vs[i] = capi.make_view(vb, w, h, fl, off)
+        off += w * h
+    tiles = np.zeros(len(views) + 1, np.uint32)
+    capi.check(capi.lib().hd_trace_tiles(vs, len(views), tiles.ctypes.data))
+    dv, dt = capi.DeviceBuffer(72 * len(views)), capi.DeviceBuffer(4 * len(tiles))
+    dv.upload(np.frombuffer(bytes(vs), np.uint8))
+    dt.upload(tiles)
+    rgba, counters = capi.DeviceBuffer(4 * pixels), capi.DeviceBuffer(32)
+    counters.upload(np.zeros(4, np.uint64))
+    capi.check(capi.lib().hd_trace_async(pool.h, pool.root, pool.color_root, dv.ptr, dt.ptr, len(views),
+                                         int(tiles[-1]), rgba.ptr, None, counters.ptr))
+    c = counters.download(np.uint64, 4)
+    pool.timer_begin()
+    capi.check(capi.lib().hd_trace_async(pool.h, pool.root, pool.color_root, dv.ptr, dt.ptr, len(views),
+                                         int(tiles[-1]), rgba.ptr, None, None))
+    trace_ms = pool.timer_end()
+    rays = int(c[0] + c[1])
+    out.update(frame_ms=float(np.median(ms)), frame_ms_all=ms, frame_wall_ms=float(np.median(wall)),
+               trace_only_ms=trace_ms, rays_per_frame=rays, rays_per_s_trace=rays / (trace_ms / 1e3),
+               pool_after_frames=pool_bytes(cfg, st), overflow=int(st.overflow))
+    log(f"16384^3 frames: {np.median(ms):.2f} ms per edit->GC->render frame ({rays / 1e6:.1f} M rays; the trace "
+        f"alone {trace_ms:.2f} ms = {rays / trace_ms / 1e6:.2f} G rays/s)")
+    pool.close()
+    return out
+
+
+def config5_section(args):
+    """Serving on a 32768^3 terrain (the value-noise recipe, 9 octaves, seed 4, z in [4096, 24576]):
+    64 clients, Poisson arrivals at 2000 req/s aggregate; 70% edits (40% sphere add, 40% sphere
+    remove, 20% box paint, radii log-uniform in [2, 512], colored), 30% renders at uniform-random
+    resolutions 256x256..1920x1080 from random orbit cameras; continuous batching vs one request at a
+    time. The deepest levels get more buckets (within the 2^32-word pointer space) and the terrain
+    carries no colors: its per-block colors would need about 1.2 G color-leaf words at this size, past
+    the 30-bit color pointers (DAGColorPool.hpp:24); the edits' colors are stored."""
+    from vkhashdag_amd import capi
+    from vkhashdag_amd import workloads as W
+
+    res = 32768
+    bits = [16] * 10 + [18, 19, 19, 16]
+    cfg = W.PoolConfig(9, 2, bits, 15 - 4, keep_history=True, color_node_capacity=1 << 24,
+                       color_leaf_capacity=1 << 28)
+    out = {"workload": f"{args.c5_requests} requests from 64 clients, Poisson at 2000/s aggregate: 70% edits "
+                       f"(r log-uniform 2..512, colored) and 30% renders (256x256..1920x1080, primary + shadow) on "
+                       f"a 32768^3 value-noise terrain (9 octaves, seed 4, z in [4096,24576]; terrain uncolored)",
+           "bucket_bits": bits}
+    heights = None
+    for name, plain in (("continuous", False), ("plain", True)):
+        pool = capi.DevicePool(cfg)
+        t0 = time.perf_counter()
+        h = pool.value_noise_heightmap(9, 4, res // 8, (res * 3) // 4, download=heights is None)
+        heights = h if heights is None else heights
+        pool.edit(W.edit(W.COLUMN_FILL, (0, 0, 0), (res, res, 0), r2=4, colored=False, skin=8))
+        pool.sync()
+        out.setdefault("terrain_build_s", time.perf_counter() - t0)
+        if "pool_after_build" not in out:
+            out["pool_after_build"] = pool_bytes(cfg, pool.stats())
+            reqs = W.serving_trace(args.c5_requests, res, heights, clients=64, rate_per_s=2000.0, seed=4, rmin=2,
+                                   rmax=512)
+            out["edited_voxels"] = W.edited_voxels(reqs[reqs["kind"] == W.REQ_EDIT]["edit"], res)
+        r, st, _ = pool.serve(reqs, max_edits_per_tick=256, max_pixels_per_tick=1 << 23, min_free_pages=1 << 16,
+                              plain=plain)
+        lat = r["done_ms"] - reqs["arrival_ms"]
+        le, lr = lat[reqs["kind"] == W.REQ_EDIT], lat[reqs["kind"] == W.REQ_RENDER]
+        wall = st.wall_ms / 1e3
+        out[name] = {"wall_s": wall, "requests_per_s": len(reqs) / wall, "ticks": int(st.ticks), "gcs": int(st.gcs),
+                     "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                    "edit_p99": float(np.percentile(le, 99)),
+                                    "render_p99": float(np.percentile(lr, 99))},
+                     "overflow": int(pool.stats().overflow)}
+        log(f"32768^3 serve {name}: {len(reqs)} requests in {wall:.2f}s, p50 {out[name]['latency_ms']['p50']:.2f} "
+            f"ms p99 {out[name]['latency_ms']['p99']:.2f} ms")
+        pool.close()
     return out
 
 

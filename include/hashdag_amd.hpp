@@ -13,20 +13,29 @@
 //   DAGNodePool::Create / SetRoot / GetRoot (src/DAGNodePool.hpp:41-99) and DAGColorPool::Config
 //   (src/DAGColorPool.hpp:36-41)                              -> DeviceNodePool::Create, ColorConfig
 //
+//   NodeCoord (NodeCoord.hpp:14-94), StatelessEditor (Editor.hpp:36-40) -> hashdag_amd::NodeCoord,
+//   HostStatelessEditor: any functor with the reference's EditNode / EditVoxel runs on the host
+//   (hd_edit_host) and composes with device edits on the same pool
+//   VBREditorWrapper / StatelessEditorWrapper (VBREditor.hpp:26-107, Editor.hpp:42-55)
+//                                                             -> the same names, over descriptors
+//
 // Error behaviour follows the reference (built with -fno-exceptions): Create returns nullptr, a
-// failed edit returns the root it was given, GC leaves the roots unchanged, Traversal returns
-// nullopt; GetLastError() explains the last failure. Editors are plain descriptors of the closed
-// set the device runs (the integer predicates of src/main.cpp); an arbitrary CPU functor editor has
-// no device counterpart.
+// failed edit returns the root it was given, GC returns the roots unchanged, Traversal returns
+// nullopt; GetLastError() explains the last failure. Device editors are plain descriptors of the
+// closed set the device runs (the integer predicates of src/main.cpp); any other stateless functor
+// editor runs on the host. The busy_pool and max_task_level arguments of the reference's threaded
+// calls are accepted and unused: the device schedules its own parallelism.
 #pragma once
 
 #include "hd_capi.h"
 
 #include <array>
+#include <concepts>
 #include <cstdint>
 #include <memory>
 #include <optional>
 #include <span>
+#include <variant>
 #include <vector>
 
 namespace hashdag_amd {
@@ -106,6 +115,31 @@ public:
 };
 
 enum class EditType { kNotAffected = 0, kProceed, kFill, kClear };
+static_assert((int)EditType::kProceed == HD_EDIT_TYPE_PROCEED && (int)EditType::kFill == HD_EDIT_TYPE_FILL &&
+              (int)EditType::kClear == HD_EDIT_TYPE_CLEAR);
+
+// hashdag::NodeCoord (NodeCoord.hpp:14-94): a node at `level`, pos in nodes of that level. The member
+// names match the reference's (pos.x / .y / .z), so its editors' integer code compiles unchanged.
+struct NodeCoord {
+	struct Vec3 {
+		Word x = 0, y = 0, z = 0;
+	};
+	Word level = 0;
+	Vec3 pos{};
+	inline constexpr NodeCoord GetChildCoord(Word child_index) const {
+		return {level + 1, {(pos.x << 1u) | (child_index & 1u), (pos.y << 1u) | ((child_index >> 1u) & 1u),
+		                    (pos.z << 1u) | ((child_index >> 2u) & 1u)}};
+	}
+	inline constexpr Word GetChildIndex() const { return (pos.x & 1u) | ((pos.y & 1u) << 1u) | ((pos.z & 1u) << 2u); }
+	inline constexpr Vec3 GetLowerBoundAtLevel(Word at_level) const {
+		const Word b = at_level - level;
+		return {pos.x << b, pos.y << b, pos.z << b};
+	}
+	inline constexpr Vec3 GetUpperBoundAtLevel(Word at_level) const {
+		const Word b = at_level - level;
+		return {(pos.x + 1u) << b, (pos.y + 1u) << b, (pos.z + 1u) << b};
+	}
+};
 
 // hashdag::VBRColor: RGB8 when bits_per_weight == 0, else two RGB565 endpoints + a weight.
 struct VBRColor {
@@ -183,6 +217,43 @@ concept DeviceEditor = requires(const T &e) {
 	{ e.ToDesc() } -> std::same_as<hd_edit_desc>;
 };
 
+// hashdag::StatelessEditor (Editor.hpp:36-40) over this facade's Config / NodeCoord / NodePointer:
+// any such functor that is not a device descriptor runs on the host (hd_edit_host).
+template <typename T>
+concept HostStatelessEditor = !DeviceEditor<T> && requires(const T &e) {
+	{ e.EditNode(std::declval<const Config &>(), NodeCoord{}, NodePointer{}) } -> std::convertible_to<EditType>;
+	{ e.EditVoxel(std::declval<const Config &>(), NodeCoord{}, bool{}) } -> std::convertible_to<bool>;
+};
+
+// VBREditorWrapper::NodeState after an edit (VBREditor.hpp:28-32): octree_node is the new color
+// octree root (a DAGColorPool pointer word). Stateless edits hand over std::monostate, as the
+// reference's StatelessEditorWrapper does (Editor.hpp:45).
+struct VBRNodeState {
+	Word octree_node = HD_COLOR_NULL;
+};
+
+// hashdag::VBREditorWrapper / StatelessEditorWrapper (VBREditor.hpp:26-107, Editor.hpp:42-55): the
+// reference's call sites wrap editors like this; here they select colored or stateless device
+// passes. p_octree is the pool that holds the color octree (the device pool itself); octree_root is
+// the color root the edit starts from.
+template <typename Word_T, DeviceEditor Editor_T, typename Octree_T> struct VBREditorWrapper {
+	Editor_T editor;
+	Octree_T *p_octree = nullptr;
+	Word_T octree_root = HD_COLOR_NULL;
+	using NodeState = VBRNodeState;
+	inline hd_edit_desc ToDesc() const {
+		hd_edit_desc d = editor.ToDesc();
+		d.colored = d.kind == HD_EDIT_SPHERE_DIG || d.kind == HD_EDIT_AABB_DIG ? 0u : 1u;
+		return d;
+	}
+};
+template <typename Word_T, typename Editor_T> struct StatelessEditorWrapper {
+	Editor_T editor;
+	using NodeState = std::monostate;
+};
+template <typename T>
+concept VBRWrapped = requires { typename T::NodeState; } && std::same_as<typename T::NodeState, VBRNodeState>;
+
 // DAGColorPool::Config (src/DAGColorPool.hpp:36-41) + capacities of the device pool.
 struct ColorConfig {
 	Word leaf_level;
@@ -250,9 +321,79 @@ public:
 		hd_edit_desc d = editor.ToDesc();
 		return edit_descs(root, std::span<const hd_edit_desc>(&d, 1), HD_MODE_PLAIN);
 	}
-	// NodePoolThreadedEdit::ThreadedEdit: the device pipeline is parallel; results equal Edit's.
-	template <DeviceEditor Editor_T> inline NodePointer ThreadedEdit(NodePointer root, const Editor_T &editor) {
+	// A generic stateless functor editor: NodePoolBase::Edit's recursion runs on the host over the
+	// device pool's words (hd_edit_host); the result equals the reference's Edit word for word.
+	template <HostStatelessEditor Editor_T> inline NodePointer Edit(NodePointer root, const Editor_T &editor) {
+		struct Ctx {
+			const Editor_T *ed;
+			const Config *cfg;
+		} ctx{&editor, &m_config};
+		hd_host_editor he{};
+		he.user = &ctx;
+		he.edit_node = [](void *u, uint32_t level, uint32_t x, uint32_t y, uint32_t z, uint32_t ptr) -> uint32_t {
+			auto *c = static_cast<Ctx *>(u);
+			return (uint32_t)(EditType)c->ed->EditNode(*c->cfg, NodeCoord{level, {x, y, z}}, NodePointer{ptr});
+		};
+		he.edit_voxel = [](void *u, uint32_t x, uint32_t y, uint32_t z, uint32_t v) -> uint32_t {
+			auto *c = static_cast<Ctx *>(u);
+			return c->ed->EditVoxel(*c->cfg, NodeCoord{c->cfg->GetVoxelLevel(), {x, y, z}}, v != 0u) ? 1u : 0u;
+		};
+		Word r = *root;
+		if (hd_edit_host(m_pool, *root, &he, &r) != HD_OK)
+			return root;
+		m_root = NodePointer{r};
+		return m_root;
+	}
+	// The reference's wrapped editors (main.cpp:214-238): a VBREditorWrapper edit starts from its
+	// octree_root; a StatelessEditorWrapper edit runs without colors (on the host for a functor).
+	template <typename W, DeviceEditor E, typename O>
+	inline NodePointer Edit(NodePointer root, const VBREditorWrapper<W, E, O> &wrapped) {
+		m_color_root = (Word)wrapped.octree_root;
+		hd_edit_desc d = wrapped.ToDesc();
+		return edit_descs(root, std::span<const hd_edit_desc>(&d, 1), HD_MODE_PLAIN);
+	}
+	template <typename W, typename E> inline NodePointer Edit(NodePointer root, const StatelessEditorWrapper<W, E> &w) {
+		if constexpr (DeviceEditor<E>) {
+			hd_edit_desc d = w.editor.ToDesc();
+			d.colored = 0;
+			return edit_descs(root, std::span<const hd_edit_desc>(&d, 1), HD_MODE_PLAIN);
+		} else
+			return Edit(root, w.editor);
+	}
+	// NodePoolBase::Edit(root, editor, on_edit_done) (NodePool.hpp:405-414): returns
+	// on_edit_done(new_root, state), state = VBRNodeState for colored edits, std::monostate else.
+	template <typename Editor_T, typename OnDone>
+	    requires std::invocable<OnDone, NodePointer, std::monostate>
+	inline auto Edit(NodePointer root, const Editor_T &editor, OnDone &&on_edit_done) {
+		NodePointer r = Edit(root, editor);
+		if constexpr (VBRWrapped<Editor_T>)
+			return on_edit_done(r, VBRNodeState{m_color_root});
+		else if constexpr (DeviceEditor<Editor_T> && std::invocable<OnDone, NodePointer, VBRNodeState>) {
+			if (editor.ToDesc().colored)
+				return on_edit_done(r, VBRNodeState{m_color_root});
+			return on_edit_done(r, std::monostate{});
+		} else
+			return on_edit_done(r, std::monostate{});
+	}
+	template <typename Editor_T, typename OnDone>
+	    requires(!std::invocable<OnDone, NodePointer, std::monostate>) &&
+	            std::invocable<OnDone, NodePointer, VBRNodeState>
+	inline auto Edit(NodePointer root, const Editor_T &editor, OnDone &&on_edit_done) {
+		NodePointer r = Edit(root, editor);
+		return on_edit_done(r, VBRNodeState{m_color_root});
+	}
+	// NodePoolThreadedEdit::ThreadedEdit (NodePoolThreadedEdit.hpp:104-126): the device pipeline is
+	// parallel and its results equal Edit's; the busy_pool and max_task_level are unused.
+	template <typename Editor_T> inline NodePointer ThreadedEdit(NodePointer root, const Editor_T &editor) {
 		return Edit(root, editor);
+	}
+	template <typename BusyPool, typename Editor_T>
+	inline NodePointer ThreadedEdit(BusyPool *, NodePointer root, const Editor_T &editor, Word = Word(-1)) {
+		return Edit(root, editor);
+	}
+	template <typename BusyPool, typename Editor_T, typename OnDone>
+	inline auto ThreadedEdit(BusyPool *, NodePointer root, const Editor_T &editor, Word, OnDone &&on_edit_done) {
+		return Edit(root, editor, std::forward<OnDone>(on_edit_done));
 	}
 	// A batch of edits in one call: HD_MODE_PLAIN applies them one by one (bit-exact with a
 	// sequence of Edit calls); HD_MODE_BATCHED groups commuting edits into shared passes.
@@ -268,6 +409,17 @@ public:
 			return false;
 		root = NodePointer{r};
 		return true;
+	}
+	// The reference's signatures (NodePoolThreadedGC.hpp:394-403): the remapped root / roots (the
+	// given ones when the GC fails); the busy_pool is unused.
+	template <typename BusyPool> inline NodePointer ThreadedGC(BusyPool *, NodePointer root) {
+		NodePointer r = root;
+		return ThreadedGC(r) ? r : root;
+	}
+	template <typename BusyPool>
+	inline std::vector<NodePointer> ThreadedGC(BusyPool *, std::vector<NodePointer> roots) {
+		std::vector<NodePointer> r = roots;
+		return ThreadedGC(r) ? r : roots;
 	}
 	inline bool ThreadedGC(std::vector<NodePointer> &roots) {
 		std::vector<Word> r(roots.size());

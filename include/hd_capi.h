@@ -122,6 +122,16 @@ void *hd_pool_stream(hd_pool *pool); /* the pool's hipStream_t */
 /* Heightfield for HD_EDIT_COLUMN_FILL: resolution x resolution u32 heights, row-major, x fastest. */
 int hd_pool_set_heightmap(hd_pool *pool, const uint32_t *heights, uint32_t res);
 
+/* The benchmark terrain's heightfield generated on the device and installed like
+ * hd_pool_set_heightmap (resolution = the pool's): fractal value noise, base_cells lattice cells
+ * across the grid, each octave lacunarity x the frequency and gain x the amplitude, normalised to
+ * [0, 1] and mapped to rint(zlo + t (zhi - zlo)) clamped to [0, resolution]. Identical heights to
+ * vkhashdag_amd.workloads.value_noise_heightmap (same float64 operations and order); heights_out
+ * (host, resolution^2 u32, may be null) receives them. No counterpart in the reference (synthetic
+ * workload input). */
+int hd_pool_value_noise_heightmap(hd_pool *pool, uint32_t octaves, uint64_t seed, uint32_t zlo, uint32_t zhi,
+                                  uint32_t base_cells, double lacunarity, double gain, uint32_t *heights_out);
+
 /* Host interop (the Flush format of src/DAGNodePool.cpp:56-85 without sparse binding):
  * word ranges of the node pool, per-bucket word counters, color arrays. */
 int hd_pool_read_words(hd_pool *pool, uint64_t first, uint64_t count, uint32_t *out);
@@ -156,6 +166,26 @@ enum {
  * (main.cpp:214-234). Blocking: returns after the device work completed. */
 int hd_edit(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_edit_desc *edits, uint32_t n,
             uint32_t mode, uint32_t *root_out, uint32_t *color_root_out);
+
+/* Editor decisions: hashdag::EditType (Editor.hpp:18), same order. */
+enum { HD_EDIT_TYPE_NOT_AFFECTED = 0, HD_EDIT_TYPE_PROCEED = 1, HD_EDIT_TYPE_FILL = 2, HD_EDIT_TYPE_CLEAR = 3 };
+
+/* A generic stateless editor on the host: hashdag::StatelessEditor (Editor.hpp:36-40) as two
+ * callbacks. edit_node(user, level, x, y, z, node_ptr) is EditNode(config, NodeCoord{level, pos},
+ * node_ptr) and returns an HD_EDIT_TYPE_*; edit_voxel(user, x, y, z, voxel) is EditVoxel at the
+ * voxel level and returns the voxel's new occupancy (0 / 1). */
+typedef struct hd_host_editor {
+	void *user;
+	uint32_t (*edit_node)(void *user, uint32_t level, uint32_t x, uint32_t y, uint32_t z, uint32_t node_ptr);
+	uint32_t (*edit_voxel)(void *user, uint32_t x, uint32_t y, uint32_t z, uint32_t voxel);
+} hd_host_editor;
+
+/* NodePoolBase::Edit(root, StatelessEditorWrapper{editor}) (NodePool.hpp:405-417) for an arbitrary
+ * host editor: the reference's single-threaded recursion runs on the host over the device pool's
+ * used words, then the words it appended and the changed bucket counters are uploaded (the dirty
+ * ranges of DAGNodePool::Flush, src/DAGNodePool.cpp:56-85). Host and device edits interleave on one
+ * pool word for word like a sequence of Edit calls. Colors are untouched (stateless). Blocking. */
+int hd_edit_host(hd_pool *pool, uint32_t root, const hd_host_editor *editor, uint32_t *root_out);
 
 /* NodePoolThreadedGC::ThreadedGC (NodePoolThreadedGC.hpp:394-403): keeps the DAGs below roots[0..n)
  * and the filled chain, compacts every bucket, remaps roots in place. */
@@ -215,9 +245,16 @@ int hd_traversal(hd_pool *pool, uint32_t root, const float *o, const float *d, u
  * the roots the passes leave on the device. The sequence is captured once at creation; each frame
  * sets its edits and views (hd_frame_set) and replays the graph (hd_frame_launch). Edits are grouped
  * as HD_MODE_BATCHED groups them, so a frame's DAG equals hd_edit(..., HD_MODE_BATCHED) word for
- * word, and one edit per frame equals HD_MODE_PLAIN. The node-pool GC stays outside the graph:
- * after hd_gc, hand the remapped roots back with hd_frame_set_roots. */
-enum { HD_FRAME_COLORS = 1 }; /* colored (VBR) edits allowed: the color passes are captured */
+ * word, and one edit per frame equals HD_MODE_PLAIN. With HD_FRAME_GC the GC runs inside the graph
+ * (the frame's words then equal hd_edit batched + hd_gc); without it, after an hd_gc hand the
+ * remapped roots back with hd_frame_set_roots. */
+enum {
+	HD_FRAME_COLORS = 1, /* colored (VBR) edits allowed: the color passes are captured */
+	HD_FRAME_GC = 2      /* a node-pool GC (NodePoolThreadedGC::ThreadedGC of the frame's root and the
+	                        filled chain, NodePoolThreadedGC.hpp:394-403) is captured between the passes
+	                        and the trace: each frame is edit -> GC -> render; the root the frame
+	                        returns is the compacted one and older roots become invalid */
+};
 
 typedef struct hd_frame_config {
 	uint32_t max_edits, max_passes, max_radius;

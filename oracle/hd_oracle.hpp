@@ -292,17 +292,29 @@ struct Editor {
 		return n2 <= d.r2;
 	}
 	// Column fill geometry: voxel (x,y,z) is inside iff z < h(x,y).
+	// Column rectangle [p0.x, p1.x) x [p0.y, p1.y) of the node's footprint: 0 outside, 1 crossing, 2 inside.
+	u32 column_rect(u32 k, u32 x, u32 y) const {
+		const u64 x0 = (u64)x << k, x1 = (u64)(x + 1u) << k, y0 = (u64)y << k, y1 = (u64)(y + 1u) << k;
+		if (x1 <= d.p0[0] || x0 >= d.p1[0] || y1 <= d.p0[1] || y0 >= d.p1[1])
+			return 0u;
+		return x0 >= d.p0[0] && x1 <= d.p1[0] && y0 >= d.p0[1] && y1 <= d.p1[1] ? 2u : 1u;
+	}
 	EditType column_node(const Config &cfg, u32 level, const u32 pos[3]) const {
 		u32 k = cfg.voxel_level() - level;
+		const u32 rect = column_rect(k, pos[0], pos[1]);
+		if (!rect)
+			return EditType::kNotAffected;
 		u32 hmin = hm->fmin(k, pos[0], pos[1]), hmax = hm->fmax(k, pos[0], pos[1]);
 		u64 z0 = (u64)pos[2] << k, z1 = (u64)(pos[2] + 1u) << k;
 		if (z0 >= hmax)
 			return EditType::kNotAffected;
 		if (z1 <= hmin)
-			return EditType::kFill;
+			return rect == 2u ? EditType::kFill : EditType::kProceed;
 		return EditType::kProceed;
 	}
-	bool column_voxel(const u32 p[3]) const { return p[2] < hm->h(p[0], p[1]); }
+	bool column_voxel(const u32 p[3]) const {
+		return p[0] >= d.p0[0] && p[0] < d.p1[0] && p[1] >= d.p0[1] && p[1] < d.p1[1] && p[2] < hm->h(p[0], p[1]);
+	}
 	// Per-voxel column color: deep color below the skin, else the block's ramp+noise color.
 	u32 column_rgb8(const u32 p[3]) const {
 		u32 bx = p[0] >> 2, by = p[1] >> 2, bz = p[2] >> 2;
@@ -314,6 +326,8 @@ struct Editor {
 	// Uniform column color over a node, when the rule gives one.
 	bool column_uniform(const Config &cfg, u32 level, const u32 pos[3], u32 &rgb) const {
 		u32 k = cfg.voxel_level() - level;
+		if (column_rect(k, pos[0], pos[1]) != 2u)
+			return false;
 		if (k <= 2) {
 			u32 p[3] = {pos[0] << k, pos[1] << k, pos[2] << k};
 			rgb = column_rgb8(p);

@@ -26,6 +26,35 @@ static std::vector<uint32_t> live_words(hd_pool *p) {
 	return w;
 }
 
+// SphereEditor<kDig> (main.cpp:73-150) written as a generic functor against the facade's Config /
+// NodeCoord / NodePointer: it runs on the host through DeviceNodePool::Edit (hd_edit_host).
+struct SphereDigFunctor {
+	std::array<Word, 3> center;
+	uint64_t r2;
+	inline EditType EditNode(const Config &config, const NodeCoord &coord, NodePointer) const {
+		auto lb = coord.GetLowerBoundAtLevel(config.GetVoxelLevel()), ub = coord.GetUpperBoundAtLevel(config.GetVoxelLevel());
+		const int64_t ld[3] = {(int64_t)lb.x - center[0], (int64_t)lb.y - center[1], (int64_t)lb.z - center[2]};
+		const int64_t ud[3] = {(int64_t)ub.x - center[0], (int64_t)ub.y - center[1], (int64_t)ub.z - center[2]};
+		uint64_t mx = 0, mn = 0;
+		for (int i = 0; i < 3; ++i) {
+			const uint64_t a = (uint64_t)(ld[i] * ld[i]), b = (uint64_t)(ud[i] * ud[i]);
+			mx += a > b ? a : b;
+			mn += (ld[i] > 0 ? a : 0) + (ud[i] < 0 ? b : 0);
+		}
+		if (mx <= r2)
+			return EditType::kClear;
+		return mn > r2 ? EditType::kNotAffected : EditType::kProceed;
+	}
+	inline bool EditVoxel(const Config &, const NodeCoord &coord, bool voxel) const {
+		const int64_t d[3] = {(int64_t)coord.pos.x - center[0], (int64_t)coord.pos.y - center[1],
+		                      (int64_t)coord.pos.z - center[2]};
+		return voxel && !((uint64_t)(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) <= r2);
+	}
+};
+static_assert(HostStatelessEditor<SphereDigFunctor>);
+
+struct FakeBusyPool {}; // stands in for the caller's lf::busy_pool: accepted, unused
+
 int main() {
 	Config config = DefaultConfig{.level_count = 7, .top_level_count = 3, .word_bits_per_page = 5,
 	                              .page_bits_per_bucket = 2, .bucket_bits_per_top_level = 4,
@@ -74,13 +103,44 @@ int main() {
 	CHECK(root && *root == rr && pool->GetColorRoot() == rc);
 	CHECK(live_words(pool->GetHandle()) == live_words(raw));
 
+	// a generic functor on the host equals the device's descriptor edit word for word, through the
+	// reference's call shapes: ThreadedEdit(busy_pool*, root, wrapper, max_task_level, on_done)
+	FakeBusyPool busy;
+	struct Result {
+		NodePointer node_ptr;
+		std::optional<Word> color;
+	};
+	auto on_done = [](NodePointer r, auto &&state) -> Result {
+		if constexpr (requires { state.octree_node; })
+			return {r, state.octree_node};
+		else
+			return {r, std::nullopt};
+	};
+	Result hr = pool->ThreadedEdit(&busy, root, StatelessEditorWrapper<Word, SphereDigFunctor>{{{70, 60, 50}, 20 * 20}},
+	                               cc.leaf_level, on_done);
+	hd_edit_desc dig = SphereEditor<SphereEditMode::kDig>{{70, 60, 50}, 20 * 20, {}}.ToDesc();
+	CHECK(hd_edit(raw, rr, rc, &dig, 1, HD_MODE_PLAIN, &rr, &rc) == HD_OK);
+	CHECK(!hr.color.has_value() && *hr.node_ptr == rr);
+	CHECK(live_words(pool->GetHandle()) == live_words(raw));
+	Result vr = pool->ThreadedEdit(
+	    &busy, hr.node_ptr,
+	    VBREditorWrapper<Word, AABBEditor, DeviceNodePool>{
+	        .editor = AABBEditor{{30, 30, 30}, {70, 50, 60}, VBRColor::RGB8(0x7788AA)},
+	        .p_octree = pool.get(),
+	        .octree_root = pool->GetColorRoot()},
+	    cc.leaf_level, on_done);
+	hd_edit_desc box = AABBEditor{{30, 30, 30}, {70, 50, 60}, VBRColor::RGB8(0x7788AA)}.ToDesc();
+	CHECK(hd_edit(raw, rr, rc, &box, 1, HD_MODE_PLAIN, &rr, &rc) == HD_OK);
+	CHECK(vr.color.has_value() && *vr.color == rc && *vr.node_ptr == rr);
+	CHECK(live_words(pool->GetHandle()) == live_words(raw));
+	root = vr.node_ptr;
+
 	// a failing edit (column fill without a heightmap) keeps the root, as the reference's fallback
 	NodePointer same = pool->Edit(root, ColumnFillEditor{});
 	CHECK(same == root);
 
-	// ThreadedGC remaps the root like hd_gc
-	NodePointer g = root;
-	CHECK(pool->ThreadedGC(g));
+	// ThreadedGC(busy_pool*, root) returns the remapped root like hd_gc
+	NodePointer g = pool->ThreadedGC(&busy, root);
 	uint32_t gr = rr;
 	CHECK(hd_gc(raw, &gr, 1) == HD_OK);
 	CHECK(*g == gr);

@@ -144,3 +144,39 @@ def test_config3_batched_4096_mixed_equals_sequential(heights_4096):
     rgba_o, rec_o = ora.render(view, 960, 540, shadow=True)
     assert (rec_o[:, 3] >> 31).mean() > 0.3
     assert np.array_equal(rgba_d, rgba_o)
+
+
+@pytest.mark.parametrize("res,octaves,seed,zlo,zhi", [(256, 4, 5, 20, 200), (1024, 7, 3, 100, 900),
+                                                      (4096, 6, 1, 512, 3072)])
+def test_device_heightmap_equals_host_generator(res, octaves, seed, zlo, zhi, heights_4096):
+    # the larger terrains (16384^2, 32768^2 heightfields) are generated on the device; the generator
+    # must give exactly the host recipe's heights (same float64 operations in the same order)
+    capi = _capi()
+    cfg = W.config_for_resolution(res, bucket_bits=8, wbpp=9, pbpb=2, color_leaf_depth=4)
+    want = heights_4096 if res == 4096 else W.value_noise_heightmap(res, octaves, seed, zlo, zhi)
+    dev = capi.DevicePool(cfg)
+    got = dev.value_noise_heightmap(octaves, seed, zlo, zhi)
+    bad = np.argwhere(got != want)
+    assert len(bad) == 0, (len(bad), bad[:4], got[tuple(bad[0])], want[tuple(bad[0])])
+    # and the column fill over it equals the fill over the uploaded heights
+    if res <= 1024:
+        ref = capi.DevicePool(cfg, heights=want)
+        t = W.terrain_edit(res, seed=seed, skin=6)
+        dev.edit(t)
+        ref.edit(t)
+        assert (dev.root, dev.color_root) == (ref.root, ref.color_root)
+        assert np.array_equal(dev.live_words()[1], ref.live_words()[1])
+
+
+def test_tiled_terrain_build_bit_exact():
+    # the large terrains are built tile by tile (column rectangles); device == oracle word for word
+    capi = _capi()
+    res = 256
+    h = W.value_noise_heightmap(res, 5, 7, 30, 200)
+    cfg = W.config_for_resolution(res, bucket_bits=10, wbpp=9, pbpb=2, color_leaf_depth=4)
+    dev, ora = capi.DevicePool(cfg, heights=h), O.OraclePool(cfg, heights=h)
+    t = W.terrain_tiles(res, 5, 4, skin=8)
+    for i in range(len(t)):
+        dev.edit(t[i:i + 1])
+        ora.edit(t[i:i + 1])
+        assert_same_pool(dev, ora, f"tile {i}")

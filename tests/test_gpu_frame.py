@@ -96,3 +96,31 @@ def test_frames_continue_after_gc_and_check_budgets():
     with pytest.raises(capi.HDError, match="column fills"):
         f.set(W.edits([W.terrain_edit(res, seed=1)]), [])
     f.close()
+
+
+def test_frames_with_gc_in_the_graph_equal_edit_then_gc():
+    # HD_FRAME_GC: edit -> GC -> render captured once; every replay equals hd_edit (batched) then
+    # hd_gc of the new root on a twin pool, word for word (the GC runs inside the graph), and each
+    # view equals a lone render of the compacted root
+    capi = _capi()
+    res = 128
+    cfg, h, (a, b) = _terrain(capi, res)
+    f = capi.Frame(a, max_edits=16, max_passes=6, max_radius=20, n_views=2, max_width=64, max_height=48, gc=True)
+    for k in range(4):
+        e = W.mixed_batch(12, res, h, 2, 16, seed=300 + k)
+        views = _views(res, 20 + k)
+        f.set(e, views)
+        f.launch()
+        f.wait()
+        b.edit(e, capi.MODE_BATCHED)
+        b.gc()
+        assert_same_device_pools(a, b, f"frame {k} with GC")
+        assert np.array_equal(a.filled(), b.filled())
+        for (vb, w, hh, fl), img in zip(views, f.images()):
+            assert np.array_equal(img, b.render(vb, w, hh, shadow=True)), f"frame {k}: view differs"
+    f.close()
+    # the pool goes on as a plain pool afterwards
+    e = W.mixed_batch(6, res, h, 2, 12, seed=400)
+    a.edit(e, capi.MODE_BATCHED)
+    b.edit(e, capi.MODE_BATCHED)
+    assert_same_device_pools(a, b, "after the frames")

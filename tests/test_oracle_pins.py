@@ -658,3 +658,31 @@ def test_edit_traffic_known_answers():
     a, b = q1.edit_traffic(more), q2.edit_traffic(more, batch=True)
     assert q1.root == q2.root
     assert b["visited"] <= a["visited"] and b["written"] <= a["written"] and b["upserts"] <= a["upserts"]
+
+
+@pytest.mark.parametrize("colored", [True, False])
+def test_tiled_terrain_equals_whole_terrain(colored):
+    # a column fill restricted to column rectangles [p0.xy, p1.xy): the tiles applied one after
+    # another give the whole-grid fill's voxels (a reduced DAG: canonical equality) and colors
+    res = 128
+    h = W.value_noise_heightmap(res, 4, 6, 20, 100)
+    cfg = small_cfg(res, 8, 9, 2, 3)
+    whole, tiled = O.OraclePool(cfg, heights=h), O.OraclePool(cfg, heights=h)
+    whole.edit(W.terrain_edit(res, seed=3, skin=6, colored=colored))
+    t = W.terrain_tiles(res, 3, 4, skin=6, colored=colored)
+    assert len(t) == 16 and int(t[5]["p0"][0]) == 32 and int(t[5]["p1"][1]) == 64
+    for i in range(len(t)):
+        tiled.edit(t[i:i + 1])
+    assert np.array_equal(whole.canonical(), tiled.canonical())
+    g = whole.voxels()
+    assert g.sum() == sum(int(v) for v in h.astype(np.int64).ravel())
+    if colored:
+        vox = np.ascontiguousarray(np.argwhere(g)[:, ::-1].astype(np.uint32))
+        assert np.array_equal(whole.colors(vox), tiled.colors(vox))
+    # a rectangle covers only its columns
+    part = O.OraclePool(cfg, heights=h)
+    part.edit(W.terrain_edit(res, seed=3, skin=6, lo=(16, 40), hi=(80, 64), colored=colored))
+    gp = part.voxels()
+    z, y, x = np.meshgrid(np.arange(res), np.arange(res), np.arange(res), indexing="ij")
+    want = (z < h[y, x].astype(np.int64)) & (x >= 16) & (x < 80) & (y >= 40) & (y < 64)
+    assert np.array_equal(gp, want)

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhashdag_amd.so")
-SOURCES = ["hd_pool.hip", "hd_edit.hip", "hd_trace.hip", "hd_gc.hip", "hd_serve.hip"]
+SOURCES = ["hd_pool.hip", "hd_edit.hip", "hd_trace.hip", "hd_gc.hip", "hd_serve.hip", "hd_host.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HD_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Xclang",

@@ -30,7 +30,7 @@ class hd_serve_config(C.Structure):
                 ("min_free_pages", C.c_uint64)]
 
 
-FRAME_COLORS = 1
+FRAME_COLORS, FRAME_GC = 1, 2
 
 
 class hd_frame_config(C.Structure):
@@ -69,6 +69,15 @@ class hd_tuning(C.Structure):
 TUNING_FIELDS = tuple(f for f, _ in hd_tuning._fields_)
 
 
+EDIT_NODE_FN = C.CFUNCTYPE(C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32)
+EDIT_VOXEL_FN = C.CFUNCTYPE(C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32)
+EDIT_TYPE_NOT_AFFECTED, EDIT_TYPE_PROCEED, EDIT_TYPE_FILL, EDIT_TYPE_CLEAR = range(4)
+
+
+class hd_host_editor(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("edit_node", EDIT_NODE_FN), ("edit_voxel", EDIT_VOXEL_FN)]
+
+
 class hd_view(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("look", C.c_float * 3), ("side", C.c_float * 3), ("up", C.c_float * 3),
                 ("proj_factor", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("flags", C.c_uint32),
@@ -95,6 +104,7 @@ def lib():
             "hd_pool_sync": (C.c_int, [VP]),
             "hd_pool_stream": (VP, [VP]),
             "hd_pool_set_heightmap": (C.c_int, [VP, VP, U32]),
+            "hd_pool_value_noise_heightmap": (C.c_int, [VP, U32, U64, U32, U32, U32, C.c_double, C.c_double, VP]),
             "hd_pool_read_words": (C.c_int, [VP, U64, U64, VP]),
             "hd_pool_write_words": (C.c_int, [VP, U64, U64, VP]),
             "hd_pool_read_bucket_words": (C.c_int, [VP, U64, U64, VP]),
@@ -106,6 +116,7 @@ def lib():
             "hd_pool_clear_overflow": (C.c_int, [VP]),
             "hd_edit": (C.c_int, [VP, U32, U32, VP, U32, U32, P32, P32]),
             "hd_gc": (C.c_int, [VP, VP, U32]),
+            "hd_edit_host": (C.c_int, [VP, U32, C.POINTER(hd_host_editor), P32]),
             "hd_trace": (C.c_int, [VP, U32, U32, VP, U32, VP, VP]),
             "hd_trace_async": (C.c_int, [VP, U32, U32, VP, VP, U32, U32, VP, VP, VP]),
             "hd_timer_begin": (C.c_int, [VP]),
@@ -248,6 +259,16 @@ class DevicePool:
     def __del__(self):
         self.close()
 
+    def value_noise_heightmap(self, octaves, seed, zlo, zhi, base_cells=4, lacunarity=2.0, gain=0.5,
+                              download=True):
+        """Generates workloads.value_noise_heightmap's terrain on the device (identical heights) and
+        installs it as the column-fill input; returns the heights (host) when download."""
+        res = self.cfg.resolution
+        out = np.empty((res, res), np.uint32) if download else None
+        check(lib().hd_pool_value_noise_heightmap(self.h, octaves, seed, zlo, zhi, base_cells, lacunarity, gain,
+                                                  out.ctypes.data if download else None))
+        return out
+
     def set_heightmap(self, heights):
         heights = np.ascontiguousarray(heights, np.uint32)
         check(lib().hd_pool_set_heightmap(self.h, heights.ctypes.data, heights.shape[0]))
@@ -271,6 +292,17 @@ class DevicePool:
         return roots
 
     ThreadedGC = gc
+
+    def edit_host(self, edit_node, edit_voxel):
+        """NodePoolBase::Edit with a generic stateless editor evaluated on the host (hd_edit_host):
+        edit_node(level, x, y, z, node_ptr) -> EDIT_TYPE_*, edit_voxel(x, y, z, voxel) -> bool."""
+        fn = EDIT_NODE_FN(lambda _u, l, x, y, z, p: int(edit_node(l, x, y, z, p)))
+        fv = EDIT_VOXEL_FN(lambda _u, x, y, z, v: 1 if edit_voxel(x, y, z, bool(v)) else 0)
+        ed = hd_host_editor(None, fn, fv)
+        r = U32(0)
+        check(lib().hd_edit_host(self.h, self.root, C.byref(ed), C.byref(r)))
+        self.root = r.value
+        return self.root
 
     # -- host interop
     def read_words(self, first, count):
@@ -397,10 +429,11 @@ class DevicePool:
 class Frame:
     """An edit -> render frame captured as a hipGraph (hd_frame_*): set edits + views, launch, wait."""
 
-    def __init__(self, pool, max_edits, max_passes, max_radius, n_views, max_width, max_height, colors=True):
+    def __init__(self, pool, max_edits, max_passes, max_radius, n_views, max_width, max_height, colors=True,
+                 gc=False):
         self.pool = pool
         cfg = hd_frame_config(max_edits, max_passes, max_radius, n_views, max_width, max_height,
-                              FRAME_COLORS if colors else 0, 0)
+                              (FRAME_COLORS if colors else 0) | (FRAME_GC if gc else 0), 0)
         h = VP()
         check(lib().hd_frame_create(pool.h, C.byref(cfg), pool.root, pool.color_root, C.byref(h)))
         self.h = h.value

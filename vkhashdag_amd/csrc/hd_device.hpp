@@ -241,17 +241,30 @@ struct DevEditor {
 		i64 dx = (i64)x - (i64)d.p0[0], dy = (i64)y - (i64)d.p0[1], dz = (i64)z - (i64)d.p0[2];
 		return (u64)(dx * dx + dy * dy + dz * dz) <= d.r2;
 	}
+	// The columns [p0.x, p1.x) x [p0.y, p1.y): 0 = outside, 1 = the node's footprint crosses the
+	// rectangle's edge, 2 = inside (a terrain is built whole, or tile by tile)
+	__device__ u32 column_rect(u32 k, u32 x, u32 y) const {
+		const u64 x0 = (u64)x << k, x1 = (u64)(x + 1u) << k, y0 = (u64)y << k, y1 = (u64)(y + 1u) << k;
+		if (x1 <= d.p0[0] || x0 >= d.p1[0] || y1 <= d.p0[1] || y0 >= d.p1[1])
+			return 0u;
+		return x0 >= d.p0[0] && x1 <= d.p1[0] && y0 >= d.p0[1] && y1 <= d.p1[1] ? 2u : 1u;
+	}
 	__device__ EditType column_node(u32 level, u32 x, u32 y, u32 z) const {
 		u32 k = voxel_level - level;
+		const u32 rect = column_rect(k, x, y);
+		if (!rect)
+			return kNotAffected;
 		u32 hmin = hm->fmin(k, x, y), hmax = hm->fmax(k, x, y);
 		u64 z0 = (u64)z << k, z1 = (u64)(z + 1u) << k;
 		if (z0 >= hmax)
 			return kNotAffected;
 		if (z1 <= hmin)
-			return kFill;
+			return rect == 2u ? kFill : kProceed;
 		return kProceed;
 	}
-	__device__ bool column_voxel(u32 x, u32 y, u32 z) const { return z < hm->h(x, y); }
+	__device__ bool column_voxel(u32 x, u32 y, u32 z) const {
+		return x >= d.p0[0] && x < d.p1[0] && y >= d.p0[1] && y < d.p1[1] && z < hm->h(x, y);
+	}
 	__device__ u32 column_rgb8(u32 x, u32 y, u32 z) const {
 		u32 bx = x >> 2, by = y >> 2, bz = z >> 2;
 		if ((u64)bz * 4u + 4u + d.skin <= hm->fmin(2, bx, by))
@@ -260,6 +273,8 @@ struct DevEditor {
 	}
 	__device__ bool column_uniform(u32 level, u32 x, u32 y, u32 z, u32 &rgb) const {
 		u32 k = voxel_level - level;
+		if (column_rect(k, x, y) != 2u)
+			return false;
 		if (k <= 2) {
 			rgb = column_rgb8(x << k, y << k, z << k);
 			return true;

@@ -3702,10 +3702,18 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 	std::vector<i64> box(6 * (size_t)n);
 	std::vector<u32> gid(n, 0);
 	groups.clear();
+	// A weighted (two R5G6B5 endpoints + weight) color that an edit fills at or above the color
+	// leaves is stored RGB8 by the color octree (DAGColorPool::FillNode), while a pass's per-voxel
+	// fold keeps it exact. Which of the two happens depends on the node's state when the edit runs
+	// (an empty node takes the fill), so such an edit must see exactly the sequential state: it
+	// runs in a group of its own, after every earlier edit and before every later one. RGB8 colors
+	// are stored exactly either way.
+	u32 barrier = 0;
 	for (u32 k = 0; k < n; ++k) {
 		i64 *b = &box[6 * (size_t)k];
 		edit_aabb(edits[k], res, b, b + 3);
-		u32 g = 0;
+		const bool weighted = is_stateful(edits[k]) && edits[k].color.has_value && edits[k].color.bits_per_weight;
+		u32 g = barrier;
 		bool empty = b[0] > b[3] || b[1] > b[4] || b[2] > b[5];
 		i64 c0[3], c1[3];
 		if (!empty) {
@@ -3719,6 +3727,10 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 								g = gid[j] + 1;
 						}
 		}
+		if (weighted) {
+			g = std::max<u32>(g, (u32)groups.size());
+			barrier = g + 1;
+		}
 		gid[k] = g;
 		if (g >= groups.size())
 			groups.resize(g + 1);
@@ -3729,24 +3741,6 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 					for (i64 x = c0[0]; x <= c1[0]; ++x)
 						grid[((size_t)z * cells + y) * cells + x].push_back(k);
 	}
-	// A weighted (two R5G6B5 endpoints + weight) color that an edit fills at or above the color
-	// leaves is stored RGB8 by the color octree (DAGColorPool::FillNode), while a pass's per-voxel
-	// fold keeps it exact; the two agree only where the edit's own wrapper logic decides the node.
-	// So an edit with weighted colors gets a group of its own, after its group's other edits (they
-	// are voxel-disjoint from it, so they commute): every node it reaches is then decided exactly
-	// as NodePoolBase::Edit decides it. RGB8 colors are stored exactly either way.
-	std::vector<std::vector<u32>> split;
-	split.reserve(groups.size());
-	for (auto &g : groups) {
-		std::vector<u32> rest, solo;
-		for (u32 k : g)
-			(is_stateful(edits[k]) && edits[k].color.has_value && edits[k].color.bits_per_weight ? solo : rest).push_back(k);
-		if (!rest.empty())
-			split.push_back(std::move(rest));
-		for (u32 k : solo)
-			split.push_back({k});
-	}
-	groups.swap(split);
 }
 
 // ---------------------------------------------------------------- sync-free pass: host side
@@ -4368,12 +4362,17 @@ static int frame_create(Pool &p, const hd_frame_config &cfg, u32 root, u32 croot
 	HD_TRY(hipMemcpyAsync(F.d_tiles, F.h_tiles[0], (cfg.n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
 	a_set_roots<<<1, 64, 0, p.stream>>>(F.B.roots, root, croot, F.B.acc);
 	HD_TRY(hipGetLastError());
+	const bool gc = (cfg.flags & HD_FRAME_GC) != 0;
+	if (gc) // the GC's buffers are sized by the pool's configuration: the graph keeps their addresses
+		HD_RET(p.gc_plan.prepare(p, ""));
 	HD_TRY(hipStreamSynchronize(p.stream));
-	// capture: the passes, then the trace of the roots they leave on the device
+	// capture: the passes, the GC of the root they leave on the device, then the trace of it
 	HD_TRY(hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal));
 	int rc = HD_OK;
 	for (u32 k = 0; k < cfg.max_passes && rc == HD_OK; ++k)
 		rc = queue_pass(p, F.plans[k], F.B, false);
+	if (rc == HD_OK && gc)
+		rc = gc_queue(p, p.gc_plan, p.stream, F.B.roots, 1, true);
 	if (rc == HD_OK)
 		rc = trace_on_stream(p, p.stream, 0, 0, F.d_views, F.d_tiles, cfg.n_views, F.max_tiles, F.d_rgba, F.B.roots);
 	hipError_t e = hipStreamEndCapture(p.stream, &F.graph);
@@ -4531,6 +4530,9 @@ extern "C" int hd_frame_wait(hd_frame *f, uint32_t *root_out, uint32_t *color_ro
 	HD_TRY(hipStreamSynchronize(p.stream));
 	if (F.h_roots[2 + kAccOverflow])
 		return set_error("hd_frame_wait: a level outgrew its bound (internal error)"), HD_ERR_CAPACITY;
+	if (F.cfg.flags & HD_FRAME_GC) { // the captured GC found the filled chain again on the device
+		HD_TRY(hipMemcpy(p.filled.data(), p.d_filled, p.dc.L * 4, hipMemcpyDeviceToHost));
+	}
 	if (root_out)
 		*root_out = F.h_roots[0];
 	if (color_root_out)

@@ -81,6 +81,23 @@ enum Meta : u32 {
 // bounds test, wherever in the pool it starts.
 constexpr u64 kWordsPad = 16;
 
+struct Pool;
+// Device buffers of a node-pool GC (hd_gc.hip), sized by the pool's configuration: per level a
+// bitmap of live node starts over the level's words and its popcount prefix (node ranks); new
+// pointers by rank for two adjacent levels; one level's staged words, new-bucket keys, bucket
+// lists, counts and offsets.
+struct GcPlan {
+	struct Level {
+		u32 *bits = nullptr, *rank = nullptr;
+		u64 words = 0, base = 0, nw = 0, buckets = 0;
+	};
+	std::vector<Level> lv;
+	u32 *newp[2] = {nullptr, nullptr};
+	u32 *stage = nullptr, *key = nullptr, *list = nullptr, *cnt = nullptr, *bstart = nullptr, *fill = nullptr,
+	    *bsum = nullptr;
+	int prepare(Pool &p, const char *prefix);
+};
+
 struct Pool {
 	hd_config cfg{};
 	DevConfig dc{};
@@ -106,6 +123,7 @@ struct Pool {
 	u32 last_groups = 0;
 	double last_ms = 0;
 	u32 edit_gen = 0; // generation of the sync-free pass's group table (hd_edit.hip)
+	GcPlan gc_plan;   // hd_gc's buffers
 };
 
 // Launch parameters (hd_tuning_set; process-wide): speed only, except that batch_pass_edits also
@@ -119,6 +137,10 @@ struct Tuning {
 extern Tuning g_tuning; // hd_pool.hip
 
 int make_filled(Pool &p);               // hd_pool.hip
+void launch_make_filled(Pool &p, hipStream_t s); // hd_pool.hip: the chain's lookup into d_filled, queued only
+// A whole GC queued on stream s over device roots (remapped in place); the filled chain is a root
+// when has_filled and is found again into d_filled. No host synchronisation (hd_gc.hip).
+int gc_queue(Pool &p, const GcPlan &G, hipStream_t s, u32 *d_roots, u32 n_roots, bool has_filled);
 // Multi-view trace of (root, color_root) queued on stream s; views and tile prefix on the device.
 // d_roots (optional): read the roots from device memory instead (graph replays).
 int trace_on_stream(Pool &p, hipStream_t s, u32 root, u32 color_root, const hd_view *d_views, const u32 *d_tiles,

@@ -52,6 +52,10 @@ __global__ void k_make_filled(u32 *words, u32 *bucket_words, DevConfig c, u32 *o
 	}
 }
 
+void launch_make_filled(Pool &p, hipStream_t s) {
+	k_make_filled<<<1, 64, 0, s>>>(p.words, p.bucket_words, p.dc, p.d_filled, p.meta);
+}
+
 int make_filled(Pool &p) {
 	if (!p.filled.empty())
 		return HD_OK;
@@ -243,12 +247,143 @@ int hd_pool_sync(hd_pool *h) {
 
 void *hd_pool_stream(hd_pool *h) { return h ? (void *)h->p.stream : nullptr; }
 
+// Installs a res x res heightmap as the column-fill editor's input: the heights either come from
+// the host (heights) or are already in the pool's base level (device; heights == null), then the
+// min / max pyramid is built above them.
+static int install_heightmap(Pool &p, const uint32_t *heights, uint32_t res, bool generated);
+
 int hd_pool_set_heightmap(hd_pool *h, const uint32_t *heights, uint32_t res) {
 	Pool &p = h->p;
 	if (!heights || res != (1u << p.dc.voxel_level)) {
 		set_error("heightmap must be resolution x resolution (%u)", 1u << p.dc.voxel_level);
 		return HD_ERR_ARG;
 	}
+	return install_heightmap(p, heights, res, false);
+}
+
+// ---- seeded value-noise heightfield on the device: the recipe of the benchmark's terrain, the same
+// float64 operations in the same order as the host generator (vkhashdag_amd/workloads.py
+// value_noise_heightmap; no contraction), so the heights are identical.
+namespace {
+__device__ inline double lattice(u64 ix, u64 iy, u64 c) { // splitmix64 finaliser of a lattice point
+	u64 k = (ix * 0x9E3779B97F4A7C15ull) ^ (iy * 0xC2B2AE3D27D4EB4Full) ^ c;
+	k ^= k >> 30;
+	k *= 0xBF58476D1CE4E5B9ull;
+	k ^= k >> 27;
+	k *= 0x94D049BB133111EBull;
+	k ^= k >> 31;
+	return (double)(k >> 11) * (1.0 / 9007199254740992.0);
+}
+struct NoiseArgs {
+	u32 res, octaves;
+	double base, lacunarity, gain, norm;
+	u64 consts[32];
+};
+__global__ void k_noise(NoiseArgs a, double *__restrict__ acc, unsigned long long *__restrict__ minmax) {
+	const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+	const u64 n = (u64)a.res * a.res;
+	double v = 0.0;
+	if (i < n) {
+		const u32 x = (u32)(i % a.res), y = (u32)(i / a.res);
+		const double xs = ((double)x + 0.5) / (double)a.res, ys = ((double)y + 0.5) / (double)a.res;
+		double amp = 1.0, freq = a.base;
+		for (u32 o = 0; o < a.octaves; ++o) {
+			const double fx = xs * freq, fy = ys * freq;
+			const double ix = floor(fx), iy = floor(fy);
+			double tx = fx - ix, ty = fy - iy;
+			tx = tx * tx * (3.0 - 2.0 * tx);
+			ty = ty * ty * (3.0 - 2.0 * ty);
+			const u64 x0 = (u64)ix, y0 = (u64)iy, c = a.consts[o];
+			const double top = lattice(x0, y0, c) * (1.0 - tx) + lattice(x0 + 1, y0, c) * tx;
+			const double bot = lattice(x0, y0 + 1, c) * (1.0 - tx) + lattice(x0 + 1, y0 + 1, c) * tx;
+			v = v + amp * (top * (1.0 - ty) + bot * ty);
+			amp *= a.gain;
+			freq *= a.lacunarity;
+		}
+		v = v / a.norm;
+		acc[i] = v;
+	}
+	// non-negative doubles order like their bit patterns: the wave's min / max, then one atomic each
+	unsigned long long lo = i < n ? (unsigned long long)__double_as_longlong(v) : ~0ull;
+	unsigned long long hi = i < n ? (unsigned long long)__double_as_longlong(v) : 0ull;
+	for (u32 d = 32; d >= 1; d >>= 1) {
+		lo = min(lo, (unsigned long long)__shfl_xor((long long)lo, d));
+		hi = max(hi, (unsigned long long)__shfl_xor((long long)hi, d));
+	}
+	if ((threadIdx.x & 63u) == 0) {
+		atomicMin(&minmax[0], lo);
+		atomicMax(&minmax[1], hi);
+	}
+}
+__global__ void k_noise_heights(u32 res, u32 zlo, u32 zhi, const double *__restrict__ acc,
+                                const unsigned long long *__restrict__ minmax, u32 *__restrict__ out_mn,
+                                u32 *__restrict__ out_mx) {
+	const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= (u64)res * res)
+		return;
+	const double lo = __longlong_as_double((long long)minmax[0]), hi = __longlong_as_double((long long)minmax[1]);
+	const double span = hi - lo > 1e-12 ? hi - lo : 1e-12;
+	const double t = (acc[i] - lo) / span;
+	double z = rint((double)zlo + t * (double)((i64)zhi - (i64)zlo));
+	z = z < 0.0 ? 0.0 : (z > (double)res ? (double)res : z);
+	out_mn[i] = out_mx[i] = (u32)z;
+}
+} // namespace
+
+extern "C" int hd_pool_value_noise_heightmap(hd_pool *h, uint32_t octaves, uint64_t seed, uint32_t zlo, uint32_t zhi,
+                                             uint32_t base_cells, double lacunarity, double gain,
+                                             uint32_t *heights_out) {
+	if (!h || !octaves || octaves > 32 || !base_cells || !(lacunarity > 0.0) || !(gain > 0.0))
+		return set_error("hd_pool_value_noise_heightmap: invalid argument"), HD_ERR_ARG;
+	Pool &p = h->p;
+	const u32 res = 1u << p.dc.voxel_level;
+	const u64 n = (u64)res * res;
+	NoiseArgs a{};
+	a.res = res;
+	a.octaves = octaves;
+	a.base = (double)base_cells;
+	a.lacunarity = lacunarity;
+	a.gain = gain;
+	double amp = 1.0, norm = 0.0;
+	for (u32 o = 0; o < octaves; ++o) {
+		a.consts[o] = seed * 0x165667B19E3779F9ull + (u64)o * 0x27D4EB2F165667C5ull;
+		norm += amp;
+		amp *= gain;
+	}
+	a.norm = norm;
+	double *acc;
+	unsigned long long *mm;
+	HD_RET(p.scratch.get<double>("noise_acc", n, &acc));
+	HD_RET(p.scratch.get<unsigned long long>("noise_minmax", 2, &mm));
+	const unsigned long long init[2] = {~0ull, 0ull};
+	HD_TRY(hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, p.stream));
+	// the base level of the pyramid is written in place, then install_heightmap builds the rest
+	u64 total = 0;
+	for (u32 s = res;; s >>= 1) {
+		total += (u64)s * s;
+		if (s == 1)
+			break;
+	}
+	if (p.hm_mn)
+		HD_TRY(hipFree(p.hm_mn));
+	if (p.hm_mx)
+		HD_TRY(hipFree(p.hm_mx));
+	p.hm_mn = p.hm_mx = nullptr;
+	HD_TRY(hipMalloc(&p.hm_mn, total * 4));
+	HD_TRY(hipMalloc(&p.hm_mx, total * 4));
+	k_noise<<<div_up(n, 256), 256, 0, p.stream>>>(a, acc, mm);
+	HD_TRY(hipGetLastError());
+	k_noise_heights<<<div_up(n, 256), 256, 0, p.stream>>>(res, zlo, zhi, acc, mm, p.hm_mn, p.hm_mx);
+	HD_TRY(hipGetLastError());
+	HD_RET(install_heightmap(p, nullptr, res, true));
+	if (heights_out) {
+		HD_TRY(hipMemcpyAsync(heights_out, p.hm_mn, n * 4, hipMemcpyDeviceToHost, p.stream));
+		HD_TRY(hipStreamSynchronize(p.stream));
+	}
+	return HD_OK;
+}
+
+static int install_heightmap(Pool &p, const uint32_t *heights, uint32_t res, bool generated) {
 	u64 total = 0;
 	DevHeightmap hm;
 	hm.res = res;
@@ -259,14 +394,17 @@ int hd_pool_set_heightmap(hd_pool *h, const uint32_t *heights, uint32_t res) {
 		if (s == 1)
 			break;
 	}
-	if (p.hm_mn)
-		HD_TRY(hipFree(p.hm_mn));
-	if (p.hm_mx)
-		HD_TRY(hipFree(p.hm_mx));
-	HD_TRY(hipMalloc(&p.hm_mn, total * 4));
-	HD_TRY(hipMalloc(&p.hm_mx, total * 4));
-	HD_TRY(hipMemcpyAsync(p.hm_mn, heights, (u64)res * res * 4, hipMemcpyHostToDevice, p.stream));
-	HD_TRY(hipMemcpyAsync(p.hm_mx, heights, (u64)res * res * 4, hipMemcpyHostToDevice, p.stream));
+	if (!generated) {
+		if (p.hm_mn)
+			HD_TRY(hipFree(p.hm_mn));
+		if (p.hm_mx)
+			HD_TRY(hipFree(p.hm_mx));
+		p.hm_mn = p.hm_mx = nullptr;
+		HD_TRY(hipMalloc(&p.hm_mn, total * 4));
+		HD_TRY(hipMalloc(&p.hm_mx, total * 4));
+		HD_TRY(hipMemcpyAsync(p.hm_mn, heights, (u64)res * res * 4, hipMemcpyHostToDevice, p.stream));
+		HD_TRY(hipMemcpyAsync(p.hm_mx, heights, (u64)res * res * 4, hipMemcpyHostToDevice, p.stream));
+	}
 	k = 1;
 	for (u32 s = res >> 1; s >= 1; s >>= 1, ++k) {
 		u64 n = (u64)s * s;

@@ -155,9 +155,21 @@ def value_noise_heightmap(res: int, octaves: int, seed: int, zlo: int, zhi: int,
     return np.clip(h, 0, res).astype(np.uint32)
 
 
-def terrain_edit(res: int, seed: int, deep_color: int = 0x505A64, skin: int = 8):
-    """The column-fill edit that builds the terrain (heights come from the pool's heightmap)."""
-    return edit(COLUMN_FILL, (0, 0, 0), (res, res, 0), r2=seed, color=deep_color, colored=True, skin=skin)
+def terrain_edit(res: int, seed: int, deep_color: int = 0x505A64, skin: int = 8, lo=(0, 0), hi=None,
+                 colored=True):
+    """The column-fill edit that builds the terrain (heights come from the pool's heightmap) over the
+    columns [lo, hi) (default: the whole grid)."""
+    hi = (res, res) if hi is None else hi
+    return edit(COLUMN_FILL, (lo[0], lo[1], 0), (hi[0], hi[1], 0), r2=seed, color=deep_color if colored else None,
+                colored=colored, skin=skin)
+
+
+def terrain_tiles(res: int, seed: int, tiles: int, deep_color: int = 0x505A64, skin: int = 8, colored=True):
+    """The terrain as tiles x tiles column fills over disjoint column rectangles, applied in order
+    (the same voxels and colors as the whole-grid fill; each pass touches 1/tiles^2 of the surface)."""
+    t = res // tiles
+    return edits([terrain_edit(res, seed, deep_color, skin, (i * t, j * t), ((i + 1) * t, (j + 1) * t), colored)
+                  for j in range(tiles) for i in range(tiles)])
 
 
 # ------------------------------------------------------------------ edit lists
