@@ -212,6 +212,15 @@ enum {
 int hd_trace(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *views, uint32_t n_views,
              uint32_t *rgba, uint32_t *records);
 
+/* hd_trace without any device allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * The staged views, tile and beam prefixes and the beam prepass image live in one static device
+ * array per GPU (4 MiB + 6 KiB, held until the process ends): at most 64 views and 2^20 beam
+ * texels (HD_ERR_CAPACITY beyond). Every call on a GPU shares that array, so calls that overlap
+ * on one GPU race. Same output as hd_trace; blocking. */
+int hd_trace_noalloc(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_view *views, uint32_t n_views,
+                     uint32_t *rgba, uint32_t *records);
+
 /* Asynchronous trace with views and the per-view tile prefix (hd_trace_tiles) already on the device;
  * no copies, no synchronisation. counters: device u64[4] (zeroed by the caller) selects the
  * instrumented variant counting [primary rays, shadow rays, march iterations, pool words read]. */

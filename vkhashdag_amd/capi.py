@@ -118,6 +118,7 @@ def lib():
             "hd_gc": (C.c_int, [VP, VP, U32]),
             "hd_edit_host": (C.c_int, [VP, U32, C.POINTER(hd_host_editor), P32]),
             "hd_trace": (C.c_int, [VP, U32, U32, VP, U32, VP, VP]),
+            "hd_trace_noalloc": (C.c_int, [VP, U32, U32, VP, U32, VP, VP]),
             "hd_trace_async": (C.c_int, [VP, U32, U32, VP, VP, U32, U32, VP, VP, VP]),
             "hd_timer_begin": (C.c_int, [VP]),
             "hd_timer_end": (C.c_int, [VP, C.POINTER(C.c_float)]),

@@ -42,6 +42,7 @@
 #include "hd_internal.hpp"
 
 #include <cmath>
+#include <mutex>
 
 namespace hd {
 
@@ -618,6 +619,95 @@ extern "C" int hd_traversal(hd_pool *h, uint32_t root, const float *o, const flo
 	k_traversal<<<div_up(n, kThreads), kThreads, trace_lds_bytes(p.dc.L, kThreads), p.stream>>>(p.words, root, p.dc.L, o,
 	                                                                                         d, n, out);
 	HD_TRY(hipGetLastError());
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
+
+// ---------------------------------------------------------------- trace without allocation
+// hd_trace for callers that cannot allocate device memory: the staged views, the tile and beam
+// prefixes and the beam prepass image live in one file-scope device array instead of the pool's
+// scratch. Layout (u32 words): views [0, 64 x 18), tile prefix [1152, 1217), beam prefix
+// [1217, 1282), beam image from word 1536 (2^20 floats, 256-byte aligned). Every call on a GPU
+// shares that GPU's copy: overlapping calls on one GPU race.
+namespace {
+constexpr u32 kNoAllocViews = 64;
+constexpr u32 kNoAllocBeam = 1u << 20;
+constexpr u32 kNoAllocViewWords = kNoAllocViews * (u32)(sizeof(hd_view) / 4);
+constexpr u32 kNoAllocBeamOff = 1536;
+constexpr u64 kNoAllocWords = kNoAllocBeamOff + (u64)kNoAllocBeam;
+static_assert(sizeof(hd_view) % 8 == 0 && kNoAllocViewWords + 2 * (kNoAllocViews + 1) <= kNoAllocBeamOff &&
+              kNoAllocBeamOff % 64 == 0);
+} // namespace
+
+static __device__ __align__(256) u32 scratch[kNoAllocWords];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mu;
+u32 *g_scratch_ptr[kNoAllocDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device (each GPU holds
+// its own copy at its own address; the host-side name is not a device address).
+int scratch_address(u32 **out) {
+	int dev = 0;
+	HD_TRY(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kNoAllocDevices)
+		return set_error("hd_trace_noalloc: device %d beyond the %d supported", dev, kNoAllocDevices), HD_ERR_DEVICE;
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (!g_scratch_ptr[dev]) {
+		void *ptr = nullptr;
+		HD_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+		if (!ptr)
+			return set_error("hd_trace_noalloc: no device address for the scratch array"), HD_ERR_DEVICE;
+		g_scratch_ptr[dev] = static_cast<u32 *>(ptr);
+	}
+	*out = g_scratch_ptr[dev];
+	return HD_OK;
+}
+} // namespace
+
+extern "C" int hd_trace_noalloc(hd_pool *h, uint32_t root, uint32_t color_root, const hd_view *views,
+                                uint32_t n_views, uint32_t *rgba, uint32_t *records) {
+	if (!h || !views || !n_views || !rgba)
+		return set_error("hd_trace_noalloc: null argument"), HD_ERR_ARG;
+	if (n_views > kNoAllocViews)
+		return set_error("hd_trace_noalloc: %u views, at most %u", n_views, kNoAllocViews), HD_ERR_CAPACITY;
+	Pool &p = h->p;
+	u32 tiles[kNoAllocViews + 1], beams[kNoAllocViews + 1];
+	tiles[0] = 0;
+	u64 n_beam = 0;
+	for (u32 v = 0; v < n_views; ++v) {
+		if (!views[v].width || !views[v].height)
+			return set_error("hd_trace_noalloc: empty view"), HD_ERR_ARG;
+		if ((views[v].flags & HD_TRACE_RECORDS) && !records)
+			return set_error("hd_trace_noalloc: records requested without a buffer"), HD_ERR_ARG;
+		if ((views[v].flags & HD_TRACE_BEAM) && !std::isfinite(views[v].proj_factor))
+			return set_error("hd_trace_noalloc: the beam prepass needs a finite proj_factor (LOD on)"), HD_ERR_ARG;
+		tiles[v + 1] = tiles[v] + trace_tile_count(views[v]);
+		if (views[v].flags & HD_TRACE_BEAM)
+			n_beam += (u64)div_up(views[v].width, kBeamBlock) * div_up(views[v].height, kBeamBlock);
+	}
+	if (n_beam > kNoAllocBeam)
+		return set_error("hd_trace_noalloc: %llu beam texels, at most %u", (unsigned long long)n_beam, kNoAllocBeam),
+		       HD_ERR_CAPACITY;
+	HD_RET(hd_trace_beam_sizes(views, n_views, beams));
+	u32 *base = nullptr;
+	HD_RET(scratch_address(&base));
+	hd_view *dv = reinterpret_cast<hd_view *>(base);
+	u32 *dt = base + kNoAllocViewWords, *db = dt + kNoAllocViews + 1;
+	float *beam = reinterpret_cast<float *>(base + kNoAllocBeamOff);
+	HD_TRY(hipMemcpyAsync(dv, views, n_views * sizeof(hd_view), hipMemcpyHostToDevice, p.stream));
+	HD_TRY(hipMemcpyAsync(dt, tiles, (n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
+	if (n_beam)
+		HD_TRY(hipMemcpyAsync(db, beams, (n_views + 1) * 4, hipMemcpyHostToDevice, p.stream));
+	TraceArgs A{p.words, p.cnodes, p.cleaves, root,   color_root,          p.dc.L,         p.dc.color_leaf,
+	            dv,      dt,       n_views,   rgba,   records,             nullptr,        nullptr,
+	            n_beam ? beam : nullptr,     n_beam ? db : nullptr};
+	if (n_beam)
+		k_beam<<<div_up((u32)n_beam, kThreads), kThreads, trace_lds_bytes(p.dc.L, kThreads), p.stream>>>(A, beam);
+	launch_trace(false, tiles[n_views], p.stream, A);
+	HD_TRY(hipGetLastError());
+	// the host arrays above are read by the copies: wait before they go out of scope
 	HD_TRY(hipStreamSynchronize(p.stream));
 	return HD_OK;
 }
