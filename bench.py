@@ -191,8 +191,8 @@ def main():
 
     batched = None if args.no_batched else batched_section(args, cfg, heights, res)
     serving = None if args.no_serve else serving_section(args, cfg, heights, res)
-    config4 = None if args.no_config4 else config4_section(args)
-    config5 = None if args.no_config5 else config5_section(args)
+    config4 = None if args.no_config4 else guarded(config4_section, args)
+    config5 = None if args.no_config5 else guarded(config5_section, args)
 
     cpu = None
     if state0 is not None:
@@ -332,6 +332,17 @@ def batched_section(args, cfg, heights, res):
     return out
 
 
+def guarded(section, args):
+    """Runs a workload section; a library error (capacity, argument) is reported in the line
+    instead of ending the run."""
+    from vkhashdag_amd import capi
+    try:
+        return section(args)
+    except capi.HDError as e:
+        log(f"{section.__name__}: {e}")
+        return {"error": str(e)}
+
+
 def pool_bytes(cfg, st):
     """Node-pool bytes per level (used words x 4 B) against the pool's capacity and 288 GB of HBM."""
     from vkhashdag_amd import capi  # noqa: F401
@@ -366,7 +377,8 @@ def config4_section(args):
     heights = pool.value_noise_heightmap(8, 3, res // 8, (res * 3) // 4)
     t_hm = time.perf_counter() - t0
     t0 = time.perf_counter()
-    pool.edit(W.terrain_edit(res, seed=3, skin=8))
+    # 16 column tiles: one whole-grid column fill would rebuild 4.5 M color leaves in one pass
+    pool.edit(W.terrain_tiles(res, 3, 4, skin=8), capi.MODE_PLAIN)
     pool.sync()
     t_build = time.perf_counter() - t0
     st = pool.stats()
@@ -456,7 +468,7 @@ def config5_section(args):
         t0 = time.perf_counter()
         h = pool.value_noise_heightmap(9, 4, res // 8, (res * 3) // 4, download=heights is None)
         heights = h if heights is None else heights
-        pool.edit(W.edit(W.COLUMN_FILL, (0, 0, 0), (res, res, 0), r2=4, colored=False, skin=8))
+        pool.edit(W.terrain_tiles(res, 4, 4, skin=8, colored=False), capi.MODE_PLAIN)
         pool.sync()
         out.setdefault("terrain_build_s", time.perf_counter() - t0)
         if "pool_after_build" not in out:
