@@ -53,6 +53,11 @@ def main():
                     help="terrain + the edit batches of the warmup and timed steps (the scene of the last "
                          "timed frame), then --frames frames and a copy of --copy-mib MiB (the byte-count "
                          "calibration of the HBM counters); for rocprofv3 runs")
+    ap.add_argument("--edit-only", action="store_true",
+                    help="terrain + the warmup batches, then a 64 MiB copy, the timed edit work (--steps batches of "
+                         "--mode, or with --mode batched the 4096-edit batch), then a --copy-mib copy: the copies "
+                         "bracket the edit kernels in a profile and calibrate its byte counters; for rocprofv3 runs")
+    ap.add_argument("--profile-gc", action="store_true", help="with --edit-only: the timed work is one GC")
     ap.add_argument("--frames", type=int, default=3)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--serve-requests", type=int, default=3000)
@@ -117,6 +122,8 @@ def main():
 
     if args.trace_only:
         return trace_only(args, pool, batches, render, npx)
+    if args.edit_only:
+        return edit_only(args, pool, batches, heights, res, mode)
 
     edit_ms, edit_gpu_ms, trace_ms, step_ms = [], [], [], []
     state1 = None
@@ -490,6 +497,36 @@ def config5_section(args):
             f"ms p99 {out[name]['latency_ms']['p99']:.2f} ms")
         pool.close()
     return out
+
+
+def edit_only(args, pool, batches, heights, res, mode):
+    """Profiling driver for the edit kernels: the warmup batches, a 64 MiB device copy (a marker), the
+    timed work, a --copy-mib copy (marker + the byte-count calibration: it reads and writes exactly
+    that many bytes). The timed work is --steps plain batches, or the 4096-edit mixed batch
+    (batched) on the terrain."""
+    from vkhashdag_amd import capi
+    from vkhashdag_amd import workloads as W
+    for b in batches[:args.warmup + (args.steps if args.profile_gc else 0)]:
+        pool.edit(b, capi.MODE_PLAIN)
+    marker = 64 << 20
+    m0, m1 = capi.DeviceBuffer(marker), capi.DeviceBuffer(marker)
+    capi.check(capi.lib().hd_device_copy(m1.ptr, m0.ptr, marker, pool.h))
+    pool.timer_begin()
+    if args.profile_gc:
+        pool.gc()
+    elif mode == capi.MODE_BATCHED:
+        work = W.mixed_batch(args.batch_edits, res, heights, 2, 256, seed=2)
+        pool.edit(work, capi.MODE_BATCHED)
+    else:
+        for b in batches[args.warmup:]:
+            pool.edit(b, capi.MODE_PLAIN)
+    ms = pool.timer_end()
+    nbytes = args.copy_mib << 20
+    src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
+    capi.check(capi.lib().hd_device_copy(dst.ptr, src.ptr, nbytes, pool.h))
+    pool.sync()
+    print(json.dumps({"mode": args.mode, "steps": args.steps, "device_ms": ms, "marker_bytes": marker,
+                      "copy_bytes": nbytes}))
 
 
 def trace_only(args, pool, batches, render, npx):

@@ -335,6 +335,8 @@ def serving_trace(n, res, heights, clients, rate_per_s, seed, rmin=2, rmax=64, e
     gaps = rng.exponential(1000.0 / rate_per_s, size=n)
     t = np.cumsum(gaps)
     out = np.zeros(n, REQUEST_DTYPE)
+    # the orbit centre's height: the median of a strided sample (the whole field can be 2^30 heights)
+    zc = float(np.median(heights[::max(1, res // 1024), ::max(1, res // 1024)])) if heights is not None else res / 2
     for i in range(n):
         out[i]["arrival_ms"] = t[i]
         out[i]["client"] = int(rng.integers(0, clients))
@@ -345,7 +347,6 @@ def serving_trace(n, res, heights, clients, rate_per_s, seed, rmin=2, rmax=64, e
             out[i]["kind"] = REQ_RENDER
             w = int(rng.integers(px_min[0], px_max[0] + 1))
             h = int(rng.integers(px_min[1], px_max[1] + 1))
-            zc = float(np.median(heights)) if heights is not None else res / 2
             vb = orbit_view((res / 2, res / 2, zc), res * 0.8, int(rng.integers(1 << 31)), w, h, res=res)
             out[i]["view"] = view_record(vb, w, h, 1 if shadow else 0)
     return out
