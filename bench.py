@@ -525,8 +525,10 @@ def edit_only(args, pool, batches, heights, res, mode):
     src, dst = capi.DeviceBuffer(nbytes), capi.DeviceBuffer(nbytes)
     capi.check(capi.lib().hd_device_copy(dst.ptr, src.ptr, nbytes, pool.h))
     pool.sync()
+    st = pool.stats()  # the work of the last hd_edit (the last plain step, or the whole batch)
     print(json.dumps({"mode": args.mode, "steps": args.steps, "device_ms": ms, "marker_bytes": marker,
-                      "copy_bytes": nbytes}))
+                      "copy_bytes": nbytes, "last_entries": st.last_entries, "last_upserts": st.last_upserts,
+                      "last_jobs": st.last_jobs, "last_passes": st.last_groups}))
 
 
 def trace_only(args, pool, batches, render, npx):

@@ -26,7 +26,8 @@ namespace hd {
 namespace {
 
 constexpr u32 kGcThreads = 256;
-constexpr u32 kScanBlock = 1024; // elements per block of the rank / bucket scans
+constexpr u32 kScanBlock = 4096; // elements per block of the rank / bucket scans
+constexpr u32 kGcGrid = 2048;    // workgroups of the grid-stride kernels (bounded dispatch)
 constexpr u32 kGcSortCap = 1024; // nodes a bucket can receive without overflowing (2-word nodes)
 
 __global__ void k_gc_fill(u32 *__restrict__ p, u64 n, u32 v) {
@@ -123,11 +124,12 @@ __global__ void __launch_bounds__(kGcThreads) k_scan_sums(u32 *__restrict__ bsum
 }
 // out[i] += the block's offset; out[n] = the total
 __global__ void k_scan_add(u32 *__restrict__ out, u64 n, const u32 *__restrict__ bsum, u32 nb) {
-	const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n)
-		out[i] += bsum[i / kScanBlock];
-	else if (i == n)
-		out[n] = bsum[nb];
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (u64)gridDim.x * blockDim.x) {
+		if (i < n)
+			out[i] += bsum[i / kScanBlock];
+		else
+			out[n] = bsum[nb];
+	}
 }
 
 // ---- forward
@@ -143,12 +145,14 @@ __global__ void k_gc_mark_roots(const u32 *__restrict__ roots, u32 n, const u32 
 	const u64 o = (u64)r - base;
 	atomicOr(&bits[o >> 5], 1u << (o & 31u));
 }
-// every live node of level l marks its children in level l + 1 (one thread per bitmap word)
+// every live node of level l marks its children in level l + 1 (one thread per bitmap word). A
+// hash-consed child can have millions of parents (the filled and uniform subtrees above all): the
+// bit is read first and the atomic issued only while it is still clear, so those parents read a
+// cached word instead of queueing atomics on one address. A stale read only costs a redundant
+// atomic: bits go from 0 to 1 only.
 __global__ void k_gc_expand(const u32 *__restrict__ words, const u32 *__restrict__ bits, u64 nw, u64 base,
-                            u64 cbase, u64 cwords, u32 *__restrict__ cbits) {
-	const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= nw)
-		return;
+                            u64 cbase, u64 cwords, u32 *cbits) {
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (u64)gridDim.x * blockDim.x)
 	for (u32 m = bits[i]; m; m &= m - 1u) {
 		const u64 ptr = base + (i << 5) + (u32)__builtin_ctz(m);
 		const u32 h = words[ptr] & 0xFFu;
@@ -156,8 +160,11 @@ __global__ void k_gc_expand(const u32 *__restrict__ words, const u32 *__restrict
 		for (u32 c = 0; c < 8; ++c)
 			if ((h >> c) & 1u) {
 				const u64 o = (u64)words[ptr + k++] - cbase;
-				if (o < cwords)
-					atomicOr(&cbits[o >> 5], 1u << (o & 31u));
+				if (o < cwords) {
+					const u32 bit = 1u << (o & 31u);
+					if (!(__hip_atomic_load(&cbits[o >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bit))
+						atomicOr(&cbits[o >> 5], bit);
+				}
 			}
 	}
 }
@@ -194,11 +201,13 @@ __global__ void k_gc_pack_leaves(u32 *__restrict__ words, u32 *__restrict__ buck
 // an inner level's used words copied out (the pack writes the pool's words in place)
 __global__ void k_gc_stage(const u32 *__restrict__ words, const u32 *__restrict__ bucket_words, DevConfig c,
                            u32 level, u32 *__restrict__ stage) {
-	const u32 b = blockIdx.x;
-	const u32 gb = c.level_base[level] + b, n = min(bucket_words[gb], c.words_per_bucket());
-	const u64 src = (u64)gb << c.wbpb(), dst = (u64)b << c.wbpb();
-	for (u32 i = threadIdx.x; i < n; i += blockDim.x)
-		stage[dst + i] = words[src + i];
+	const u32 lane = threadIdx.x & 63u, nb = 1u << c.bucket_bits[level];
+	for (u32 b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); b < nb; b += gridDim.x * (blockDim.x / 64)) {
+		const u32 gb = c.level_base[level] + b, n = min(bucket_words[gb], c.words_per_bucket());
+		const u64 src = (u64)gb << c.wbpb(), dst = (u64)b << c.wbpb();
+		for (u32 i = lane; i < n; i += 64)
+			stage[dst + i] = words[src + i];
+	}
 }
 // a live inner node's content with its children remapped (stage: the level's words, level-relative)
 struct ChildMap {
@@ -219,49 +228,42 @@ __device__ inline u32 remapped(const u32 *__restrict__ stage, u64 o, const Child
 // the new bucket of every live node (re-hash, :147-148) and the buckets' node counts
 __global__ void k_gc_key(const u32 *__restrict__ stage, const u32 *__restrict__ bits, const u32 *__restrict__ rank,
                          u64 nw, ChildMap M, DevConfig c, u32 level, u32 *__restrict__ key, u32 *__restrict__ cnt) {
-	const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= nw)
-		return;
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (u64)gridDim.x * blockDim.x) {
+	u32 m = bits[i];
+	if (!m)
+		continue;
 	u32 r = rank[i];
-	for (u32 m = bits[i]; m; m &= m - 1u, ++r) {
+	for (; m; m &= m - 1u, ++r) {
 		u32 buf[9];
 		const u32 n = remapped(stage, (i << 5) + (u32)__builtin_ctz(m), M, buf);
 		const u32 kb = murmur_span(buf, n) & ((1u << c.bucket_bits[level]) - 1u);
 		key[r] = kb;
 		atomicAdd(&cnt[kb], 1u);
 	}
+	}
 }
 // each live node's old offset into its new bucket's list (unordered within the bucket)
 __global__ void k_gc_scatter(const u32 *__restrict__ bits, const u32 *__restrict__ rank, u64 nw,
                              const u32 *__restrict__ key, const u32 *__restrict__ bstart, u32 *__restrict__ fill,
                              u32 *__restrict__ list) {
-	const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= nw)
-		return;
-	u32 r = rank[i];
-	for (u32 m = bits[i]; m; m &= m - 1u, ++r) {
-		const u32 kb = key[r];
-		list[bstart[kb] + atomicAdd(&fill[kb], 1u)] = (u32)((i << 5) + (u32)__builtin_ctz(m));
+	for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (u64)gridDim.x * blockDim.x) {
+		u32 m = bits[i];
+		if (!m)
+			continue;
+		u32 r = rank[i];
+		for (; m; m &= m - 1u, ++r) {
+			const u32 kb = key[r];
+			list[bstart[kb] + atomicAdd(&fill[kb], 1u)] = (u32)((i << 5) + (u32)__builtin_ctz(m));
+		}
 	}
 }
-// one wavefront per new bucket: its nodes sorted by old pointer (bitonic sort in LDS), placed one
-// after another with page-tail padding (lf_gc_shrink_inner_bucket's cache, :105-182, and the flush,
-// :198-227), then written in parallel; the bucket's counter is rewritten
-__global__ void __launch_bounds__(kGcThreads) k_gc_pack_inner(u32 *__restrict__ words, u32 *__restrict__ bucket_words,
-                                                              DevConfig c, u32 level, const u32 *__restrict__ stage,
-                                                              ChildMap M, const u32 *__restrict__ bits,
-                                                              const u32 *__restrict__ rank,
-                                                              const u32 *__restrict__ cnt,
-                                                              const u32 *__restrict__ bstart,
-                                                              const u32 *__restrict__ list, u32 *__restrict__ newp,
-                                                              u32 *__restrict__ meta) {
-	__shared__ u32 s_off[kGcThreads / 64][kGcSortCap];
-	__shared__ u32 s_pos[kGcThreads / 64][kGcSortCap];
-	const u32 lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-	const u32 b = blockIdx.x * (kGcThreads / 64) + w;
-	if (b >= (1u << c.bucket_bits[level]))
-		return;
-	u32 *so = s_off[w], *sp = s_pos[w];
+// one bucket of k_gc_pack_inner (one wavefront; so / sp: its LDS lists)
+__device__ inline void gc_pack_bucket(u32 *__restrict__ words, u32 *__restrict__ bucket_words, const DevConfig &c,
+                                      u32 level, const u32 *__restrict__ stage, const ChildMap &M,
+                                      const u32 *__restrict__ bits, const u32 *__restrict__ rank,
+                                      const u32 *__restrict__ cnt, const u32 *__restrict__ bstart,
+                                      const u32 *__restrict__ list, u32 *__restrict__ newp, u32 *__restrict__ meta,
+                                      u32 b, u32 lane, u32 *so, u32 *sp) {
 	const u32 gb = c.level_base[level] + b;
 	u32 m = cnt[b];
 	const u32 s0 = bstart[b];
@@ -272,6 +274,9 @@ __global__ void __launch_bounds__(kGcThreads) k_gc_pack_inner(u32 *__restrict__ 
 			newp[bit_rank(bits, rank, list[s0 + k])] = kNull;
 		m = kGcSortCap;
 	}
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier(); // the previous bucket's readers of so / sp are done
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	u32 P = 1;
 	while (P < m)
 		P <<= 1;
@@ -343,6 +348,26 @@ __global__ void __launch_bounds__(kGcThreads) k_gc_pack_inner(u32 *__restrict__ 
 	if (lane == 0)
 		bucket_words[gb] = bw;
 }
+// one wavefront per new bucket: its nodes sorted by old pointer (bitonic sort in LDS), placed one
+// after another with page-tail padding (lf_gc_shrink_inner_bucket's cache, :105-182, and the flush,
+// :198-227), then written in parallel; the bucket's counter is rewritten
+__global__ void __launch_bounds__(kGcThreads) k_gc_pack_inner(u32 *__restrict__ words, u32 *__restrict__ bucket_words,
+                                                              DevConfig c, u32 level, const u32 *__restrict__ stage,
+                                                              ChildMap M, const u32 *__restrict__ bits,
+                                                              const u32 *__restrict__ rank,
+                                                              const u32 *__restrict__ cnt,
+                                                              const u32 *__restrict__ bstart,
+                                                              const u32 *__restrict__ list, u32 *__restrict__ newp,
+                                                              u32 *__restrict__ meta) {
+	__shared__ u32 s_off[kGcThreads / 64][kGcSortCap];
+	__shared__ u32 s_pos[kGcThreads / 64][kGcSortCap];
+	const u32 lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	u32 *so = s_off[w], *sp = s_pos[w];
+	for (u32 b = blockIdx.x * (kGcThreads / 64) + w; b < (1u << c.bucket_bits[level]);
+	     b += gridDim.x * (kGcThreads / 64))
+		gc_pack_bucket(words, bucket_words, c, level, stage, M, bits, rank, cnt, bstart, list, newp, meta, b, lane, so,
+		               sp);
+}
 // roots remapped through level 0's new pointers
 __global__ void k_gc_roots(u32 *__restrict__ roots, u32 n, const u32 *__restrict__ bits, const u32 *__restrict__ rank,
                            const u32 *__restrict__ newp, u64 base, u64 words) {
@@ -391,7 +416,7 @@ template <typename F> static void scan(hipStream_t s, F f, u64 n, u32 *out, u32 
 	const u32 nb = (u32)((n + kScanBlock - 1) / kScanBlock);
 	k_scan_local<<<std::max(nb, 1u), kGcThreads, 0, s>>>(f, n, out, bsum);
 	k_scan_sums<<<1, kGcThreads, 0, s>>>(bsum, std::max(nb, 1u));
-	k_scan_add<<<grid(n + 1), kGcThreads, 0, s>>>(out, n, bsum, std::max(nb, 1u));
+	k_scan_add<<<std::min(grid(n + 1), kGcGrid), kGcThreads, 0, s>>>(out, n, bsum, std::max(nb, 1u));
 }
 
 // Queues a whole collection on stream s: roots (device, n_roots) are remapped in place, the filled
@@ -400,14 +425,14 @@ template <typename F> static void scan(hipStream_t s, F f, u64 n, u32 *out, u32 
 int gc_queue(Pool &p, const GcPlan &G, hipStream_t s, u32 *d_roots, u32 n_roots, bool has_filled) {
 	const u32 L = p.dc.L;
 	for (u32 l = 0; l < L; ++l)
-		k_gc_fill<<<std::min(grid(G.lv[l].nw + 1), 4096u), kGcThreads, 0, s>>>(G.lv[l].bits, G.lv[l].nw + 1, 0u);
+		k_gc_fill<<<std::min(grid(G.lv[l].nw + 1), kGcGrid), kGcThreads, 0, s>>>(G.lv[l].bits, G.lv[l].nw + 1, 0u);
 	k_gc_mark_roots<<<grid(n_roots + 1), kGcThreads, 0, s>>>(d_roots, n_roots, p.d_filled, has_filled ? 1u : 0u,
 	                                                          G.lv[0].base, G.lv[0].words, G.lv[0].bits);
 	for (u32 l = 0; l < L; ++l) {
 		const GcPlan::Level &V = G.lv[l];
 		scan(s, PopcountOf{V.bits}, V.nw, V.rank, G.bsum);
 		if (l + 1 < L)
-			k_gc_expand<<<grid(V.nw), kGcThreads, 0, s>>>(p.words, V.bits, V.nw, V.base, G.lv[l + 1].base,
+			k_gc_expand<<<std::min(grid(V.nw), kGcGrid), kGcThreads, 0, s>>>(p.words, V.bits, V.nw, V.base, G.lv[l + 1].base,
 			                                              G.lv[l + 1].words, G.lv[l + 1].bits);
 	}
 	for (int l = (int)L - 1; l >= 0; --l) {
@@ -420,13 +445,13 @@ int gc_queue(Pool &p, const GcPlan &G, hipStream_t s, u32 *d_roots, u32 n_roots,
 		}
 		const GcPlan::Level &C = G.lv[l + 1];
 		const ChildMap M{C.bits, C.rank, G.newp[(l + 1) & 1], C.base, C.words};
-		k_gc_stage<<<(u32)V.buckets, kGcThreads, 0, s>>>(p.words, p.bucket_words, p.dc, (u32)l, G.stage);
-		k_gc_fill<<<std::min(grid(V.buckets), 4096u), kGcThreads, 0, s>>>(G.cnt, V.buckets, 0u);
-		k_gc_fill<<<std::min(grid(V.buckets), 4096u), kGcThreads, 0, s>>>(G.fill, V.buckets, 0u);
-		k_gc_key<<<grid(V.nw), kGcThreads, 0, s>>>(G.stage, V.bits, V.rank, V.nw, M, p.dc, (u32)l, G.key, G.cnt);
+		k_gc_stage<<<std::min(grid(V.buckets * 64), kGcGrid), kGcThreads, 0, s>>>(p.words, p.bucket_words, p.dc, (u32)l, G.stage);
+		k_gc_fill<<<std::min(grid(V.buckets), kGcGrid), kGcThreads, 0, s>>>(G.cnt, V.buckets, 0u);
+		k_gc_fill<<<std::min(grid(V.buckets), kGcGrid), kGcThreads, 0, s>>>(G.fill, V.buckets, 0u);
+		k_gc_key<<<std::min(grid(V.nw), kGcGrid), kGcThreads, 0, s>>>(G.stage, V.bits, V.rank, V.nw, M, p.dc, (u32)l, G.key, G.cnt);
 		scan(s, ValueOf{G.cnt}, V.buckets, G.bstart, G.bsum);
-		k_gc_scatter<<<grid(V.nw), kGcThreads, 0, s>>>(V.bits, V.rank, V.nw, G.key, G.bstart, G.fill, G.list);
-		k_gc_pack_inner<<<grid(V.buckets * 64), kGcThreads, 0, s>>>(p.words, p.bucket_words, p.dc, (u32)l, G.stage, M,
+		k_gc_scatter<<<std::min(grid(V.nw), kGcGrid), kGcThreads, 0, s>>>(V.bits, V.rank, V.nw, G.key, G.bstart, G.fill, G.list);
+		k_gc_pack_inner<<<std::min(grid(V.buckets * 64), kGcGrid), kGcThreads, 0, s>>>(p.words, p.bucket_words, p.dc, (u32)l, G.stage, M,
 		                                                           V.bits, V.rank, G.cnt, G.bstart, G.list, newp,
 		                                                           p.meta);
 	}
