@@ -52,7 +52,9 @@ typedef struct hd_config {
 	uint64_t color_leaf_capacity;  /* max color leaf words, <= 2^30 */
 } hd_config;
 
-/* hashdag::VBRColor (VBRColor.hpp:19-44). bits_per_weight == 0: colors = RGB8 0x00BBGGRR. */
+/* hashdag::VBRColor (VBRColor.hpp:19-44). bits_per_weight == 0: colors = RGB8 0x00BBGGRR; otherwise two
+ * R5G6B5 endpoints (low, high 16 bits) and a weight. bits_per_weight is at most 3 (the chunk headers'
+ * 2-bit field, VBRColor.hpp:368-487): the edit entry points reject larger values with HD_ERR_ARG. */
 typedef struct hd_color {
 	uint32_t colors;
 	uint8_t weight;
@@ -166,6 +168,12 @@ enum {
  * (main.cpp:214-234). Blocking: returns after the device work completed. */
 int hd_edit(hd_pool *pool, uint32_t root, uint32_t color_root, const hd_edit_desc *edits, uint32_t n,
             uint32_t mode, uint32_t *root_out, uint32_t *color_root_out);
+
+/* Plans hd_edit(pool, ..., edits, n, mode) and allocates the device buffers its passes need,
+ * without editing: a later hd_edit on these edits, or on any call whose passes are no larger,
+ * allocates nothing. No counterpart in the reference (its edits allocate host memory as they go);
+ * for serving loops that must not stall on device allocation. Blocking. */
+int hd_edit_reserve(hd_pool *pool, const hd_edit_desc *edits, uint32_t n, uint32_t mode);
 
 /* Editor decisions: hashdag::EditType (Editor.hpp:18), same order. */
 enum { HD_EDIT_TYPE_NOT_AFFECTED = 0, HD_EDIT_TYPE_PROCEED = 1, HD_EDIT_TYPE_FILL = 2, HD_EDIT_TYPE_CLEAR = 3 };

@@ -117,3 +117,59 @@ def test_disjoint_edits_one_group():
     assert dev.stats().last_groups == 1
     ora.edit(e)
     assert_equivalent(dev, ora, cfg)
+
+
+@pytest.mark.parametrize("top_pairs", [4096, 64])
+def test_one_pass_long_edit_lists(top_pairs):
+    # 432 pairwise-disjoint edits form one pass: the root's list holds all of them and the upper
+    # levels' lists stay longer than a wavefront splits (16), in the one-workgroup top-down
+    # (top_pairs 4096) and in the grid kernels (top_pairs 64 sends every level after the root there)
+    capi = _capi()
+    old = capi.get_tuning()
+    try:
+        capi.set_tuning(**dict(old, top_pairs=top_pairs))
+        res = 256
+        cfg = small_cfg(res, 8, 9, 2, 3)
+        rng = np.random.default_rng(21)
+        ed = []
+        for k in range(432):
+            x, y, z = 12 + 28 * (k % 9), 12 + 28 * (k // 9 % 8), 12 + 40 * (k // 72)
+            if k % 3 == 2:  # a paint box inside the cell
+                ed.append(W.edit(W.AABB_PAINT, (x - 9, y - 9, z - 9), (x + 9, y + 9, z + 9),
+                                 color=int(rng.integers(1 << 24))))
+            else:
+                ed.append(W.edit(W.SPHERE_FILL, (x, y, z), r2=int(rng.integers(4, 120)),
+                                 color=int(rng.integers(1 << 24))))
+        e = W.edits(ed)
+        dev, ora = capi.DevicePool(cfg), O.OraclePool(cfg)
+        # a filled slab first, so the paints and fills meet existing voxels and colors
+        base = W.edits([W.edit(W.AABB_FILL, (0, 0, 0), (res, res, 100), color=0x304050)])
+        dev.edit(base)
+        ora.edit(base)
+        dev.edit(e, capi.MODE_BATCHED)
+        assert dev.stats().last_groups == 1
+        ora.edit(e)
+        assert_equivalent(dev, ora, cfg)
+    finally:
+        capi.set_tuning(**old)
+
+
+def test_reserve_changes_nothing_and_edits_match():
+    # hd_edit_reserve allocates the passes' buffers only: the pool is unchanged, and the edit that
+    # follows gives the same words as one without the reservation
+    capi = _capi()
+    cfg = small_cfg(128, 8, 9, 2, 3)
+    e = random_edits(60, 128, 17, True)
+    a, b = capi.DevicePool(cfg), capi.DevicePool(cfg)
+    before = a.live_words()
+    a.reserve(e, capi.MODE_BATCHED)
+    a.reserve(e[:5], capi.MODE_PLAIN)
+    after = a.live_words()
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+    assert a.stats().used_words == b.stats().used_words
+    a.edit(e, capi.MODE_BATCHED)
+    b.edit(e, capi.MODE_BATCHED)
+    assert (a.root, a.color_root) == (b.root, b.color_root)
+    assert np.array_equal(a.live_words()[1], b.live_words()[1])
+    assert np.array_equal(a.color_nodes(), b.color_nodes())
+    assert np.array_equal(a.color_leaves(), b.color_leaves())

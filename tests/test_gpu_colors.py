@@ -95,3 +95,26 @@ def test_weighted_colors_batched_equal_sequential(seed):
     dev.edit(e, capi.MODE_BATCHED)
     ora.edit(e)
     assert_equivalent(dev, ora, cfg)
+
+
+def test_wide_setnode_levels_plain_bit_exact():
+    # color leaves of 4^3 voxels under a 512^3 grid: a sphere of radius 200 has tens of thousands
+    # of entries on the levels above the color leaves, so SetNode runs its lower levels as grid
+    # launches (decisions before the workgroup, placement after it) and its upper ones in LDS
+    capi = _capi()
+    res = 512
+    cfg = small_cfg(res, 10, 9, 2, 2)
+    c = res // 2
+    e = W.edits([
+        W.edit(W.SPHERE_FILL, (c, c, c), r2=200 * 200, color=0x2050A0),
+        W.edit(W.SPHERE_PAINT, (c + 60, c - 40, c + 30), r2=150 * 150, color=0xA03020),
+        W.edit(W.AABB_FILL, (40, 60, 20), (300, 280, 140), color=0x10C040),
+        W.edit(W.SPHERE_DIG, (c - 50, c + 20, c + 90), r2=120 * 120),
+        W.edit(W.SPHERE_PAINT, (c, c, c - 40), r2=220 * 220, color=0x808080),
+        W.edit(W.AABB_PAINT, (100, 100, 100), (420, 400, 380), color=0x0000F0),
+    ])
+    dev, ora = capi.DevicePool(cfg), O.OraclePool(cfg)
+    for i in range(len(e)):
+        dev.edit(e[i:i + 1])
+        ora.edit(e[i:i + 1])
+        assert_same_pool(dev, ora, f"edit {i} kind {int(e[i]['kind'])}")

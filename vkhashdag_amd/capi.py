@@ -115,6 +115,7 @@ def lib():
             "hd_pool_stats": (C.c_int, [VP, C.POINTER(hd_stats)]),
             "hd_pool_clear_overflow": (C.c_int, [VP]),
             "hd_edit": (C.c_int, [VP, U32, U32, VP, U32, U32, P32, P32]),
+            "hd_edit_reserve": (C.c_int, [VP, VP, U32, U32]),
             "hd_gc": (C.c_int, [VP, VP, U32]),
             "hd_edit_host": (C.c_int, [VP, U32, C.POINTER(hd_host_editor), P32]),
             "hd_trace": (C.c_int, [VP, U32, U32, VP, U32, VP, VP]),
@@ -285,6 +286,11 @@ class DevicePool:
 
     Edit = edit
     ThreadedEdit = edit
+
+    def reserve(self, edits, mode=MODE_PLAIN):
+        """hd_edit_reserve: allocate what edit(edits, mode) would, without editing."""
+        e = np.ascontiguousarray(np.atleast_1d(edits).astype(EDIT_DTYPE))
+        check(lib().hd_edit_reserve(self.h, e.ctypes.data, len(e), mode))
 
     def gc(self, extra_roots=()):
         roots = np.asarray([self.root, *extra_roots], np.uint32)

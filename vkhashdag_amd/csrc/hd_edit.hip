@@ -853,9 +853,14 @@ struct PairOut {
 // (9 words, the pool is padded) so its mask and child arrive in one memory round trip.
 // bits: where the classification records which list edits touch the child (bit k for list entry k;
 // at least (llen + 31) / 32 words, private to the thread), for emit_pair_e; or null.
+// pre: the list edits touching the child as a wavefront counted them (coop_classify), or null to
+// test the list here.
+struct ListHits {
+	u32 cnt, single, active;
+};
 __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel &A, u64 t, const Entry &E,
                                         PushEvent *__restrict__ ev, u32 *__restrict__ pm, u32 ev_cap,
-                                        u32 *bits = nullptr) {
+                                        u32 *bits = nullptr, const ListHits *pre = nullptr) {
 	const u32 i = (u32)(t & 7u);
 	u32 old = kNull;
 	if (E.ptr != kNull) {
@@ -885,10 +890,14 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 	else {
 		cnt = 0;
 		bool active = false;
-		if (bits)
+		if (pre) {
+			cnt = pre->cnt;
+			single = pre->single;
+			active = pre->active != 0u;
+		} else if (bits)
 			for (u32 w = 0; w < (E.llen + 31u) >> 5; ++w)
 				bits[w] = 0u;
-		for (u32 k0 = 0; k0 < E.llen; k0 += kListBatch) { // list words of a batch in one round trip
+		for (u32 k0 = 0; !pre && k0 < E.llen; k0 += kListBatch) { // list words of a batch in one round trip
 			u32 ix[kListBatch];
 #pragma unroll
 			for (u32 j = 0; j < kListBatch; ++j)
@@ -948,9 +957,10 @@ __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &
 // bits 0-7, its changed terminal children in bits 8-15 and its pushed children in bits 16-23 (valid
 // on slot 0), idx / lo: the child's entry index and list offset at level l + 1.
 // bits: expand_pair_e's record of the list edits that touch the child, or null to test them again.
+// by_wave: the list is written by coop_emit (only the bound is checked here).
 __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A, const ALevel &N,
                                     u32 *__restrict__ pm, u64 t, const Entry &E, u32 kid, u32 m8, const PairOut &o,
-                                    u32 idx, u32 lo, const u32 *bits = nullptr) {
+                                    u32 idx, u32 lo, const u32 *bits = nullptr, bool by_wave = false) {
 	u32 e = (u32)(t >> 3), i = (u32)(t & 7u);
 	if (i == 0) {
 		stg(A.first_child, e, idx);
@@ -967,7 +977,7 @@ __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A,
 		if ((u64)lo + ln > N.list_cap) {
 			atomicOr(&pm[kPmOverflow], 1u);
 			ln = 0;
-		} else {
+		} else if (!by_wave) {
 			u32 w = lo;
 			for (u32 k0 = 0; k0 < E.llen; k0 += kListBatch) { // loads and tests first, then the stores
 				u32 ix[kListBatch], keep = 0;
@@ -998,11 +1008,100 @@ __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A,
 	return out;
 }
 __device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, const ALevel &N, u32 *__restrict__ pm,
-                                 u64 t, u32 m8, const PairOut &o, u32 idx, u32 lo) {
+                                 u64 t, u32 m8, const PairOut &o, u32 idx, u32 lo, bool by_wave = false) {
 	if (o.flag & 1u)
-		emit_pair_e(P, l, A, N, pm, t, A.ent[(u32)(t >> 3)], A.kids[t], m8, o, idx, lo);
+		emit_pair_e(P, l, A, N, pm, t, A.ent[(u32)(t >> 3)], A.kids[t], m8, o, idx, lo, nullptr, by_wave);
 	else
 		emit_pair_e(P, l, A, N, pm, t, Entry{}, kNull, m8, o, idx, lo);
+}
+
+// Edit lists longer than kCoopList are split by a whole wavefront: lane j takes list entries
+// j, j + 64, ... and tests each against all 8 children, so a list of n edits costs n / 64 coalesced
+// loads (kCoopBatch of them in flight) instead of n / kListBatch dependent ones per pair. The
+// callers keep the wavefront converged (the loops below are uniform).
+constexpr u32 kCoopList = 16, kCoopBatch = 4;
+__device__ inline u32 child_coord(u32 p, u32 i, u32 axis) { return (p << 1) | ((i >> axis) & 1u); }
+// Per child i of E: the list edits that touch it (cnt[i]), the last of them (single[i]), and
+// whether one of them is stateful (bit i of active), as expand_pair_e's own list loop finds them.
+struct CoopHits {
+	u32 cnt[8], single[8], active;
+};
+__device__ inline void coop_classify(const EditParams &P, u32 l, const ALevel &A, const Entry &E, u32 lane,
+                                     CoopHits &o) {
+#pragma unroll
+	for (u32 i = 0; i < 8; ++i)
+		o.cnt[i] = 0u, o.single[i] = 0u;
+	o.active = 0u;
+	const u32 cl = l + 1u;
+	for (u32 k0 = 0; k0 < E.llen; k0 += 64u * kCoopBatch) {
+		u32 ix[kCoopBatch];
+#pragma unroll
+		for (u32 j = 0; j < kCoopBatch; ++j) {
+			const u32 k = k0 + 64u * j + lane;
+			ix[j] = k < E.llen ? ldg(A.lists, (u64)E.loff + k) : 0u;
+		}
+#pragma unroll
+		for (u32 j = 0; j < kCoopBatch; ++j) {
+			if (k0 + 64u * j >= E.llen) // uniform
+				break;
+			const bool v = k0 + 64u * j + lane < E.llen;
+			const DevEditor ed = P.editor(ix[j]);
+			const bool st = ed.stateful();
+#pragma unroll
+			for (u32 i = 0; i < 8; ++i) {
+				const bool hit = v && ed.geom(cl, child_coord(E.x, i, 0), child_coord(E.y, i, 1),
+				                              child_coord(E.z, i, 2)) != kNotAffected;
+				const u64 m = __ballot(hit);
+				o.cnt[i] += (u32)__popcll(m);
+				if (m)
+					o.single[i] = __builtin_amdgcn_readlane(ix[j], 63 - (int)__builtin_clzll(m));
+				if (__ballot(hit && st))
+					o.active |= 1u << i;
+			}
+		}
+	}
+}
+// The children's lists of E in list (submission) order: child i's at N.lists[lo[i]...] when bit i
+// of need is set (a proceeding child with more than one edit whose list fits the level's bound).
+__device__ inline void coop_emit(const EditParams &P, u32 l, const ALevel &A, const ALevel &N, const Entry &E,
+                                 u32 lane, const u32 (&lo)[8], u32 need) {
+	const u32 cl = l + 1u;
+	const u64 below = (1ull << lane) - 1ull;
+	u32 run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	for (u32 k0 = 0; k0 < E.llen; k0 += 64u * kCoopBatch) {
+		u32 ix[kCoopBatch];
+#pragma unroll
+		for (u32 j = 0; j < kCoopBatch; ++j) {
+			const u32 k = k0 + 64u * j + lane;
+			ix[j] = k < E.llen ? ldg(A.lists, (u64)E.loff + k) : 0u;
+		}
+#pragma unroll
+		for (u32 j = 0; j < kCoopBatch; ++j) {
+			if (k0 + 64u * j >= E.llen)
+				break;
+			const bool v = k0 + 64u * j + lane < E.llen;
+			const DevEditor ed = P.editor(ix[j]);
+#pragma unroll
+			for (u32 i = 0; i < 8; ++i) {
+				if (!((need >> i) & 1u))
+					continue;
+				const bool hit = v && ed.geom(cl, child_coord(E.x, i, 0), child_coord(E.y, i, 1),
+				                              child_coord(E.z, i, 2)) != kNotAffected;
+				const u64 m = __ballot(hit);
+				if (hit)
+					stg(N.lists, (u64)lo[i] + run[i] + (u32)__popcll(m & below), ix[j]);
+				run[i] += (u32)__popcll(m);
+			}
+		}
+	}
+}
+__device__ inline ListHits coop_pick(const CoopHits &c, u32 i) {
+	ListHits h{0u, 0u, (c.active >> i) & 1u};
+#pragma unroll
+	for (u32 k = 0; k < 8; ++k)
+		if (k == i)
+			h.cnt = c.cnt[k], h.single = c.single[k];
+	return h;
 }
 
 // chunk of the pair range [0, 8n) owned by this block (whole nodes per block)
@@ -1021,15 +1120,18 @@ __device__ inline Pair expand_emit_range(const EditParams &P, u32 l, const ALeve
                                          u32 *__restrict__ pm, PushEvent *ev, u32 ev_cap, u64 begin, u64 end, Pair base,
                                          const u32 *__restrict__ flag, const u32 *__restrict__ lcnt,
                                          const uint4 *__restrict__ cstate, Pair *sh, u32 *sflag) {
+	const u32 lane = threadIdx.x & 63u;
 	for (u64 t0 = begin; t0 < end; t0 += kAT) {
 		u64 t = t0 + threadIdx.x;
 		bool in = t < end;
 		PairOut o{0, 0, make_uint4(0, 0, 0, 0)};
+		bool by_wave = false; // the node's list is longer than kCoopList: the wave writes its children's
 		if (in) {
 			if (flag)
 				o = PairOut{flag[t], lcnt[t], cstate[t]};
 			else
 				o = expand_pair(P, l, A, t, ev, pm, ev_cap);
+			by_wave = ldg(&A.ent[(u32)(t >> 3)].llen, 0) > kCoopList;
 		}
 		sflag[threadIdx.x] = o.flag;
 		Pair tile;
@@ -1041,7 +1143,22 @@ __device__ inline Pair expand_emit_range(const EditParams &P, u32 l, const ALeve
 					u32 f = sflag[threadIdx.x + k];
 					m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k)) | (((f >> 2) & 1u) << (16 + k));
 				}
-			emit_pair(P, l, A, N, pm, t, m8, o, base.a + off.a, base.b + off.b);
+			emit_pair(P, l, A, N, pm, t, m8, o, base.a + off.a, base.b + off.b, by_wave);
+		}
+		{ // the long lists' children, by the wave whose lanes hold the node's pairs (whole nodes per wave)
+			const u32 lo = base.b + off.b;
+			const u64 need = __ballot(in && by_wave && o.lcnt > 0u && (u64)lo + o.lcnt <= N.list_cap);
+			const u64 wnode = (t0 + (threadIdx.x & ~63u)) >> 3;
+			for (u32 e = 0; e < 8u; ++e) {
+				const u32 need8 = (u32)(need >> (8u * e)) & 0xFFu;
+				if (!need8)
+					continue;
+				u32 los[8];
+#pragma unroll
+				for (u32 i = 0; i < 8; ++i)
+					los[i] = __builtin_amdgcn_readlane(lo, (int)(8u * e + i));
+				coop_emit(P, l, A, N, A.ent[(u32)(wnode + e)], lane, los, need8);
+			}
 		}
 		base.a += tile.a;
 		base.b += tile.b;
@@ -1058,23 +1175,35 @@ __device__ inline void set_level_count(const EditParams &P, u32 l, const ALevel 
 		pm[kPmOverflow] |= 1u;
 }
 
-__device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__restrict__ roots, u32 *__restrict__ pm) {
-	const u32 root = roots[0], croot = roots[1], L = P.c.L;
+// Wave 0 (all 64 lanes): the root's edit list, 64 edits per step in submission order; lane 0 then
+// classifies the root.
+__device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__restrict__ roots, u32 *__restrict__ pm,
+                                  u32 lane) {
 	u32 cnt = 0, single = 0;
 	bool active = false;
 	if (P.n_edits == 1)
 		cnt = 1;
 	else
-		for (u32 k = 0; k < P.n_edits; ++k) {
-			DevEditor e = P.editor(k);
-			if (e.geom(0, 0, 0, 0) != kNotAffected) {
-				if (cnt < A0.list_cap)
-					A0.lists[cnt] = k;
-				++cnt;
-				single = k;
-				active |= e.stateful();
+		for (u32 k0 = 0; k0 < P.n_edits; k0 += 64u) {
+			const u32 k = k0 + lane;
+			bool hit = false, st = false;
+			if (k < P.n_edits) {
+				const DevEditor e = P.editor(k);
+				hit = e.geom(0, 0, 0, 0) != kNotAffected;
+				st = e.stateful();
 			}
+			const u64 m = __ballot(hit);
+			const u32 at = cnt + (u32)__popcll(m & ((1ull << lane) - 1ull));
+			if (hit && at < A0.list_cap)
+				A0.lists[at] = k;
+			cnt += (u32)__popcll(m);
+			if (m)
+				single = k0 + 63u - (u32)__builtin_clzll(m);
+			active |= __ballot(hit && st) != 0ull;
 		}
+	if (lane != 0)
+		return;
+	const u32 root = roots[0], croot = roots[1], L = P.c.L;
 	Classified c{kNotAffected, croot, 0u, false, Color::none()};
 	if (cnt == 1)
 		c = classify_single(P, P.editor(single), 0, 0, 0, 0, root, croot, 0u);
@@ -1128,12 +1257,14 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 		sfilled[threadIdx.x] = P.filled[threadIdx.x];
 	P.filled = sfilled;
 	__syncthreads();
-	if (threadIdx.x == 0) {
-		pm[kPmGen] = atomicAdd(gen_ctr, P.c.L) + 1u; // this pass's dedup generations: pm[kPmGen] + level
-		root_entry(P, lv[0], roots, pm);
-		scount = pm[kPmHdr];
-		if (scount)
-			sent[0][0] = lv[0].ent[0];
+	if (threadIdx.x < 64) {
+		root_entry(P, lv[0], roots, pm, threadIdx.x);
+		if (threadIdx.x == 0) {
+			pm[kPmGen] = atomicAdd(gen_ctr, P.c.L) + 1u; // this pass's dedup generations: pm[kPmGen] + level
+			scount = pm[kPmHdr];
+			if (scount)
+				sent[0][0] = lv[0].ent[0];
+		}
 	}
 	__syncthreads();
 	// levels of at most 8 entries (64 pairs) by wave 0 alone: ballots and a wave scan replace the
@@ -1150,10 +1281,25 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			PairOut o{0, 0, make_uint4(0, 0, 0, 0), kNull};
 			Entry E{};
 			u32 *bits = nullptr;
+			// long lists: the whole wave classifies each such node's children (lane 8 e + i is pair i of e)
+			ListHits hits{0u, 0u, 0u};
+			bool by_wave = false;
+			if (kLists)
+				for (u32 e = 0; e < n; ++e) {
+					const Entry Ee = cur[e];
+					if (Ee.llen <= kCoopList)
+						continue;
+					CoopHits co;
+					coop_classify(P, l, A, Ee, lane, co);
+					if ((lane >> 3) == e) {
+						hits = coop_pick(co, lane & 7u);
+						by_wave = true;
+					}
+				}
 			if (in) {
 				E = cur[lane >> 3];
-				bits = kLists && E.llen <= kTopEdits ? sbits[lane] : nullptr;
-				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap, bits);
+				bits = kLists && !by_wave && E.llen <= kTopEdits ? sbits[lane] : nullptr;
+				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap, bits, by_wave ? &hits : nullptr);
 			}
 			const u64 bp = __ballot(o.flag & 1u), bt = __ballot((o.flag >> 1) & 1u), bu = __ballot((o.flag >> 2) & 1u);
 			const u64 below = (1ull << lane) - 1ull;
@@ -1170,9 +1316,23 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 				                                      ((u32)((bu >> lane) & 0xFFu) << 16)
 				                                : 0u;
 				const u32 idx = (u32)__popcll(bp & below);
-				Entry ne = emit_pair_e(P, l, A, N, pm, lane, E, o.kid, m8, o, idx, lc - o.lcnt, bits);
+				Entry ne = emit_pair_e(P, l, A, N, pm, lane, E, o.kid, m8, o, idx, lc - o.lcnt, bits, by_wave);
 				if ((o.flag & 1u) && idx < kTopEnt)
 					nxt[idx] = ne;
+			}
+			if (kLists) { // the long lists' children, by the whole wave
+				const u32 lo = lc - o.lcnt;
+				const u64 need = __ballot(by_wave && o.lcnt > 0u && (u64)lo + o.lcnt <= N.list_cap);
+				for (u32 e = 0; e < n; ++e) {
+					const u32 need8 = (u32)(need >> (8u * e)) & 0xFFu;
+					if (!need8)
+						continue;
+					u32 los[8];
+#pragma unroll
+					for (u32 i = 0; i < 8; ++i)
+						los[i] = __builtin_amdgcn_readlane(lo, (int)(8u * e + i));
+					coop_emit(P, l, A, N, cur[e], lane, los, need8);
+				}
 			}
 			if (lane == 0)
 				set_level_count(P, l, N, pm, tot);
@@ -1199,10 +1359,27 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			PairOut o{0, 0, make_uint4(0, 0, 0, 0), kNull};
 			Entry E{};
 			u32 *bits = nullptr;
+			// long lists: each wave classifies the children of the 8 nodes whose pairs are its lanes
+			const u32 lane = threadIdx.x & 63u;
+			const u64 wnode = (t0 + (threadIdx.x & ~63u)) >> 3; // the wave's first node
+			ListHits hits{0u, 0u, 0u};
+			bool by_wave = false;
+			if (kLists)
+				for (u32 e = 0; e < 8u && wnode + e < n; ++e) {
+					const Entry Ee = cur[wnode + e];
+					if (Ee.llen <= kCoopList)
+						continue;
+					CoopHits co;
+					coop_classify(P, l, A, Ee, lane, co);
+					if ((lane >> 3) == e) {
+						hits = coop_pick(co, lane & 7u);
+						by_wave = true;
+					}
+				}
 			if (in) {
 				E = cur[t >> 3];
-				bits = kLists && E.llen <= kTopEdits ? sbits[threadIdx.x] : nullptr;
-				o = expand_pair_e(P, l, A, t, E, ev, pm, ev_cap, bits);
+				bits = kLists && !by_wave && E.llen <= kTopEdits ? sbits[threadIdx.x] : nullptr;
+				o = expand_pair_e(P, l, A, t, E, ev, pm, ev_cap, bits, by_wave ? &hits : nullptr);
 			}
 			sflag[threadIdx.x] = o.flag;
 			Pair tile;
@@ -1215,9 +1392,23 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 						m8 |= ((f & 1u) << k) | (((f >> 1) & 1u) << (8 + k)) | (((f >> 2) & 1u) << (16 + k));
 					}
 				const u32 idx = base.a + off.a;
-				Entry ne = emit_pair_e(P, l, A, N, pm, t, E, o.kid, m8, o, idx, base.b + off.b, bits);
+				Entry ne = emit_pair_e(P, l, A, N, pm, t, E, o.kid, m8, o, idx, base.b + off.b, bits, by_wave);
 				if ((o.flag & 1u) && idx < kTopEnt)
 					nxt[idx] = ne;
+			}
+			if (kLists) { // the long lists' children, by the wave that classified them
+				const u32 lo = base.b + off.b;
+				const u64 need = __ballot(by_wave && o.lcnt > 0u && (u64)lo + o.lcnt <= N.list_cap);
+				for (u32 e = 0; e < 8u && wnode + e < n; ++e) {
+					const u32 need8 = (u32)(need >> (8u * e)) & 0xFFu;
+					if (!need8)
+						continue;
+					u32 los[8];
+#pragma unroll
+					for (u32 i = 0; i < 8; ++i)
+						los[i] = __builtin_amdgcn_readlane(lo, (int)(8u * e + i));
+					coop_emit(P, l, A, N, cur[wnode + e], lane, los, need8);
+				}
 			}
 			base.a += tile.a;
 			base.b += tile.b;
@@ -1241,8 +1432,29 @@ __global__ void __launch_bounds__(kAT) a_count(EditParams P, u32 l, const ALevel
 	u64 begin, end;
 	pair_chunk(pm[kPmHdr + l], begin, end);
 	Pair acc{0, 0};
-	for (u64 t = begin + threadIdx.x; t < end; t += kAT) {
-		PairOut o = expand_pair(P, l, A, t, ev, pm, ev_cap);
+	const u32 lane = threadIdx.x & 63u;
+	for (u64 t0 = begin; t0 < end; t0 += kAT) { // uniform trip count: waves classify long lists together
+		const u64 t = t0 + threadIdx.x;
+		const bool in = t < end;
+		// nodes with lists longer than kCoopList: the wave whose lanes hold their pairs counts their
+		// children's edits (whole nodes per wave: chunks start on node boundaries)
+		const u64 wnode = (t0 + (threadIdx.x & ~63u)) >> 3;
+		const u64 longl = __ballot(in && (t & 7u) == 0 && ldg(&A.ent[(u32)(t >> 3)].llen, 0) > kCoopList);
+		ListHits hits{0u, 0u, 0u};
+		bool by_wave = false;
+		for (u32 e = 0; e < 8u; ++e) {
+			if (!((longl >> (8u * e)) & 1ull))
+				continue;
+			CoopHits co;
+			coop_classify(P, l, A, A.ent[(u32)(wnode + e)], lane, co);
+			if ((lane >> 3) == e) {
+				hits = coop_pick(co, lane & 7u);
+				by_wave = true;
+			}
+		}
+		if (!in)
+			continue;
+		PairOut o = expand_pair_e(P, l, A, t, A.ent[(u32)(t >> 3)], ev, pm, ev_cap, nullptr, by_wave ? &hits : nullptr);
 		flag[t] = o.flag;
 		lcnt[t] = o.lcnt;
 		cstate[t] = o.cs;
@@ -1868,7 +2080,7 @@ __device__ inline u32 walk_entry(const EditParams &P, u32 l, const ALevel &A, co
 
 __global__ void __launch_bounds__(kAT) a_build(EditParams P, u32 l, const ALevel *__restrict__ lv,
                                                const u32 *__restrict__ pm, GroupTable G) {
-	constexpr u32 kCap = 16384; // staging words per workgroup (64 KiB)
+	constexpr u32 kCap = 10240; // staging words per workgroup (40 KiB: one fits beside three color-job workgroups)
 	__shared__ __align__(16) u32 stage[kCap];
 	__shared__ u32 top;
 	const ALevel A = lv[l], C = lv[l + 1];
@@ -2105,20 +2317,84 @@ __global__ void a_job_ranges(const ALevel *__restrict__ lv, const u32 *__restric
 // first and marked covered (pushes cover terminal subtrees, leaves proceeding ones: disjoint); the
 // job's current colors (DAGColorPool's leaf decode, headers staged in LDS) fill the rest. Every
 // load of a phase is independent of the others in it: a job costs about three memory round trips.
-constexpr u32 kJobHdr = 4096; // staged header words (larger chunks are decoded voxel by voxel)
+constexpr u32 kJobHdr = 3072; // staged header words (larger chunks are decoded voxel by voxel); also the
+                              // largest encoded chunk staged here (larger ones go through dense)
+// A voxel's VBRColor in LDS: colors, and weight | bits_per_weight << 8 | has_value << 15 in 16 bits
+// (hd_edit accepts bits_per_weight <= 3), so that four jobs fit a CU's LDS.
+__device__ inline u32 pack_wb16(u32 wb) { return (wb & 0xFFu) | (((wb >> 8) & 0x7Fu) << 8) | (((wb >> 16) & 1u) << 15); }
+__device__ inline u32 unpack_wb16(u32 w) { return (w & 0xFFu) | (((w >> 8) & 0x7Fu) << 8) | ((w >> 15) << 16); }
+static_assert(kJobHdr <= kEncTmp, "a staged chunk fits its kEncTmp-word slot");
 struct JobLds {
-	Color col[kJobV];
+	u32 colc[kJobV];
+	unsigned short colw[kJobV];
 	union {
 		u32 hdr[kJobHdr];
-		u32 stmp[kEncTmp];
+		u32 stmp[kJobHdr];
 	};
-	u32 cov[kJobV / 32];
-	uint4 fill[kJobV / 64 + 8]; // pushes: {voxel offset, colors, wb, voxel count}
-	u32 nfill;
-	u32 lbase[kJobV / 64];      // per leaf of the job: its first voxel, or kNull without assigned colors
-	u64 lmask[kJobV / 64];
+	union { // the rebuild's records (dead once the encode starts) / the encode's wave totals
+		struct {
+			u32 cov[kJobV / 32];
+			uint4 fill[kJobV / 64 + 8]; // pushes: {voxel offset, colors, wb, voxel count}
+			u32 nfill;
+			u32 lbase[kJobV / 64];      // per leaf of the job: its first voxel, or kNull without assigned colors
+			u64 lmask[kJobV / 64];
+		};
+		EncScan sh[kEncThreads / 64];
+	};
 };
-static_assert(kEncTmp <= kJobHdr, "the chunk stage shares the header stage");
+__device__ inline void job_set(JobLds &S, u32 v, Color c) {
+	S.colc[v] = c.colors;
+	S.colw[v] = (unsigned short)pack_wb16(c.wb);
+}
+// enc_load over the job's LDS colors.
+__device__ inline void enc_load_job(const JobLds &S, u32 V, EncRegs &R) {
+	const u32 per = (V + kEncThreads - 1) / kEncThreads;
+	R.v0 = threadIdx.x * per;
+	const u32 v1 = min(V, R.v0 + per);
+	R.n = R.v0 < v1 ? v1 - R.v0 : 0u;
+	u32 pc = 0, pb = 0;
+	if (R.n && R.v0)
+		pc = S.colc[R.v0 - 1], pb = (S.colw[R.v0 - 1] >> 8) & 0x7Fu;
+	if (R.n == kEncPer) {
+		const uint4 *qc = reinterpret_cast<const uint4 *>(S.colc + R.v0);
+		const uint4 *qw = reinterpret_cast<const uint4 *>(S.colw + R.v0);
+#pragma unroll
+		for (u32 k = 0; k < kEncPer / 4; ++k) {
+			const uint4 t = qc[k];
+			R.c[4 * k] = t.x, R.c[4 * k + 1] = t.y, R.c[4 * k + 2] = t.z, R.c[4 * k + 3] = t.w;
+		}
+#pragma unroll
+		for (u32 k = 0; k < kEncPer / 8; ++k) {
+			const uint4 t = qw[k];
+			const u32 h[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+			for (u32 j = 0; j < 4; ++j) {
+				R.w[8 * k + 2 * j] = unpack_wb16(h[j] & 0xFFFFu);
+				R.w[8 * k + 2 * j + 1] = unpack_wb16(h[j] >> 16);
+			}
+		}
+	} else {
+#pragma unroll
+		for (u32 k = 0; k < kEncPer; ++k)
+			if (k < R.n) {
+				R.c[k] = S.colc[R.v0 + k];
+				R.w[k] = unpack_wb16(S.colw[R.v0 + k]);
+			}
+	}
+	R.heads = 0;
+	R.x = EncScan{0, 0};
+#pragma unroll
+	for (u32 k = 0; k < kEncPer; ++k)
+		if (k < R.n) {
+			const u32 v = R.v0 + k, b = (R.w[k] >> 8) & 0xFFu;
+			if (v == 0 || (v & 0x3FFFu) == 0 || R.c[k] != pc || b != pb)
+				R.heads |= 1u << k;
+			R.x.bits += b;
+			pc = R.c[k];
+			pb = b;
+		}
+	R.x.heads = (u32)__builtin_popcount(R.heads);
+}
 __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, const ALevel *__restrict__ lv,
                                                              u32 *__restrict__ pm, const uint4 *__restrict__ jrange,
                                                              const Color *__restrict__ lcol,
@@ -2126,7 +2402,6 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
                                                              u32 V, u32 *__restrict__ info, u32 *__restrict__ app,
                                                              u32 *__restrict__ etmp, u32 *__restrict__ rest) {
 	__shared__ JobLds S;
-	__shared__ EncScan sh[kEncThreads];
 	__shared__ u64 s_wm;
 	const u32 leafL = P.c.color_leaf, L = P.c.L, vl = P.c.voxel_level;
 	const u32 *cleaves = P.cleaves;
@@ -2192,7 +2467,7 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 		for (u32 f = 0; f < S.nfill; ++f) {
 			const uint4 q = S.fill[f];
 			for (u32 v = threadIdx.x; v < q.w; v += kEncThreads)
-				S.col[q.x + v] = Color{q.y, q.z};
+				job_set(S, q.x + v, Color{q.y, q.z});
 			if (q.w >= 32u)
 				for (u32 w = (q.x >> 5) + threadIdx.x; w < (q.x + q.w) >> 5; w += kEncThreads)
 					S.cov[w] = 0xFFFFFFFFu;
@@ -2204,7 +2479,7 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 			const u32 k = threadIdx.x + r * kEncThreads, li = k >> 6, lane = k & 63u;
 			if (li < nl && S.lbase[li] != kNull && ((S.lmask[li] >> lane) & 1ull)) {
 				const u32 at = S.lbase[li] | lane;
-				S.col[at] = c[r];
+				job_set(S, at, c[r]);
 				atomicOr(&S.cov[at >> 5], 1u << (at & 31u));
 			}
 		}
@@ -2214,11 +2489,11 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 			const Color fill = chunk ? Color::none() : get_fill(E.octree);
 			for (u32 v = threadIdx.x; v < V; v += kEncThreads)
 				if (!((S.cov[v >> 5] >> (v & 31u)) & 1u))
-					S.col[v] = fill;
+					job_set(S, v, fill);
 		} else if (!staged_hdr) {
 			for (u32 v = threadIdx.x; v < V; v += kEncThreads)
 				if (!((S.cov[v >> 5] >> (v & 31u)) & 1u))
-					S.col[v] = chunk_voxel(cleaves, idx, v);
+					job_set(S, v, chunk_voxel(cleaves, idx, v));
 		} else {
 			const u32 *mac = S.hdr, *hd = S.hdr + 2u * M, wo = idx + 4u + 2u * M + 2u * B0;
 			for (u32 v0 = threadIdx.x * kDenseRun; v0 < V; v0 += kEncThreads * kDenseRun) {
@@ -2253,18 +2528,18 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 						w &= (1u << bpw) - 1u;
 						c = Color{hc, w | (bpw << 8) | (1u << 16)};
 					}
-					S.col[v] = c;
+					job_set(S, v, c);
 				}
 			}
 		}
 		__syncthreads();
 		// encode: a_encode_size's register path over the LDS colors
 		EncRegs R;
-		enc_load(S.col, V, R);
+		enc_load_job(S, V, R);
 		EncScan tot;
-		EncScan run = block_excl_scan(R.x, sh, tot);
+		EncScan run = block_excl_scan(R.x, S.sh, tot);
 		const u32 B = tot.heads, W = (tot.bits + 31u) >> 5, ds = 6u + 2u * B + W;
-		const bool staged = ds <= kEncTmp; // block-uniform
+		const bool staged = ds <= kJobHdr; // block-uniform
 		if (staged) {
 			const u32 bo = 6u, wo = bo + 2u * B;
 			for (u32 w = threadIdx.x; w < W; w += kEncThreads)
@@ -2301,7 +2576,7 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 		} else { // a_encode_rest encodes it from the dense colors
 			Color *out = dense + (u64)e * V;
 			for (u32 v = threadIdx.x; v < V; v += kEncThreads)
-				out[v] = S.col[v];
+				out[v] = Color{S.colc[v], unpack_wb16(S.colw[v])};
 		}
 		if (threadIdx.x == 0) {
 			u32 as = (ds & 1u) ? ds + 1u : ds;
@@ -2323,17 +2598,18 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 	}
 }
 
-// single block: exclusive prefix of the append sizes in entry (DFS) order
-// Exclusive scan of the jobs' encoded sizes, one workgroup: 1024 threads x 8 consecutive jobs per
-// tile (a running sum in registers, a wave scan, the waves' totals through LDS).
-constexpr u32 kScanThreads = 1024, kScanPer = 8;
-__global__ void __launch_bounds__(kScanThreads) a_scan_app(u32 leafL, u32 *__restrict__ pm,
-                                                           const u32 *__restrict__ app, u32 *__restrict__ appo) {
+// Exclusive scan of the jobs' encoded sizes in entry (DFS) order, in tiles of kScanThreads x
+// kScanPer jobs: a_scan_app scans each tile in one workgroup (1024 threads x 8 consecutive jobs: a
+// running sum in registers, a wave scan, the waves' totals through LDS) and records its total;
+// a_scan_fix adds the totals of the tiles before each job's and writes the grand total.
+constexpr u32 kScanThreads = 1024, kScanPer = 8, kScanTile = kScanThreads * kScanPer;
+__global__ void __launch_bounds__(kScanThreads) a_scan_app(u32 leafL, const u32 *__restrict__ pm,
+                                                           const u32 *__restrict__ app, u32 *__restrict__ appo,
+                                                           u32 *__restrict__ tsum) {
 	__shared__ u32 swave[kScanThreads / 64];
 	const u32 n = pm[kPmHdr + leafL], lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-	u32 run = 0;
-	for (u32 b = 0; b < n; b += kScanThreads * kScanPer) {
-		const u32 first = b + threadIdx.x * kScanPer;
+	for (u32 t = blockIdx.x; (u64)t * kScanTile < n; t += gridDim.x) {
+		const u32 first = t * kScanTile + threadIdx.x * kScanPer;
 		u32 v[kScanPer], sum = 0;
 #pragma unroll
 		for (u32 k = 0; k < kScanPer; ++k) {
@@ -2353,22 +2629,37 @@ __global__ void __launch_bounds__(kScanThreads) a_scan_app(u32 leafL, u32 *__res
 		u32 pre = 0, tot = 0;
 #pragma unroll
 		for (u32 k = 0; k < kScanThreads / 64; ++k) {
-			const u32 t = swave[k];
-			pre += k < w ? t : 0u;
-			tot += t;
+			const u32 s = swave[k];
+			pre += k < w ? s : 0u;
+			tot += s;
 		}
-		u32 o = run + pre + x - sum;
+		u32 o = pre + x - sum;
 #pragma unroll
 		for (u32 k = 0; k < kScanPer; ++k) {
 			if (first + k < n)
 				appo[first + k] = o;
 			o += v[k];
 		}
-		run += tot;
+		if (threadIdx.x == 0)
+			tsum[t] = tot;
 		__syncthreads(); // swave reuse
 	}
-	if (threadIdx.x == 0)
-		pm[kPmLeafTotal] = run;
+}
+__global__ void a_scan_fix(u32 leafL, u32 *__restrict__ pm, u32 *__restrict__ appo, const u32 *__restrict__ tsum) {
+	const u32 n = pm[kPmHdr + leafL], nt = (u32)(((u64)n + kScanTile - 1) / kScanTile);
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		u32 total = 0;
+		for (u32 t = 0; t < nt; ++t)
+			total += tsum[t];
+		pm[kPmLeafTotal] = total;
+	}
+	for (u32 i = kScanTile + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const u32 t = i / kScanTile;
+		u32 pre = 0;
+		for (u32 k = 0; k < t; ++k)
+			pre += ldg(tsum, k);
+		appo[i] += pre;
+	}
 }
 __global__ void __launch_bounds__(kEncThreads) a_encode_write(EditParams P, const ALevel *__restrict__ lv,
                                                                const u32 *__restrict__ pm, const Color *__restrict__ dense,
@@ -2540,121 +2831,159 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_rest(EditParams P, const
 	}
 }
 
-// SetNode (k_setnode_a/b/c) for every level above the color leaves, in one workgroup: pass A
-// bottom-up decisions + subtree append counts, pass B top-down post-order indices, pass C
-// bottom-up writes; then the pool counters and the color root.
-// The 8 children's color results of entry e with every load issued before any is used: proceeding
-// children from the next level's results (entries first_child + rank), the others from okids.
-__device__ inline void color_children(const ALevel &A, const ALevel &C, u32 e, u32 (&ch)[8]) {
+// SetNode (DAGColorPool.hpp:139-171) above the color leaves, level by level: bottom-up, each
+// entry's decision (its new octree state; kColorNewNode when it appends a node) and the color
+// nodes its subtree appends; then top-down, post-order indices (a node after its children's
+// subtrees: a new child's index is its base + count - 1) and every appended node written with its
+// children's final pointers. The per-entry steps below take the child level's decisions through
+// an accessor, so the same code serves levels in LDS, levels in global memory and grid launches.
+//
+// Decision of entry e of level l. child(ci, c) returns proceeding child ci's decision and adds its
+// append count to c.
+template <class Child>
+__device__ inline void setnode_decide(const EditParams &P, const ALevel &A, const ALevel &C, u32 e, Child child,
+                                      u32 &r_out, u32 &c_out) {
+	const Entry E = ldg(A.ent, e);
+	u32 r = E.octree, c = 0;
+	if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
+		const u32 m = ldg(A.pmask, e) & 0xFFu, fc = ldg(A.first_child, e);
+		u32 ch[8];
+		bool all_null = true, same_color;
+		for (u32 i = 0; i < 8; ++i) {
+			if ((m >> i) & 1u) {
+				const u32 ci = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
+				ch[i] = ci >= C.cap ? kColorNull : child(ci, c);
+			} else
+				ch[i] = ldg(A.okids, (u64)e * 8u + i);
+			all_null = all_null && ctag(ch[i]) == kTagNull;
+		}
+		same_color = ctag(ch[0]) == kTagColor;
+		for (u32 i = 1; i < 8 && same_color; ++i)
+			same_color = ch[i] == ch[0];
+		if (all_null)
+			r = kColorNull;
+		else if (same_color)
+			r = ch[0];
+		else {
+			bool unchanged = ctag(E.octree) == kTagNode;
+			for (u32 i = 0; i < 8 && unchanged; ++i)
+				unchanged = P.cnodes[(u64)cdata(E.octree) * 8u + i] == ch[i];
+			r = unchanged ? E.octree : kColorNewNode;
+		}
+		c += r == kColorNewNode ? 1u : 0u;
+	}
+	r_out = r;
+	c_out = c;
+}
+// Top-down step of entry e (first post-order index run, decision r): place(ci, run) hands proceeding
+// child ci its base, advances run past its subtree and returns its final pointer. Writes the
+// entry's node if it is new and returns the entry's final pointer.
+template <class Place>
+__device__ inline u32 setnode_place(const EditParams &P, const ALevel &A, const ALevel &C, u32 e, u32 run, u32 r,
+                                    u64 node_cap, Place place) {
 	const u32 m = ldg(A.pmask, e) & 0xFFu, fc = ldg(A.first_child, e);
-	const uint4 o0 = ldg(reinterpret_cast<const uint4 *>(A.okids), 2 * (u64)e);
-	const uint4 o1 = ldg(reinterpret_cast<const uint4 *>(A.okids), 2 * (u64)e + 1);
-	const u32 ok[8] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, o1.z, o1.w};
-#pragma unroll
+	u32 ch[8];
 	for (u32 i = 0; i < 8; ++i) {
-		const u32 c = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
-		const u32 v = ldg(C.ores, min(c, C.cap - 1u));
-		ch[i] = ((m >> i) & 1u) ? (c < C.cap ? v : kColorNull) : ok[i];
+		if ((m >> i) & 1u) {
+			const u32 ci = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
+			ch[i] = ci >= C.cap ? kColorNull : place(ci, run);
+		} else
+			ch[i] = ldg(A.okids, (u64)e * 8u + i);
+	}
+	if (r == kColorNewNode) {
+		if ((u64)run < node_cap) {
+			for (u32 i = 0; i < 8; ++i)
+				P.cnodes[(u64)run * 8u + i] = ch[i];
+			r = cmake(kTagNode, run);
+		} else {
+			r = ldg(&A.ent[e].octree, 0);
+			atomicOr(&P.meta[kMetaOverflow], 2u);
+		}
+	}
+	return r;
+}
+// A new child's final pointer from its base + count (its own index is the last of its subtree).
+__device__ inline u32 setnode_child_final(const ALevel &C, u32 ci, u32 cr, u32 end, u64 node_cap) {
+	if (cr != kColorNewNode)
+		return cr;
+	const u32 idx = end - 1u;
+	return (u64)idx < node_cap ? cmake(kTagNode, idx) : ldg(&C.ent[ci].octree, 0);
+}
+// Child accessors for a child level kept in global memory (ores / ccnt / cbase); the color leaves'
+// results are final (no count, no base).
+struct SetNodeGlobal {
+	const ALevel &C;
+	bool leaf;
+	u64 node_cap;
+	__device__ u32 operator()(u32 ci, u32 &c) const {
+		if (!leaf)
+			c += ldg(C.ccnt, ci);
+		return ldg(C.ores, ci);
+	}
+	__device__ u32 place(u32 ci, u32 &run) const {
+		const u32 cr = ldg(C.ores, ci);
+		if (leaf)
+			return cr;
+		C.cbase[ci] = run;
+		run += ldg(C.ccnt, ci);
+		return setnode_child_final(C, ci, cr, run, node_cap);
+	}
+};
+// Grid launches for the large levels [lb, leafL): the decisions bottom-up (one launch per level,
+// before the single workgroup), then the placement top-down (after it).
+__global__ void a_setnode_up(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm, u32 l) {
+	const ALevel A = lv[l], C = lv[l + 1];
+	const u32 n = pm[kPmHdr + l];
+	const SetNodeGlobal g{C, l + 1u == P.c.color_leaf, 0};
+	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+		u32 r, c;
+		setnode_decide(P, A, C, e, g, r, c);
+		A.ores[e] = r;
+		A.ccnt[e] = c;
 	}
 }
-__device__ inline u32 color_child_result(const ALevel &A, const ALevel &C, u32 e, u32 i) {
-	u32 m = A.pmask[e] & 0xFFu;
-	if ((m >> i) & 1u) {
-		u32 c = A.first_child[e] + (u32)__builtin_popcount(m & ((1u << i) - 1u));
-		return c < C.cap ? C.ores[c] : kColorNull;
-	}
-	return A.okids[(u64)e * 8u + i];
+__global__ void a_setnode_down(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm, u32 l,
+                               u64 node_cap) {
+	const ALevel A = lv[l], C = lv[l + 1];
+	const u32 n = pm[kPmHdr + l];
+	const SetNodeGlobal g{C, l + 1u == P.c.color_leaf, node_cap};
+	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+		setnode_place(P, A, C, e, ldg(A.cbase, e), ldg(A.ores, e), node_cap,
+		              [&](u32 ci, u32 &run) { return g.place(ci, run); });
 }
+// The levels [0, lb) in one workgroup with their decisions in global memory (the fallback when
+// they do not fit LDS); the levels from lb on are in global memory already.
 __device__ inline void setnode_block(const EditParams &P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
-                                     u32 *__restrict__ roots, u64 node_cap) {
+                                     u32 *__restrict__ roots, u64 node_cap, u32 lb) {
 	const u32 leafL = P.c.color_leaf;
-	for (int l = (int)leafL - 1; l >= 0; --l) { // pass A
+	for (int l = (int)lb - 1; l >= 0; --l) {
 		const ALevel A = lv[l], C = lv[l + 1];
 		const u32 n = pm[kPmHdr + l];
+		const SetNodeGlobal g{C, (u32)l + 1u == leafL, node_cap};
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			Entry E = A.ent[e];
-			u32 r = E.octree, c = 0;
-			if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
-				u32 ch[8];
-				bool all_null = true, same_color;
-				color_children(A, C, e, ch);
-				for (u32 i = 0; i < 8; ++i)
-					all_null = all_null && ctag(ch[i]) == kTagNull;
-				same_color = ctag(ch[0]) == kTagColor;
-				for (u32 i = 1; i < 8 && same_color; ++i)
-					same_color = ch[i] == ch[0];
-				if (all_null)
-					r = kColorNull;
-				else if (same_color)
-					r = ch[0];
-				else {
-					bool unchanged = ctag(E.octree) == kTagNode;
-					for (u32 i = 0; i < 8 && unchanged; ++i)
-						unchanged = P.cnodes[(u64)cdata(E.octree) * 8u + i] == ch[i];
-					r = unchanged ? E.octree : kColorNewNode;
-				}
-				if (l + 1 < (int)leafL) {
-					u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
-					for (u32 k = 0; k < (u32)__builtin_popcount(m); ++k)
-						if (fc + k < C.cap)
-							c += C.ccnt[fc + k];
-				}
-				c += r == kColorNewNode ? 1u : 0u;
-			}
-			A.ccnt[e] = c;
+			u32 r, c;
+			setnode_decide(P, A, C, e, g, r, c);
 			A.ores[e] = r;
+			A.ccnt[e] = c;
 		}
 		__syncthreads();
 	}
-	const bool any = leafL > 0 && pm[kPmHdr] > 0;
-	if (threadIdx.x == 0 && any)
+	if (threadIdx.x == 0 && pm[kPmHdr] > 0)
 		lv[0].cbase[0] = P.meta[kMetaColorNodes];
 	__syncthreads();
-	for (u32 l = 0; l < leafL; ++l) { // pass B
+	for (u32 l = 0; l < lb; ++l) {
 		const ALevel A = lv[l], C = lv[l + 1];
 		const u32 n = pm[kPmHdr + l];
+		const SetNodeGlobal g{C, l + 1u == leafL, node_cap};
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			u32 run = A.cbase[e];
-			if (l + 1 < leafL) {
-				u32 m = A.pmask[e] & 0xFFu, fc = A.first_child[e];
-				for (u32 k = 0; k < (u32)__builtin_popcount(m); ++k)
-					if (fc + k < C.cap) {
-						C.cbase[fc + k] = run;
-						run += C.ccnt[fc + k];
-					}
+			const u32 r = setnode_place(P, A, C, e, A.cbase[e], A.ores[e], node_cap,
+			                            [&](u32 ci, u32 &run) { return g.place(ci, run); });
+			if (l == 0 && e == 0) {
+				roots[1] = r;
+				P.meta[kMetaColorNodes] += A.ccnt[0];
 			}
-			A.own[e] = run;
 		}
 		__syncthreads();
-	}
-	for (int l = (int)leafL - 1; l >= 0; --l) { // pass C
-		const ALevel A = lv[l], C = lv[l + 1];
-		const u32 n = pm[kPmHdr + l];
-		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			if (A.ores[e] != kColorNewNode)
-				continue;
-			u32 idx = A.own[e], r;
-			if ((u64)idx < node_cap) {
-				u32 ch[8];
-				color_children(A, C, e, ch);
-				for (u32 i = 0; i < 8; ++i)
-					P.cnodes[(u64)idx * 8u + i] = ch[i];
-				r = cmake(kTagNode, idx);
-			} else {
-				r = A.ent[e].octree;
-				atomicOr(&P.meta[kMetaOverflow], 2u);
-			}
-			A.ores[e] = r;
-		}
-		__syncthreads();
-	}
-	if (threadIdx.x == 0) {
-		if (pm[kPmHdr] > 0) {
-			if (leafL > 0)
-				P.meta[kMetaColorNodes] += lv[0].ccnt[0];
-			roots[1] = lv[0].ores[0];
-		}
-		P.meta[kMetaColorLeaves] += pm[kPmLeafTotal];
 	}
 }
 
@@ -2781,53 +3110,24 @@ __device__ inline void small_walk_bucket(const EditParams &P, bool leaf, BottomL
 		atomicOr(&P.meta[kMetaOverflow], 1u);
 }
 
-// SetNode with its decisions and post-order indices in LDS: pass A bottom-up (decision + subtree
-// append counts), then one top-down pass that hands out the indices and writes every appended
-// node with its children's final pointers (a new child's index is its base + count - 1).
+// SetNode for the levels [0, lb) with their decisions, counts and bases in LDS (level lb and
+// below, when lb < color_leaf, come from global memory: the grid launches' levels).
 __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
-                                   u32 *__restrict__ roots, u64 node_cap, ColorLds &S) {
+                                   u32 *__restrict__ roots, u64 node_cap, ColorLds &S, u32 lb) {
 	const u32 leafL = P.c.color_leaf;
-	for (int l = (int)leafL - 1; l >= 0; --l) { // pass A
+	for (int l = (int)lb - 1; l >= 0; --l) { // decisions, bottom-up
 		const ALevel A = lv[l], C = lv[l + 1];
 		const u32 n = pm[kPmHdr + l], o = S.off[l], oc = S.off[l + 1];
-		const bool cl = (u32)l + 1 == leafL;
+		const SetNodeGlobal g{C, (u32)l + 1u == leafL, node_cap};
+		const bool cg = (u32)l + 1u >= lb;
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			Entry E = ldg(A.ent, e);
-			u32 r = E.octree, c = 0;
-			if (!(E.sf & kSfFinal) && (E.sf & kSfActive)) {
-				const u32 m = ldg(A.pmask, e) & 0xFFu, fc = ldg(A.first_child, e);
-				u32 ch[8];
-				bool all_null = true, same_color;
-				for (u32 i = 0; i < 8; ++i) {
-					if ((m >> i) & 1u) {
-						u32 ci = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
-						if (ci >= C.cap)
-							ch[i] = kColorNull;
-						else if (cl)
-							ch[i] = ldg(C.ores, ci);
-						else {
-							ch[i] = S.r[oc + ci];
-							c += S.cnt[oc + ci];
-						}
-					} else
-						ch[i] = ldg(A.okids, (u64)e * 8u + i);
-					all_null = all_null && ctag(ch[i]) == kTagNull;
-				}
-				same_color = ctag(ch[0]) == kTagColor;
-				for (u32 i = 1; i < 8 && same_color; ++i)
-					same_color = ch[i] == ch[0];
-				if (all_null)
-					r = kColorNull;
-				else if (same_color)
-					r = ch[0];
-				else {
-					bool unchanged = ctag(E.octree) == kTagNode;
-					for (u32 i = 0; i < 8 && unchanged; ++i)
-						unchanged = P.cnodes[(u64)cdata(E.octree) * 8u + i] == ch[i];
-					r = unchanged ? E.octree : kColorNewNode;
-				}
-				c += r == kColorNewNode ? 1u : 0u;
-			}
+			u32 r, c;
+			setnode_decide(P, A, C, e, [&](u32 ci, u32 &cc) {
+				if (cg)
+					return g(ci, cc);
+				cc += S.cnt[oc + ci];
+				return S.r[oc + ci];
+			}, r, c);
 			S.r[o + e] = r;
 			S.cnt[o + e] = c;
 		}
@@ -2836,55 +3136,26 @@ __device__ inline void setnode_lds(const EditParams &P, const ALevel *__restrict
 	if (threadIdx.x == 0 && pm[kPmHdr] > 0)
 		S.base[S.off[0]] = P.meta[kMetaColorNodes];
 	__syncthreads();
-	for (u32 l = 0; l < leafL; ++l) { // top-down: indices + writes
+	for (u32 l = 0; l < lb; ++l) { // top-down: indices + writes
 		const ALevel A = lv[l], C = lv[l + 1];
 		const u32 n = pm[kPmHdr + l], o = S.off[l], oc = S.off[l + 1];
-		const bool cl = l + 1 == leafL;
+		const SetNodeGlobal g{C, l + 1u == leafL, node_cap};
+		const bool cg = l + 1u >= lb;
 		for (u32 e = threadIdx.x; e < n; e += blockDim.x) {
-			u32 run = S.base[o + e], r = S.r[o + e];
-			const u32 m = ldg(A.pmask, e) & 0xFFu, fc = ldg(A.first_child, e);
-			u32 ch[8];
-			for (u32 i = 0; i < 8; ++i) {
-				if ((m >> i) & 1u) {
-					u32 ci = fc + (u32)__builtin_popcount(m & ((1u << i) - 1u));
-					if (ci >= C.cap)
-						ch[i] = kColorNull;
-					else if (cl)
-						ch[i] = ldg(C.ores, ci);
-					else {
-						u32 cr = S.r[oc + ci], cc = S.cnt[oc + ci];
-						S.base[oc + ci] = run;
-						run += cc;
-						if (cr == kColorNewNode) {
-							u32 idx = run - 1u;
-							cr = (u64)idx < node_cap ? cmake(kTagNode, idx) : ldg(&C.ent[ci].octree, 0);
-						}
-						ch[i] = cr;
-					}
-				} else
-					ch[i] = ldg(A.okids, (u64)e * 8u + i);
-			}
-			if (r == kColorNewNode) {
-				if ((u64)run < node_cap) {
-					for (u32 i = 0; i < 8; ++i)
-						P.cnodes[(u64)run * 8u + i] = ch[i];
-					r = cmake(kTagNode, run);
-				} else {
-					r = ldg(&A.ent[e].octree, 0);
-					atomicOr(&P.meta[kMetaOverflow], 2u);
-				}
-			}
+			const u32 r = setnode_place(P, A, C, e, S.base[o + e], S.r[o + e], node_cap, [&](u32 ci, u32 &run) {
+				if (cg)
+					return g.place(ci, run);
+				const u32 cr = S.r[oc + ci];
+				S.base[oc + ci] = run;
+				run += S.cnt[oc + ci];
+				return setnode_child_final(C, ci, cr, run, node_cap);
+			});
 			if (l == 0 && e == 0) {
 				roots[1] = r;
 				P.meta[kMetaColorNodes] += S.cnt[o];
 			}
 		}
 		__syncthreads();
-	}
-	if (threadIdx.x == 0) {
-		if (leafL == 0 && pm[kPmHdr] > 0)
-			roots[1] = lv[0].ores[0];
-		P.meta[kMetaColorLeaves] += pm[kPmLeafTotal];
 	}
 }
 
@@ -3306,21 +3577,28 @@ __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__re
 // beside the node levels.
 constexpr u32 kSetNodeThreads = 1024;
 __global__ void __launch_bounds__(kSetNodeThreads) a_setnode(EditParams P, const ALevel *__restrict__ lv,
-                                                 const u32 *__restrict__ pm, u32 *__restrict__ roots, u64 node_cap) {
+                                                 const u32 *__restrict__ pm, u32 *__restrict__ roots, u64 node_cap,
+                                                 u32 lb) {
 	__shared__ ColorLds Cs;
+	const u32 leafL = P.c.color_leaf;
 	if (threadIdx.x == 0) {
 		u32 o = 0;
-		for (u32 l = 0; l <= P.c.color_leaf; ++l) {
+		for (u32 l = 0; l <= lb; ++l) {
 			Cs.off[l] = o;
-			o += l < P.c.color_leaf ? pm[kPmHdr + l] : 0u;
+			o += l < lb ? pm[kPmHdr + l] : 0u;
 		}
 		Cs.fits = o <= kColorLds ? 1u : 0u;
 	}
 	__syncthreads();
 	if (Cs.fits)
-		setnode_lds(P, lv, pm, roots, node_cap, Cs);
+		setnode_lds(P, lv, pm, roots, node_cap, Cs, lb);
 	else
-		setnode_block(P, lv, pm, roots, node_cap);
+		setnode_block(P, lv, pm, roots, node_cap, lb);
+	if (threadIdx.x == 0) {
+		if (leafL == 0 && pm[kPmHdr] > 0)
+			roots[1] = lv[0].ores[0];
+		P.meta[kMetaColorLeaves] += pm[kPmLeafTotal];
+	}
 }
 
 __global__ void a_set_roots(u32 *roots, u32 root, u32 croot, u32 *acc) {
@@ -3861,6 +4139,7 @@ struct AsyncBufs {
 	ALevel *d_lv = nullptr;
 	u32 *pm = nullptr, *roots = nullptr, *acc = nullptr, *bcnt = nullptr, *head = nullptr, *blist = nullptr;
 	u32 *flag = nullptr, *lcnt = nullptr, *info = nullptr, *app = nullptr, *appo = nullptr, *etmp = nullptr, *rest = nullptr;
+	u32 *tsum = nullptr; // per-tile totals of the jobs' append scan
 	uint4 *cstate = nullptr, *jrange = nullptr;
 	Pair *bsum = nullptr;
 	PushEvent *ev = nullptr;
@@ -3944,6 +4223,7 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		HD_RET(p.scratch.get<u32>(pre + "info", 4 * jobs, &B.info));
 		HD_RET(p.scratch.get<u32>(pre + "app", jobs, &B.app));
 		HD_RET(p.scratch.get<u32>(pre + "appo", jobs, &B.appo));
+		HD_RET(p.scratch.get<u32>(pre + "tsum", (jobs + kScanTile - 1) / kScanTile, &B.tsum));
 		HD_RET(p.scratch.get<u32>(pre + "etmp", jobs * kEncTmp, &B.etmp));
 		HD_RET(p.scratch.get<u32>(pre + "rest", jobs, &B.rest));
 		HD_RET(p.scratch.get<uint4>(pre + "jrange", 2 * jobs, &B.jrange));
@@ -4033,6 +4313,22 @@ static u32 grid_for(u64 items, u32 per_block, u32 max_blocks) {
 	return (u32)std::max<u64>(1, std::min<u64>(div_up(std::max<u64>(items, 1), per_block), max_blocks));
 }
 
+// SetNode: the levels above the color leaves whose bound exceeds kSetNodeWide entries run as grid
+// launches (their decisions before the single workgroup, their placement after it); the levels
+// above them stay in the workgroup.
+constexpr u64 kSetNodeWide = 4096;
+static void queue_setnode(const Pool &p, const PassPlan &pp, const EditParams &P, const AsyncBufs &B, hipStream_t s) {
+	const u32 leafL = p.dc.color_leaf;
+	u32 lb = leafL;
+	while (lb > 1 && pp.cap[lb - 1] > kSetNodeWide)
+		--lb;
+	for (u32 l = leafL; l-- > lb;)
+		a_setnode_up<<<grid_for(pp.cap[l], 256, 4096), 256, 0, s>>>(P, B.d_lv, B.pm, l);
+	a_setnode<<<1, kSetNodeThreads, 0, s>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap, lb);
+	for (u32 l = lb; l < leafL; ++l)
+		a_setnode_down<<<grid_for(pp.cap[l], 256, 4096), 256, 0, s>>>(P, B.d_lv, B.pm, l, p.cnode_cap);
+}
+
 // Queues one sync-free pass (no host synchronization).
 static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = true) {
 	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
@@ -4059,7 +4355,7 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	// them) has no color leaf job, yet the octree above the leaves may change (a fill color at a
 	// proceeding node, cleared children): SetNode alone, once the top-down has classified them.
 	if (pp.stateful && !colors && leafL > 0)
-		a_setnode<<<1, kSetNodeThreads, 0, p.stream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
+		queue_setnode(p, pp, P, B, p.stream);
 	const bool ljobs = lds_jobs(p.dc);
 	if (colors && !ljobs) { // the jobs' dense colors on the color stream, beside the leaves
 		HD_TRY(hipEventRecord(p.ev_top, p.stream));
@@ -4096,12 +4392,14 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 			a_encode_size<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.app, B.etmp,
 			                                                B.rest);
 		}
-		a_scan_app<<<1, kScanThreads, 0, p.cstream>>>(leafL, B.pm, B.app, B.appo);
+		a_scan_app<<<grid_for(pp.cap[leafL], kScanTile, 1024), kScanThreads, 0, p.cstream>>>(leafL, B.pm, B.app,
+		                                                                                    B.appo, B.tsum);
+		a_scan_fix<<<grid_for(pp.cap[leafL], 256 * 8, 1024), 256, 0, p.cstream>>>(leafL, B.pm, B.appo, B.tsum);
 		a_encode_write<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo, p.cleaf_cap,
 		                                                 B.etmp);
 		a_encode_rest<<<std::min(G, 1024u), kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info, B.appo,
 		                                                                 p.cleaf_cap, B.rest);
-		a_setnode<<<1, kSetNodeThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.roots, p.cnode_cap);
+		queue_setnode(p, pp, P, B, p.cstream);
 		HD_TRY(hipEventRecord(p.ev_color, p.cstream));
 		return HD_OK;
 	};
@@ -4188,11 +4486,11 @@ static int edit_sequence(Pool &p, u32 root, u32 croot, const std::vector<hd_edit
 	return HD_OK;
 }
 
-int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
+// The passes of a batched call: dependency groups (schedule_groups), split at batch_pass_edits.
+static void batched_plans(const Pool &p, const hd_edit_desc *edits, u32 n, std::vector<hd_edit_desc> &flat,
+                          std::vector<PassPlan> &plans) {
 	std::vector<std::vector<u32>> groups;
 	schedule_groups(edits, n, 1u << p.dc.voxel_level, groups);
-	std::vector<hd_edit_desc> flat;
-	std::vector<PassPlan> plans;
 	flat.reserve(n);
 	// a group's edits are pairwise disjoint, so any split of it into consecutive passes applies them
 	// in an order they commute under
@@ -4206,19 +4504,51 @@ int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n,
 				flat.push_back(edits[g[j]]);
 			plans.push_back(pp);
 		}
+}
+static void plain_plans(const hd_edit_desc *edits, u32 n, std::vector<hd_edit_desc> &flat,
+                        std::vector<PassPlan> &plans) {
+	flat.assign(edits, edits + n);
+	plans.assign(n, PassPlan{});
+	for (u32 i = 0; i < n; ++i)
+		plans[i].first = i, plans[i].count = 1;
+}
+
+int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
+	std::vector<hd_edit_desc> flat;
+	std::vector<PassPlan> plans;
+	batched_plans(p, edits, n, flat, plans);
 	p.last_groups = (u32)plans.size();
 	return edit_sequence(p, root, croot, flat, plans, root_out, croot_out);
 }
 
 int edit_plain(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
-	std::vector<hd_edit_desc> flat(edits, edits + n);
-	std::vector<PassPlan> plans(n);
-	for (u32 i = 0; i < n; ++i)
-		plans[i].first = i, plans[i].count = 1;
+	std::vector<hd_edit_desc> flat;
+	std::vector<PassPlan> plans;
+	plain_plans(edits, n, flat, plans);
 	p.last_groups = n;
 	return edit_sequence(p, root, croot, flat, plans, root_out, croot_out);
 }
 
+// hd_edit_reserve: the buffers the call's sync-free passes would allocate, allocated now.
+static int edit_reserve(Pool &p, const hd_edit_desc *edits, u32 n, u32 mode) {
+	std::vector<hd_edit_desc> flat;
+	std::vector<PassPlan> plans;
+	if (mode == HD_MODE_BATCHED && n > 1)
+		batched_plans(p, edits, n, flat, plans);
+	else
+		plain_plans(edits, n, flat, plans);
+	bool any_async = false;
+	for (auto &pp : plans) {
+		plan_pass(p, flat.data(), pp);
+		any_async |= pp.async;
+	}
+	if (!any_async || force_sync_path())
+		return HD_OK;
+	AsyncBufs B;
+	HD_RET(async_setup(p, plans, flat, B));
+	HD_TRY(hipStreamSynchronize(p.stream));
+	return HD_OK;
+}
 
 // ---------------------------------------------------------------- frames captured as hipGraphs
 // A frame = up to max_passes sync-free passes (groups of pairwise-disjoint edits, max_edits each,
@@ -4392,6 +4722,8 @@ static int frame_set(FrameState &F, const hd_edit_desc *edits, u32 n, const hd_v
 		const hd_edit_desc &d = edits[k];
 		if (d.kind > HD_EDIT_AABB_PAINT)
 			return set_error("hd_frame_set: edit %u: column fills and unknown kinds are not frame edits", k), HD_ERR_ARG;
+		if (!edit_color_ok(d))
+			return set_error("hd_frame_set: edit %u: bits_per_weight > 3", k), HD_ERR_ARG;
 		if (is_stateful(d) && !(cfg.flags & HD_FRAME_COLORS))
 			return set_error("hd_frame_set: edit %u is colored but the frame has no color passes", k), HD_ERR_ARG;
 		for (u32 l = 1; l < L; ++l)
@@ -4460,6 +4792,8 @@ extern "C" int hd_edit(hd_pool *h, uint32_t root, uint32_t color_root, const hd_
 			return set_error("unknown edit kind %u", edits[i].kind), HD_ERR_ARG;
 		if (edits[i].kind == HD_EDIT_COLUMN_FILL && !p.hm_mn)
 			return set_error("column fill needs hd_pool_set_heightmap"), HD_ERR_ARG;
+		if (!edit_color_ok(edits[i]))
+			return set_error("edit %u: bits_per_weight %u > 3", i, edits[i].color.bits_per_weight), HD_ERR_ARG;
 	}
 	if (n == 0) {
 		p.last_groups = 0;
@@ -4473,6 +4807,18 @@ extern "C" int hd_edit(hd_pool *h, uint32_t root, uint32_t color_root, const hd_
 	*root_out = r;
 	*color_root_out = c;
 	return HD_OK;
+}
+
+extern "C" int hd_edit_reserve(hd_pool *h, const hd_edit_desc *edits, uint32_t n, uint32_t mode) {
+	if (!h || (!edits && n))
+		return set_error("hd_edit_reserve: null argument"), HD_ERR_ARG;
+	for (u32 i = 0; i < n; ++i) {
+		if (edits[i].kind > HD_EDIT_COLUMN_FILL)
+			return set_error("unknown edit kind %u", edits[i].kind), HD_ERR_ARG;
+		if (!edit_color_ok(edits[i]))
+			return set_error("edit %u: bits_per_weight %u > 3", i, edits[i].color.bits_per_weight), HD_ERR_ARG;
+	}
+	return edit_reserve(h->p, edits, n, mode);
 }
 
 // ---- frames (hipGraph): the C-ABI

@@ -14,6 +14,8 @@
 namespace hd {
 
 void set_error(const char *fmt, ...);
+// VBR colors carry at most 3 bits per weight (the chunk headers' 2-bit field, VBRColor.hpp:368-487).
+inline bool edit_color_ok(const hd_edit_desc &d) { return !d.color.has_value || d.color.bits_per_weight <= 3; }
 
 #define HD_TRY(expr)                                                                                              \
 	do {                                                                                                          \

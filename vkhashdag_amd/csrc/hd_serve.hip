@@ -127,6 +127,8 @@ extern "C" int hd_serve_run(hd_pool *h, const hd_serve_config *cfg, uint32_t roo
 			need_pixels += (u64)v.width * v.height;
 		} else if (reqs[i].kind != HD_REQ_EDIT)
 			return set_error("hd_serve_run: unknown request kind %u", reqs[i].kind), HD_ERR_ARG;
+		else if (!edit_color_ok(reqs[i].edit))
+			return set_error("hd_serve_run: edit request %u: bits_per_weight > 3", i), HD_ERR_ARG;
 	}
 	// output pixels: every render keeps its image when rgba holds them all, else a tick scratch
 	const bool keep = rgba && rgba_pixels >= need_pixels;
