@@ -1,13 +1,16 @@
 """Edit -> render frames captured as hipGraphs (hd_frame_*).
 
-Bar: bit-exact. A replayed frame leaves the same pool words, roots and color pool as
-hd_edit(..., HD_MODE_BATCHED) on the same edits (one edit per frame: HD_MODE_PLAIN), and each of its
-views equals hd_trace of that view alone on the new roots. Frames queue back to back, survive a GC
-(roots handed back), and reject edits outside their budget."""
+Bar: a one-edit frame leaves the same pool words, roots and color pool as hd_edit(...,
+HD_MODE_PLAIN), bit-exact. A frame's passes are groups of pairwise-disjoint edits, while
+hd_edit(..., HD_MODE_BATCHED) folds overlapping edits into one pass, so a multi-edit frame meets the
+batched bar against it: the same reachable DAG up to node renumbering and the same colors on every
+occupied voxel. Each view equals hd_trace of that view alone on the frame's new roots, bit-exact.
+Frames queue back to back, survive a GC (roots handed back), and reject edits outside their
+budget."""
 import numpy as np
 import pytest
 
-from test_gpu_async import assert_same_device_pools
+from test_gpu_async import assert_equivalent_device_pools, assert_same_device_pools
 from test_oracle_pins import small_cfg
 from vkhashdag_amd import workloads as W
 
@@ -49,10 +52,10 @@ def test_frames_equal_batched_edits_and_lone_renders():
         f.launch()
         f.wait()
         b.edit(e, capi.MODE_BATCHED)
-        assert_same_device_pools(a, b, f"frame {k}")
+        assert_equivalent_device_pools(a, b, cfg, f"frame {k}")
         imgs = f.images()
         for (vb, w, hh, fl), img in zip(views, imgs):
-            assert np.array_equal(img, b.render(vb, w, hh, shadow=True)), f"frame {k}: view differs"
+            assert np.array_equal(img, a.render(vb, w, hh, shadow=True)), f"frame {k}: view differs"
     f.close()
 
 
@@ -90,7 +93,7 @@ def test_frames_continue_after_gc_and_check_budgets():
     f.launch()
     f.wait()
     b.edit(e2, capi.MODE_BATCHED)
-    assert_same_device_pools(a, b, "after gc")
+    assert_equivalent_device_pools(a, b, cfg, "after gc")
     with pytest.raises(capi.HDError, match="radius budget"):
         f.set(W.edits([W.edit(W.SPHERE_FILL, (64, 64, 64), r2=40 * 40, color=1)]), [])
     with pytest.raises(capi.HDError, match="column fills"):
@@ -100,8 +103,8 @@ def test_frames_continue_after_gc_and_check_budgets():
 
 def test_frames_with_gc_in_the_graph_equal_edit_then_gc():
     # HD_FRAME_GC: edit -> GC -> render captured once; every replay equals hd_edit (batched) then
-    # hd_gc of the new root on a twin pool, word for word (the GC runs inside the graph), and each
-    # view equals a lone render of the compacted root
+    # hd_gc of the new root on a twin pool (the GC runs inside the graph), and each view equals a
+    # lone render of the compacted root
     capi = _capi()
     res = 128
     cfg, h, (a, b) = _terrain(capi, res)
@@ -114,13 +117,12 @@ def test_frames_with_gc_in_the_graph_equal_edit_then_gc():
         f.wait()
         b.edit(e, capi.MODE_BATCHED)
         b.gc()
-        assert_same_device_pools(a, b, f"frame {k} with GC")
-        assert np.array_equal(a.filled(), b.filled())
+        assert_equivalent_device_pools(a, b, cfg, f"frame {k} with GC")
         for (vb, w, hh, fl), img in zip(views, f.images()):
-            assert np.array_equal(img, b.render(vb, w, hh, shadow=True)), f"frame {k}: view differs"
+            assert np.array_equal(img, a.render(vb, w, hh, shadow=True)), f"frame {k}: view differs"
     f.close()
     # the pool goes on as a plain pool afterwards
     e = W.mixed_batch(6, res, h, 2, 12, seed=400)
     a.edit(e, capi.MODE_BATCHED)
     b.edit(e, capi.MODE_BATCHED)
-    assert_same_device_pools(a, b, "after the frames")
+    assert_equivalent_device_pools(a, b, cfg, "after the frames")

@@ -55,6 +55,9 @@ struct EditParams {
 	const u32 *filled;
 	u32 *cnodes, *cleaves, *meta;
 	u32 stateful; // some edit of the pass runs through VBREditorWrapper
+	// the pass's edits may overlap: node edit lists are folded in order (list_chain), and a node an
+	// edit only crosses takes per-voxel colors instead of a whole-node fill color
+	u32 folded = 0;
 	u32 octree_root;
 	u32 leaf_bits; // voxel_level - color_leaf_level
 	__device__ DevEditor editor(u32 k) const {
@@ -92,6 +95,11 @@ __device__ inline Classified classify_single(const EditParams &P, const DevEdito
 	const u32 leafL = P.c.color_leaf;
 	Color col = get_fill(octree);
 	r.t = ed.edit_node_vbr(cl, cx, cy, cz, old != kNull, col);
+	// A folded pass: the fill color a crossed node would take (empty, or already the edit's color)
+	// comes from the node's state before the pass, which earlier edits of the pass may have changed;
+	// per-voxel colors give the occupied voxels the same colors without it.
+	if (P.folded && col.has() && ed.geom(cl, cx, cy, cz) != kFill)
+		col = Color::none();
 	if (!(r.sf & kSfFinal)) {
 		if (cl <= leafL) {
 			if (col.has()) {
@@ -177,7 +185,7 @@ __global__ void k_expand(EditParams P, u32 l, const Entry *__restrict__ ent, u32
 		octree = color_child(P.cnodes, E.octree, i);
 	const u32 sf_in = i | (E.sf & (kSfFinal | kSfWriter));
 	Classified c{kNotAffected, octree, sf_in, false, Color::none()};
-	u32 cnt = 1, single = E.loff;
+	u32 cnt = 1, single = E.loff, kid_base = old;
 	if (E.llen == 1)
 		c = classify_single(P, P.editor(E.loff), cl, cx, cy, cz, old, octree, sf_in);
 	else {
@@ -853,16 +861,15 @@ struct PairOut {
 // (9 words, the pool is padded) so its mask and child arrive in one memory round trip.
 // bits: where the classification records which list edits touch the child (bit k for list entry k;
 // at least (llen + 31) / 32 words, private to the thread), for emit_pair_e; or null.
-// pre: the list edits touching the child as a wavefront counted them (coop_classify), or null to
-// test the list here.
-struct ListHits {
-	u32 cnt, single, active;
+// Child i of entry E (level l): its old pointer (the node read whole, 9 words, the pool is padded,
+// so its mask and child arrive in one memory round trip), coordinates, color state and inherited
+// flags.
+struct PairIn {
+	u32 old, octree, sf_in, cx, cy, cz;
 };
-__device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel &A, u64 t, const Entry &E,
-                                        PushEvent *__restrict__ ev, u32 *__restrict__ pm, u32 ev_cap,
-                                        u32 *bits = nullptr, const ListHits *pre = nullptr) {
-	const u32 i = (u32)(t & 7u);
-	u32 old = kNull;
+__device__ inline PairIn pair_in(const EditParams &P, u32 l, const Entry &E, u32 i) {
+	PairIn q;
+	q.old = kNull;
 	if (E.ptr != kNull) {
 		u32 w[9];
 #pragma unroll
@@ -874,54 +881,163 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 #pragma unroll
 			for (u32 k = 0; k < 8; ++k)
 				if (k == r)
-					old = w[1 + k];
+					q.old = w[1 + k];
 		}
 	}
-	const u32 cl = l + 1, leafL = P.c.color_leaf;
-	const u32 cx = (E.x << 1) | (i & 1u), cy = (E.y << 1) | ((i >> 1) & 1u), cz = (E.z << 1) | ((i >> 2) & 1u);
-	u32 octree = E.octree;
-	if (P.stateful && cl <= leafL)
-		octree = color_child(P.cnodes, E.octree, i);
-	const u32 sf_in = i | (E.sf & (kSfFinal | kSfWriter));
+	q.cx = (E.x << 1) | (i & 1u), q.cy = (E.y << 1) | ((i >> 1) & 1u), q.cz = (E.z << 1) | ((i >> 2) & 1u);
+	q.octree = E.octree;
+	if (P.stateful && l + 1 <= P.c.color_leaf)
+		q.octree = color_child(P.cnodes, E.octree, i);
+	q.sf_in = i | (E.sf & (kSfFinal | kSfWriter));
+	return q;
+}
+// A list entry's child, folded (list_chain). pre: the list's last edit that resets the child, in
+// pre->single, whether some edit covers it whole, in pre->cnt, and pre2: the edits touching it from
+// the chain's stop on (count, first, any stateful), as a wavefront found them (coop_node); null to
+// scan the list here.
+struct ListHits {
+	u32 cnt, single, active;
+};
+// A list entry's child after folding its list (list_chain): the state before its first remaining
+// edit (pointer, octree state, flags), that edit's list index k (llen: none remains, the child is
+// terminal) and the last color a whole-node edit pushed on it.
+struct ChainRes {
+	Classified c;
+	u32 ptr, k;
+	bool pushed;
+	Color pcol;
+};
+// From list index rs (the last edit that fills or clears the child whole, or 0): each edit that
+// touches the child is applied in turn while it covers the child whole (classify_single: fill,
+// clear, whole-node paint, or nothing); the first one that crosses it stops the chain. A child whose
+// list holds more than one edit starts without the is_final bit: each edit of a folded list runs
+// its own traversal.
+__device__ inline ChainRes list_chain(const EditParams &P, const ALevel &A, const Entry &E, u32 rs, u32 cl, u32 cx,
+                                      u32 cy, u32 cz, u32 old, u32 octree, u32 sf_in) {
+	ChainRes r{Classified{kNotAffected, octree, sf_in & ~kSfFinal, false, Color::none()}, old, E.llen, false,
+	           Color::none()};
+	for (u32 k = rs; k < E.llen; ++k) {
+		const DevEditor ed = P.editor(ldg(A.lists, (u64)E.loff + k));
+		if (ed.geom(cl, cx, cy, cz) == kNotAffected)
+			continue;
+		const Classified s = classify_single(P, ed, cl, cx, cy, cz, r.ptr, r.c.octree, r.c.sf & ~kSfFinal);
+		if (s.t == kProceed) {
+			r.k = k;
+			return r;
+		}
+		r.ptr = s.t == kClear ? kNull : (s.t == kFill ? P.filled[cl] : r.ptr);
+		r.c.octree = s.octree;
+		r.c.sf = s.sf & ~kSfFinal;
+		if (s.push)
+			r.pushed = true, r.pcol = s.col;
+	}
+	return r;
+}
+__device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel &A, u64 t, const Entry &E,
+                                        PushEvent *__restrict__ ev, u32 *__restrict__ pm, u32 ev_cap,
+                                        u32 *bits = nullptr, const ListHits *pre = nullptr,
+                                        const ListHits *pre2 = nullptr) {
+	const u32 i = (u32)(t & 7u);
+	const PairIn q = pair_in(P, l, E, i);
+	const u32 old = q.old, cl = l + 1, leafL = P.c.color_leaf, cx = q.cx, cy = q.cy, cz = q.cz;
+	const u32 octree = q.octree, sf_in = q.sf_in;
 	Classified c{kNotAffected, octree, sf_in, false, Color::none()};
-	u32 cnt = 1, single = E.loff;
+	u32 cnt = 1, single = E.loff, kid_base = old;
 	if (E.llen == 1)
 		c = classify_single(P, P.editor(E.loff), cl, cx, cy, cz, old, octree, sf_in);
 	else {
+		// the folded list: the last edit that resets the child (its touching edits marked in bits);
+		// without an edit that covers the child whole, the scan's count is the list's (no chain)
+		u32 rs = 0, cnt0 = 0, single0 = 0;
+		bool full = true, active0 = false;
+		if (pre)
+			rs = pre->single, full = pre->cnt != 0u; // coop_reset's results
+		else {
+			full = false;
+			if (bits)
+				for (u32 w = 0; w < (E.llen + 31u) >> 5; ++w)
+					bits[w] = 0u;
+			for (u32 k0 = 0; k0 < E.llen; k0 += kListBatch) { // list words of a batch in one round trip
+				u32 ix[kListBatch];
+#pragma unroll
+				for (u32 j = 0; j < kListBatch; ++j)
+					ix[j] = k0 + j < E.llen ? ldg(A.lists, (u64)E.loff + k0 + j) : 0u;
+#pragma unroll
+				for (u32 j = 0; j < kListBatch; ++j)
+					if (k0 + j < E.llen) {
+						const DevEditor ed = P.editor(ix[j]);
+						const EditType g = ed.geom(cl, cx, cy, cz);
+						if (g != kNotAffected) {
+							if (bits)
+								bits[(k0 + j) >> 5] |= 1u << ((k0 + j) & 31u);
+							if (cnt0 == 0)
+								single0 = ix[j];
+							++cnt0;
+							active0 |= ed.stateful();
+						}
+						if (g == kFill) {
+							full = true;
+							if (!ed.is_paint())
+								rs = k0 + j;
+						}
+					}
+			}
+		}
+		// without a whole-node edit every touching edit remains: the list starts at index 0 (the count
+		// is the wavefront's when it scanned the list, pre2, else this thread's)
+		const u32 ntouch = pre2 ? pre2->cnt : cnt0;
+		const ChainRes ch = full ? list_chain(P, A, E, rs, cl, cx, cy, cz, old, octree, sf_in)
+		                         : ChainRes{Classified{kNotAffected, octree, sf_in & ~kSfFinal, false, Color::none()},
+		                                    old, ntouch ? 0u : E.llen, false, Color::none()};
+		kid_base = ch.ptr;
 		cnt = 0;
 		bool active = false;
-		if (pre) {
-			cnt = pre->cnt;
-			single = pre->single;
-			active = pre->active != 0u;
-		} else if (bits)
-			for (u32 w = 0; w < (E.llen + 31u) >> 5; ++w)
-				bits[w] = 0u;
-		for (u32 k0 = 0; !pre && k0 < E.llen; k0 += kListBatch) { // list words of a batch in one round trip
-			u32 ix[kListBatch];
+		if (!full) { // every touching edit remains
+			if (pre2)
+				cnt = pre2->cnt, single = pre2->single, active = pre2->active != 0u;
+			else
+				cnt = cnt0, single = single0, active = active0;
+		} else if (ch.k < E.llen) { // the remaining edits: those that touch the child from list index ch.k on
+			if (pre2) {
+				cnt = pre2->cnt;
+				single = pre2->single;
+				active = pre2->active != 0u;
+			} else
+				for (u32 k0 = ch.k; k0 < E.llen; k0 += kListBatch) {
+					u32 ix[kListBatch];
 #pragma unroll
-			for (u32 j = 0; j < kListBatch; ++j)
-				ix[j] = k0 + j < E.llen ? ldg(A.lists, (u64)E.loff + k0 + j) : 0u;
+					for (u32 j = 0; j < kListBatch; ++j)
+						ix[j] = k0 + j < E.llen ? ldg(A.lists, (u64)E.loff + k0 + j) : 0u;
 #pragma unroll
-			for (u32 j = 0; j < kListBatch; ++j)
-				if (k0 + j < E.llen) {
-					DevEditor ed = P.editor(ix[j]);
-					if (ed.geom(cl, cx, cy, cz) != kNotAffected) {
-						++cnt;
-						single = ix[j];
-						active |= ed.stateful();
-						if (bits)
-							bits[(k0 + j) >> 5] |= 1u << ((k0 + j) & 31u);
-					}
+					for (u32 j = 0; j < kListBatch; ++j)
+						if (k0 + j < E.llen) {
+							const DevEditor ed = P.editor(ix[j]);
+							if (ed.geom(cl, cx, cy, cz) != kNotAffected) {
+								if (cnt == 0)
+									single = ix[j];
+								++cnt;
+								active |= ed.stateful();
+							}
+						}
 				}
+			if (bits) // the child's list: the touching edits from ch.k on
+				for (u32 w = 0; w < (E.llen + 31u) >> 5; ++w)
+					bits[w] &= w < (ch.k >> 5) ? 0u : (w == (ch.k >> 5) ? ~((1u << (ch.k & 31u)) - 1u) : ~0u);
 		}
-		if (cnt == 1)
-			c = classify_single(P, P.editor(single), cl, cx, cy, cz, old, octree, sf_in);
-		else if (cnt > 1) {
+		if (cnt == 1) { // one edit remains: its own classification from the chain's state
+			c = classify_single(P, P.editor(single), cl, cx, cy, cz, ch.ptr, ch.c.octree, ch.c.sf);
+		} else if (cnt > 1) {
+			c = ch.c;
 			c.t = kProceed;
 			if (active)
 				c.sf |= kSfActive | (cl == leafL ? (kSfWriter | kSfNewJob) : 0u);
+			single = ch.k; // a list child's list starts at this index of the parent's
+		} else { // the chain covered the child: terminal
+			c = ch.c;
+			c.t = ch.ptr == old ? kNotAffected : (ch.ptr == kNull ? kClear : kFill);
 		}
+		if (ch.pushed && !c.push) // the chain's last whole-node color, under what remains
+			c.push = true, c.col = ch.pcol;
 	}
 	if (c.push && A.pev) { // the job rebuilds its colors from its pairs' records
 		const u32 d = cl - leafL, msk = (1u << d) - 1u, sh = 3u * (P.c.voxel_level - cl);
@@ -939,13 +1055,15 @@ __device__ inline PairOut expand_pair_e(const EditParams &P, u32 l, const ALevel
 		else
 			atomicOr(&pm[kPmOverflow], 1u);
 	}
-	const u32 kid = c.t == kClear ? kNull : (c.t == kFill ? P.filled[cl] : old);
-	stg(A.kids, t, kid); // the old child for proceeding and untouched slots
+	const u32 kid = c.t == kClear ? kNull : (c.t == kFill ? P.filled[cl] : kid_base);
+	stg(A.kids, t, kid); // the child's starting pointer for proceeding and untouched slots
 	if (A.okids)
 		stg(A.okids, t, c.octree);
 	bool proceed = c.t == kProceed;
-	// bit 0: proceeds, bit 1: a terminal child whose pointer changed, bit 2: pushed a color
-	u32 flag = (proceed ? 1u : 0u) | (!proceed && kid != old ? 2u : 0u) | (c.push ? 4u : 0u);
+	// bit 0: proceeds, bit 1: the child's pointer changed (a terminal result, or the state a folded
+	// list leaves a proceeding child in: the join compares proceeding results with kids, so it must
+	// know the slot changed even when the result equals that state), bit 2: pushed a color
+	u32 flag = (proceed ? 1u : 0u) | (kid != old ? 2u : 0u) | (c.push ? 4u : 0u);
 	return PairOut{flag, proceed && cnt > 1 ? cnt : 0u, make_uint4(c.octree, c.sf, single, cnt), kid};
 }
 __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &A, u64 t, PushEvent *__restrict__ ev,
@@ -954,7 +1072,7 @@ __device__ inline PairOut expand_pair(const EditParams &P, u32 l, const ALevel &
 }
 
 // Writes the child entry of a proceeding pair (k_emit). m8: proceed flags of the pair's node in
-// bits 0-7, its changed terminal children in bits 8-15 and its pushed children in bits 16-23 (valid
+// bits 0-7, its changed children in bits 8-15 and its pushed children in bits 16-23 (valid
 // on slot 0), idx / lo: the child's entry index and list offset at level l + 1.
 // bits: expand_pair_e's record of the list edits that touch the child, or null to test them again.
 // by_wave: the list is written by coop_emit (only the bound is checked here).
@@ -987,10 +1105,10 @@ __device__ inline Entry emit_pair_e(const EditParams &P, u32 l, const ALevel &A,
 				static_assert(32 % kListBatch == 0, "a batch's bits sit in one word");
 				if (bits)
 					keep = (bits[k0 >> 5] >> (k0 & 31u)) & ((1u << kListBatch) - 1u);
-				else
+				else // the touching edits from the fold's stop (cs.z) on
 #pragma unroll
 					for (u32 j = 0; j < kListBatch; ++j)
-						if (k0 + j < E.llen && P.editor(ix[j]).geom(l + 1, cx, cy, cz) != kNotAffected)
+						if (k0 + j < E.llen && k0 + j >= o.cs.z && P.editor(ix[j]).geom(l + 1, cx, cy, cz) != kNotAffected)
 							keep |= 1u << j;
 #pragma unroll
 				for (u32 j = 0; j < kListBatch; ++j)
@@ -1021,17 +1139,19 @@ __device__ inline void emit_pair(const EditParams &P, u32 l, const ALevel &A, co
 // callers keep the wavefront converged (the loops below are uniform).
 constexpr u32 kCoopList = 16, kCoopBatch = 4;
 __device__ inline u32 child_coord(u32 p, u32 i, u32 axis) { return (p << 1) | ((i >> axis) & 1u); }
-// Per child i of E: the list edits that touch it (cnt[i]), the last of them (single[i]), and
-// whether one of them is stateful (bit i of active), as expand_pair_e's own list loop finds them.
+// Per child i of E: the list edits that touch it from index ks[i] on (cnt[i]), the first of them
+// (single[i]), and whether one of them is stateful (bit i of active).
 struct CoopHits {
 	u32 cnt[8], single[8], active;
 };
-__device__ inline void coop_classify(const EditParams &P, u32 l, const ALevel &A, const Entry &E, u32 lane,
-                                     CoopHits &o) {
+// Per child i of E: the list index of the last edit that fills or clears it whole (0 if none), and
+// in bit i of full whether some edit covers it whole (a paint too).
+__device__ inline void coop_reset(const EditParams &P, u32 l, const ALevel &A, const Entry &E, u32 lane,
+                                  u32 (&rs)[8], u32 &full) {
+	full = 0u;
 #pragma unroll
 	for (u32 i = 0; i < 8; ++i)
-		o.cnt[i] = 0u, o.single[i] = 0u;
-	o.active = 0u;
+		rs[i] = 0u;
 	const u32 cl = l + 1u;
 	for (u32 k0 = 0; k0 < E.llen; k0 += 64u * kCoopBatch) {
 		u32 ix[kCoopBatch];
@@ -1046,25 +1166,107 @@ __device__ inline void coop_classify(const EditParams &P, u32 l, const ALevel &A
 				break;
 			const bool v = k0 + 64u * j + lane < E.llen;
 			const DevEditor ed = P.editor(ix[j]);
+			const bool paint = ed.is_paint();
+#pragma unroll
+			for (u32 i = 0; i < 8; ++i) {
+				const bool f =
+				    v && ed.geom(cl, child_coord(E.x, i, 0), child_coord(E.y, i, 1), child_coord(E.z, i, 2)) == kFill;
+				const u64 m = __ballot(f && !paint);
+				if (m)
+					rs[i] = k0 + 64u * j + 63u - (u32)__builtin_clzll(m);
+				if (__ballot(f))
+					full |= 1u << i;
+			}
+		}
+	}
+}
+__device__ inline void coop_count(const EditParams &P, u32 l, const ALevel &A, const Entry &E, u32 lane,
+                                  const u32 (&ks)[8], CoopHits &o) {
+#pragma unroll
+	for (u32 i = 0; i < 8; ++i)
+		o.cnt[i] = 0u, o.single[i] = 0u;
+	o.active = 0u;
+	const u32 cl = l + 1u;
+	u32 kmin = ks[0];
+#pragma unroll
+	for (u32 i = 1; i < 8; ++i)
+		kmin = min(kmin, ks[i]);
+	for (u32 k0 = kmin & ~63u; k0 < E.llen; k0 += 64u * kCoopBatch) {
+		u32 ix[kCoopBatch];
+#pragma unroll
+		for (u32 j = 0; j < kCoopBatch; ++j) {
+			const u32 k = k0 + 64u * j + lane;
+			ix[j] = k < E.llen ? ldg(A.lists, (u64)E.loff + k) : 0u;
+		}
+#pragma unroll
+		for (u32 j = 0; j < kCoopBatch; ++j) {
+			if (k0 + 64u * j >= E.llen) // uniform
+				break;
+			const u32 k = k0 + 64u * j + lane;
+			const bool v = k < E.llen;
+			const DevEditor ed = P.editor(ix[j]);
 			const bool st = ed.stateful();
 #pragma unroll
 			for (u32 i = 0; i < 8; ++i) {
-				const bool hit = v && ed.geom(cl, child_coord(E.x, i, 0), child_coord(E.y, i, 1),
-				                              child_coord(E.z, i, 2)) != kNotAffected;
+				const bool hit = v && k >= ks[i] &&
+				                 ed.geom(cl, child_coord(E.x, i, 0), child_coord(E.y, i, 1), child_coord(E.z, i, 2)) !=
+				                     kNotAffected;
 				const u64 m = __ballot(hit);
+				if (m && o.cnt[i] == 0u)
+					o.single[i] = __builtin_amdgcn_readlane(ix[j], (int)__builtin_ctzll(m));
 				o.cnt[i] += (u32)__popcll(m);
-				if (m)
-					o.single[i] = __builtin_amdgcn_readlane(ix[j], 63 - (int)__builtin_clzll(m));
 				if (__ballot(hit && st))
 					o.active |= 1u << i;
 			}
 		}
 	}
 }
+__device__ inline ListHits coop_pick(const CoopHits &c, u32 i) {
+	ListHits h{0u, 0u, (c.active >> i) & 1u};
+#pragma unroll
+	for (u32 k = 0; k < 8; ++k)
+		if (k == i)
+			h.cnt = c.cnt[k], h.single = c.single[k];
+	return h;
+}
+// The whole wave folds the list of its node e (lanes 8 e .. 8 e + 7 hold the node's pairs): the
+// last reset of every child, the chain on each pair's lane (list_chain, as expand_pair_e repeats
+// it), then the touching edits from each chain's stop. h1 / h2: expand_pair_e's pre / pre2.
+__device__ inline void coop_node(const EditParams &P, u32 l, const ALevel &A, const Entry &E, u32 e, u32 lane,
+                                 ListHits &h1, ListHits &h2, bool &by_wave) {
+	u32 rs[8], full;
+	coop_reset(P, l, A, E, lane, rs, full);
+	const bool mine = (lane >> 3) == e;
+	const u32 i = lane & 7u;
+	const bool my_full = (full >> i) & 1u;
+	u32 my_rs = 0, kk = E.llen;
+#pragma unroll
+	for (u32 k = 0; k < 8; ++k)
+		my_rs = k == i ? rs[k] : my_rs;
+	if (mine) {
+		if (my_full) {
+			const PairIn q = pair_in(P, l, E, i);
+			kk = list_chain(P, A, E, my_rs, l + 1u, q.cx, q.cy, q.cz, q.old, q.octree, q.sf_in).k;
+		} else
+			kk = 0; // no edit covers the child whole: all its touching edits remain
+	}
+	u32 ks[8];
+#pragma unroll
+	for (u32 k = 0; k < 8; ++k)
+		ks[k] = __builtin_amdgcn_readlane(kk, (int)(8u * e + k));
+	CoopHits co;
+	coop_count(P, l, A, E, lane, ks, co);
+	if (mine) {
+		h1 = ListHits{my_full ? 1u : 0u, my_rs, 0u};
+		h2 = coop_pick(co, i);
+		by_wave = true;
+	}
+}
 // The children's lists of E in list (submission) order: child i's at N.lists[lo[i]...] when bit i
-// of need is set (a proceeding child with more than one edit whose list fits the level's bound).
+// of need is set (a proceeding child with more than one edit whose list fits the level's bound):
+// the edits that touch it from list index ks[i] on.
 __device__ inline void coop_emit(const EditParams &P, u32 l, const ALevel &A, const ALevel &N, const Entry &E,
-                                 u32 lane, const u32 (&lo)[8], u32 need) {
+                                 u32 lane, const u32 (&lo)[8], const u32 (&ks)[8], u32 need) {
 	const u32 cl = l + 1u;
 	const u64 below = (1ull << lane) - 1ull;
 	u32 run[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1085,8 +1287,9 @@ __device__ inline void coop_emit(const EditParams &P, u32 l, const ALevel &A, co
 			for (u32 i = 0; i < 8; ++i) {
 				if (!((need >> i) & 1u))
 					continue;
-				const bool hit = v && ed.geom(cl, child_coord(E.x, i, 0), child_coord(E.y, i, 1),
-				                              child_coord(E.z, i, 2)) != kNotAffected;
+				const bool hit = v && k0 + 64u * j + lane >= ks[i] &&
+				                 ed.geom(cl, child_coord(E.x, i, 0), child_coord(E.y, i, 1), child_coord(E.z, i, 2)) !=
+				                     kNotAffected;
 				const u64 m = __ballot(hit);
 				if (hit)
 					stg(N.lists, (u64)lo[i] + run[i] + (u32)__popcll(m & below), ix[j]);
@@ -1095,15 +1298,6 @@ __device__ inline void coop_emit(const EditParams &P, u32 l, const ALevel &A, co
 		}
 	}
 }
-__device__ inline ListHits coop_pick(const CoopHits &c, u32 i) {
-	ListHits h{0u, 0u, (c.active >> i) & 1u};
-#pragma unroll
-	for (u32 k = 0; k < 8; ++k)
-		if (k == i)
-			h.cnt = c.cnt[k], h.single = c.single[k];
-	return h;
-}
-
 // chunk of the pair range [0, 8n) owned by this block (whole nodes per block)
 __device__ inline void pair_chunk(u32 n, u64 &begin, u64 &end) {
 	u64 P8 = (u64)n * 8u, chunk = (P8 + gridDim.x - 1) / gridDim.x;
@@ -1153,11 +1347,13 @@ __device__ inline Pair expand_emit_range(const EditParams &P, u32 l, const ALeve
 				const u32 need8 = (u32)(need >> (8u * e)) & 0xFFu;
 				if (!need8)
 					continue;
-				u32 los[8];
+				u32 los[8], ks[8];
 #pragma unroll
-				for (u32 i = 0; i < 8; ++i)
+				for (u32 i = 0; i < 8; ++i) {
 					los[i] = __builtin_amdgcn_readlane(lo, (int)(8u * e + i));
-				coop_emit(P, l, A, N, A.ent[(u32)(wnode + e)], lane, los, need8);
+					ks[i] = __builtin_amdgcn_readlane(o.cs.z, (int)(8u * e + i));
+				}
+				coop_emit(P, l, A, N, A.ent[(u32)(wnode + e)], lane, los, ks, need8);
 			}
 		}
 		base.a += tile.a;
@@ -1175,19 +1371,50 @@ __device__ inline void set_level_count(const EditParams &P, u32 l, const ALevel 
 		pm[kPmOverflow] |= 1u;
 }
 
-// Wave 0 (all 64 lanes): the root's edit list, 64 edits per step in submission order; lane 0 then
+// Wave 0 (all 64 lanes): the root's folded edit list (as list_chain folds a child's): the last
+// edit that fills or clears the whole grid, the whole-grid edits after it applied in turn, then the
+// edits from the first that crosses the root on, 64 per step in submission order; lane 0 then
 // classifies the root.
 __device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__restrict__ roots, u32 *__restrict__ pm,
                                   u32 lane) {
+	const u32 root = roots[0], croot = roots[1], L = P.c.L, n = P.n_edits;
+	ChainRes ch{Classified{kNotAffected, croot, 0u, false, Color::none()}, root, 0u, false, Color::none()};
 	u32 cnt = 0, single = 0;
 	bool active = false;
-	if (P.n_edits == 1)
+	if (n == 1)
 		cnt = 1;
-	else
-		for (u32 k0 = 0; k0 < P.n_edits; k0 += 64u) {
+	else {
+		u32 rs = 0;
+		for (u32 k0 = 0; k0 < n; k0 += 64u) {
+			const u32 k = k0 + lane;
+			bool z = false;
+			if (k < n) {
+				const DevEditor e = P.editor(k);
+				z = !e.is_paint() && e.geom(0, 0, 0, 0) == kFill;
+			}
+			const u64 m = __ballot(z);
+			if (m)
+				rs = k0 + 63u - (u32)__builtin_clzll(m);
+		}
+		// the chain on every lane (the same values everywhere)
+		ch.k = n;
+		for (u32 k = rs; k < n; ++k) {
+			const DevEditor ed = P.editor(k);
+			if (ed.geom(0, 0, 0, 0) == kNotAffected)
+				continue;
+			const Classified sc = classify_single(P, ed, 0, 0, 0, 0, ch.ptr, ch.c.octree, ch.c.sf);
+			if (sc.t == kProceed) {
+				ch.k = k;
+				break;
+			}
+			ch.ptr = sc.t == kClear ? kNull : (sc.t == kFill ? P.filled[0] : ch.ptr);
+			ch.c.octree = sc.octree;
+			ch.c.sf = sc.sf & ~kSfFinal;
+		}
+		for (u32 k0 = ch.k & ~63u; k0 < n; k0 += 64u) {
 			const u32 k = k0 + lane;
 			bool hit = false, st = false;
-			if (k < P.n_edits) {
+			if (k < n && k >= ch.k) {
 				const DevEditor e = P.editor(k);
 				hit = e.geom(0, 0, 0, 0) != kNotAffected;
 				st = e.stateful();
@@ -1196,31 +1423,32 @@ __device__ inline void root_entry(const EditParams &P, const ALevel &A0, u32 *__
 			const u32 at = cnt + (u32)__popcll(m & ((1ull << lane) - 1ull));
 			if (hit && at < A0.list_cap)
 				A0.lists[at] = k;
+			if (m && cnt == 0)
+				single = k0 + (u32)__builtin_ctzll(m);
 			cnt += (u32)__popcll(m);
-			if (m)
-				single = k0 + 63u - (u32)__builtin_clzll(m);
 			active |= __ballot(hit && st) != 0ull;
 		}
+	}
 	if (lane != 0)
 		return;
-	const u32 root = roots[0], croot = roots[1], L = P.c.L;
-	Classified c{kNotAffected, croot, 0u, false, Color::none()};
+	Classified c = ch.c;
 	if (cnt == 1)
-		c = classify_single(P, P.editor(single), 0, 0, 0, 0, root, croot, 0u);
+		c = classify_single(P, P.editor(single), 0, 0, 0, 0, ch.ptr, ch.c.octree, ch.c.sf);
 	else if (cnt > 1) {
 		c.t = kProceed;
 		if (active)
 			c.sf |= kSfActive | (P.c.color_leaf == 0 ? (kSfWriter | kSfNewJob) : 0u);
 		if (cnt > A0.list_cap)
 			pm[kPmOverflow] |= 1u;
-	}
+	} else
+		c.t = ch.ptr == root ? kNotAffected : (ch.ptr == kNull ? kClear : kFill);
 	if (c.t == kProceed) {
-		A0.ent[0] = Entry{0, 0, 0, root, kNull, c.sf & ~kSfNewJob, c.octree, (c.sf & kSfWriter) ? 0u : kNull,
+		A0.ent[0] = Entry{0, 0, 0, ch.ptr, kNull, c.sf & ~kSfNewJob, c.octree, (c.sf & kSfWriter) ? 0u : kNull,
 		                  cnt == 1 ? single : 0u, cnt};
 		pm[kPmHdr] = 1;
 		pm[kPmHdr + L] = cnt > 1 ? cnt : 0u;
 	} else {
-		roots[0] = c.t == kClear ? kNull : (c.t == kFill ? P.filled[0] : root);
+		roots[0] = c.t == kClear ? kNull : (c.t == kFill ? P.filled[0] : ch.ptr);
 		if (P.stateful)
 			roots[1] = c.octree;
 	}
@@ -1282,24 +1510,19 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			Entry E{};
 			u32 *bits = nullptr;
 			// long lists: the whole wave classifies each such node's children (lane 8 e + i is pair i of e)
-			ListHits hits{0u, 0u, 0u};
+			ListHits hits{0u, 0u, 0u}, hits2{0u, 0u, 0u};
 			bool by_wave = false;
 			if (kLists)
 				for (u32 e = 0; e < n; ++e) {
 					const Entry Ee = cur[e];
-					if (Ee.llen <= kCoopList)
-						continue;
-					CoopHits co;
-					coop_classify(P, l, A, Ee, lane, co);
-					if ((lane >> 3) == e) {
-						hits = coop_pick(co, lane & 7u);
-						by_wave = true;
-					}
+					if (Ee.llen > kCoopList)
+						coop_node(P, l, A, Ee, e, lane, hits, hits2, by_wave);
 				}
 			if (in) {
 				E = cur[lane >> 3];
 				bits = kLists && !by_wave && E.llen <= kTopEdits ? sbits[lane] : nullptr;
-				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap, bits, by_wave ? &hits : nullptr);
+				o = expand_pair_e(P, l, A, lane, E, ev, pm, ev_cap, bits, by_wave ? &hits : nullptr,
+				                  by_wave ? &hits2 : nullptr);
 			}
 			const u64 bp = __ballot(o.flag & 1u), bt = __ballot((o.flag >> 1) & 1u), bu = __ballot((o.flag >> 2) & 1u);
 			const u64 below = (1ull << lane) - 1ull;
@@ -1327,11 +1550,13 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 					const u32 need8 = (u32)(need >> (8u * e)) & 0xFFu;
 					if (!need8)
 						continue;
-					u32 los[8];
+					u32 los[8], ks[8];
 #pragma unroll
-					for (u32 i = 0; i < 8; ++i)
+					for (u32 i = 0; i < 8; ++i) {
 						los[i] = __builtin_amdgcn_readlane(lo, (int)(8u * e + i));
-					coop_emit(P, l, A, N, cur[e], lane, los, need8);
+						ks[i] = __builtin_amdgcn_readlane(o.cs.z, (int)(8u * e + i));
+					}
+					coop_emit(P, l, A, N, cur[e], lane, los, ks, need8);
 				}
 			}
 			if (lane == 0)
@@ -1362,24 +1587,19 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			// long lists: each wave classifies the children of the 8 nodes whose pairs are its lanes
 			const u32 lane = threadIdx.x & 63u;
 			const u64 wnode = (t0 + (threadIdx.x & ~63u)) >> 3; // the wave's first node
-			ListHits hits{0u, 0u, 0u};
+			ListHits hits{0u, 0u, 0u}, hits2{0u, 0u, 0u};
 			bool by_wave = false;
 			if (kLists)
 				for (u32 e = 0; e < 8u && wnode + e < n; ++e) {
 					const Entry Ee = cur[wnode + e];
-					if (Ee.llen <= kCoopList)
-						continue;
-					CoopHits co;
-					coop_classify(P, l, A, Ee, lane, co);
-					if ((lane >> 3) == e) {
-						hits = coop_pick(co, lane & 7u);
-						by_wave = true;
-					}
+					if (Ee.llen > kCoopList)
+						coop_node(P, l, A, Ee, e, lane, hits, hits2, by_wave);
 				}
 			if (in) {
 				E = cur[t >> 3];
 				bits = kLists && !by_wave && E.llen <= kTopEdits ? sbits[threadIdx.x] : nullptr;
-				o = expand_pair_e(P, l, A, t, E, ev, pm, ev_cap, bits, by_wave ? &hits : nullptr);
+				o = expand_pair_e(P, l, A, t, E, ev, pm, ev_cap, bits, by_wave ? &hits : nullptr,
+				                  by_wave ? &hits2 : nullptr);
 			}
 			sflag[threadIdx.x] = o.flag;
 			Pair tile;
@@ -1403,11 +1623,13 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 					const u32 need8 = (u32)(need >> (8u * e)) & 0xFFu;
 					if (!need8)
 						continue;
-					u32 los[8];
+					u32 los[8], ks[8];
 #pragma unroll
-					for (u32 i = 0; i < 8; ++i)
+					for (u32 i = 0; i < 8; ++i) {
 						los[i] = __builtin_amdgcn_readlane(lo, (int)(8u * e + i));
-					coop_emit(P, l, A, N, cur[wnode + e], lane, los, need8);
+						ks[i] = __builtin_amdgcn_readlane(o.cs.z, (int)(8u * e + i));
+					}
+					coop_emit(P, l, A, N, cur[wnode + e], lane, los, ks, need8);
 				}
 			}
 			base.a += tile.a;
@@ -1440,21 +1662,15 @@ __global__ void __launch_bounds__(kAT) a_count(EditParams P, u32 l, const ALevel
 		// children's edits (whole nodes per wave: chunks start on node boundaries)
 		const u64 wnode = (t0 + (threadIdx.x & ~63u)) >> 3;
 		const u64 longl = __ballot(in && (t & 7u) == 0 && ldg(&A.ent[(u32)(t >> 3)].llen, 0) > kCoopList);
-		ListHits hits{0u, 0u, 0u};
+		ListHits hits{0u, 0u, 0u}, hits2{0u, 0u, 0u};
 		bool by_wave = false;
-		for (u32 e = 0; e < 8u; ++e) {
-			if (!((longl >> (8u * e)) & 1ull))
-				continue;
-			CoopHits co;
-			coop_classify(P, l, A, A.ent[(u32)(wnode + e)], lane, co);
-			if ((lane >> 3) == e) {
-				hits = coop_pick(co, lane & 7u);
-				by_wave = true;
-			}
-		}
+		for (u32 e = 0; e < 8u; ++e)
+			if ((longl >> (8u * e)) & 1ull)
+				coop_node(P, l, A, A.ent[(u32)(wnode + e)], e, lane, hits, hits2, by_wave);
 		if (!in)
 			continue;
-		PairOut o = expand_pair_e(P, l, A, t, A.ent[(u32)(t >> 3)], ev, pm, ev_cap, nullptr, by_wave ? &hits : nullptr);
+		PairOut o = expand_pair_e(P, l, A, t, A.ent[(u32)(t >> 3)], ev, pm, ev_cap, nullptr, by_wave ? &hits : nullptr,
+		                          by_wave ? &hits2 : nullptr);
 		flag[t] = o.flag;
 		lcnt[t] = o.lcnt;
 		cstate[t] = o.cs;
@@ -1784,8 +2000,8 @@ __global__ void __launch_bounds__(kAT) a_leafcolor(EditParams P, const ALevel *_
 
 // edit_node's join (k_build) for entry e of level l, with the children's results resolved from
 // level l + 1, then the find among the bucket's existing nodes.
-// The old child of each proceeding or untouched slot is in kids; pmask bits 8-15 flag the
-// terminal children whose pointer changed.
+// The starting pointer of each proceeding or untouched slot is in kids (the old child, or the state
+// a folded edit list leaves it in); pmask bits 8-15 flag the slots whose pointer changed.
 struct Built {
 	u32 fin;   // final pointer when known (unchanged / emptied / found)
 	bool need; // new content that is not in the bucket yet
@@ -1832,7 +2048,7 @@ __device__ inline void children_by_slot(const JoinIn &q, ChildFn child, u32 (&cs
 	}
 }
 // The join itself: slot i takes cs[i] when the child proceeded (pmask bit i), else its old or
-// terminal pointer; pmask bits 8-15 flag the terminal children whose pointer changed.
+// terminal pointer; pmask bits 8-15 flag the slots whose pointer changed.
 __device__ inline Built build_node(const EditParams &P, u32 l, const JoinIn &q, const u32 (&cs)[8]) {
 	Built o;
 	const u32 pmw = q.pmw, pmk = pmw & 0xFFu;
@@ -2463,16 +2679,25 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 				c[r] = ldg(lcol, 64ull * llo + k);
 		}
 		__syncthreads();
-		// the pushed and assigned colors (loads of all of a thread's voxels first), marked covered
-		for (u32 f = 0; f < S.nfill; ++f) {
-			const uint4 q = S.fill[f];
-			for (u32 v = threadIdx.x; v < q.w; v += kEncThreads)
-				job_set(S, q.x + v, Color{q.y, q.z});
-			if (q.w >= 32u)
-				for (u32 w = (q.x >> 5) + threadIdx.x; w < (q.x + q.w) >> 5; w += kEncThreads)
-					S.cov[w] = 0xFFFFFFFFu;
-			else if (threadIdx.x == 0)
-				atomicOr(&S.cov[q.x >> 5], ((1u << q.w) - 1u) << (q.x & 31u));
+		// the pushed and assigned colors (loads of all of a thread's voxels first), marked covered. A
+		// folded pass can push a color on a proceeding child (the edit that reset it) and again on its
+		// children or assign leaf colors inside it: the larger pushes go first, the leaves last.
+		const u32 big = 1u << (3u * (vl - (leafL + 1u)));
+		for (u32 pass = 0; pass < 2; ++pass) {
+			for (u32 f = 0; f < S.nfill; ++f) {
+				const uint4 q = S.fill[f];
+				if ((q.w >= big) != (pass == 0))
+					continue;
+				for (u32 v = threadIdx.x; v < q.w; v += kEncThreads)
+					job_set(S, q.x + v, Color{q.y, q.z});
+				if (q.w >= 32u)
+					for (u32 w = (q.x >> 5) + threadIdx.x; w < (q.x + q.w) >> 5; w += kEncThreads)
+						S.cov[w] = 0xFFFFFFFFu;
+				else if (threadIdx.x == 0)
+					atomicOr(&S.cov[q.x >> 5], ((1u << q.w) - 1u) << (q.x & 31u));
+			}
+			if (P.folded)
+				__syncthreads();
 		}
 #pragma unroll
 		for (u32 r = 0; r < kPer; ++r) {
@@ -4033,6 +4258,7 @@ constexpr u64 kAsyncBudget = 32ull << 30; // scratch bytes a sync-free pass may 
 struct PassPlan {
 	u32 first = 0, count = 0;
 	bool stateful = false, async = false;
+	bool folded = false; // the edits may overlap (folded lists)
 	std::vector<u64> cap, list_cap;
 	u64 ev_cap = 0, bytes = 0;
 };
@@ -4102,11 +4328,36 @@ static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
 	pp.cap[0] = 1;
 	pp.list_cap[0] = pp.count > 1 ? pp.count : 0;
 	const u64 lim = 1ull << 31;
+	// A folded pass: a node's list holds the edits from the first that crosses it on (the last one
+	// that fills or clears it whole, and the whole-node edits after that, are folded into its
+	// state), so besides crossing edits (S) only paints that cover it whole: at most the nodes that
+	// lie whole inside the paint's box, per paint.
+	std::vector<u64> inner(L, 0);
+	if (pp.folded)
+		for (u32 k = 0; k < pp.count; ++k) {
+			const hd_edit_desc &d = edits[pp.first + k];
+			if (d.kind != HD_EDIT_SPHERE_PAINT && d.kind != HD_EDIT_AABB_PAINT)
+				continue;
+			i64 lo[3], hi[3];
+			edit_aabb(d, 1u << vl, lo, hi); // voxels [lo, hi] inclusive
+			for (u32 l = 1; l < L; ++l) {
+				const i64 Sz = (i64)1 << (vl - l);
+				u64 a = 1;
+				for (int i = 0; i < 3; ++i) {
+					const i64 f = (lo[i] + Sz - 1) / Sz, t = (hi[i] + 1) / Sz; // nodes [f, t) inside
+					a *= t > f ? (u64)(t - f) : 0u;
+				}
+				inner[l] += a;
+			}
+		}
 	for (u32 l = 1; l < L; ++l) {
 		pp.cap[l] = std::min(std::min(S[l], 8 * pp.cap[l - 1]), lim);
 		// a multi-edit node is crossed by the boundary of all its edits but at most one, which
 		// fills it (two fills would overlap): list words <= 2 S
-		pp.list_cap[l] = pp.count > 1 ? std::min(2 * S[l], lim) : 0;
+		if (pp.folded)
+			pp.list_cap[l] = std::min(S[l] + std::min(inner[l], (u64)pp.count * pp.cap[l]), lim);
+		else
+			pp.list_cap[l] = pp.count > 1 ? std::min(2 * S[l], lim) : 0;
 	}
 	pp.ev_cap = 0;
 	u64 pev_bytes = 0;
@@ -4291,8 +4542,9 @@ static int gen_reserve(Pool &p, AsyncBufs &B, u32 passes) {
 	return HD_OK;
 }
 
-static EditParams pass_params(const Pool &p, const hd_edit_desc *d_edits, u32 n, bool stateful) {
+static EditParams pass_params(const Pool &p, const hd_edit_desc *d_edits, u32 n, bool stateful, bool folded = false) {
 	EditParams P;
+	P.folded = folded && n > 1 ? 1u : 0u;
 	P.c = p.dc;
 	P.hm = p.d_hm;
 	P.edits = d_edits;
@@ -4332,7 +4584,7 @@ static void queue_setnode(const Pool &p, const PassPlan &pp, const EditParams &P
 // Queues one sync-free pass (no host synchronization).
 static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = true) {
 	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
-	const EditParams P = pass_params(p, B.edits + pp.first, pp.count, pp.stateful);
+	const EditParams P = pass_params(p, B.edits + pp.first, pp.count, pp.stateful, pp.folded);
 	const u32 V = pp.stateful ? 1u << (3u * P.leaf_bits) : 0u;
 	// A pair of a multi-edit pass tests its parent's edit list: count those tests too (lists hold at
 	// most the pass's edits, and shrink with depth), so list-heavy levels go to the wide kernels
@@ -4513,10 +4765,86 @@ static void plain_plans(const hd_edit_desc *edits, u32 n, std::vector<hd_edit_de
 		plans[i].first = i, plans[i].count = 1;
 }
 
+// Folded passes: runs of consecutive edits, each run one pass whose node edit lists keep submission
+// order (list_chain folds a node's list; leaves apply it voxel by voxel), so every voxel sees its
+// edits in order and overlapping edits share a pass. The occupied voxels' colors are the last
+// color written to each, as in sequential application. An edit whose colors would depend on the
+// path the sequential edits take runs in a pass of its own, between the runs: a weighted color (a
+// whole-node fill stores it RGB8, a voxel keeps it exact), a column fill, and an uncolored fill
+// among colored edits (it refills voxels with the colors earlier edits left on them).
+constexpr u32 kFoldedPassEdits = 512;
+static bool folds(const hd_edit_desc &d, bool colored_batch) {
+	if (d.kind == HD_EDIT_COLUMN_FILL)
+		return false;
+	if (is_stateful(d) && d.color.has_value && d.color.bits_per_weight)
+		return false;
+	const bool dig = d.kind == HD_EDIT_SPHERE_DIG || d.kind == HD_EDIT_AABB_DIG;
+	return !(colored_batch && !is_stateful(d) && !dig);
+}
+static void folded_plans(const hd_edit_desc *edits, u32 n, std::vector<hd_edit_desc> &flat,
+                         std::vector<PassPlan> &plans) {
+	flat.assign(edits, edits + n);
+	bool colored = false;
+	for (u32 k = 0; k < n; ++k)
+		colored |= is_stateful(edits[k]);
+	const u32 cap = g_tuning.batch_pass_edits ? g_tuning.batch_pass_edits : kFoldedPassEdits;
+	PassPlan cur;
+	auto flush = [&]() {
+		if (cur.count) {
+			cur.folded = cur.count > 1;
+			plans.push_back(cur);
+		}
+		cur = PassPlan{};
+	};
+	for (u32 k = 0; k < n; ++k) {
+		if (!folds(edits[k], colored)) {
+			flush();
+			PassPlan solo;
+			solo.first = k, solo.count = 1;
+			plans.push_back(solo);
+			continue;
+		}
+		if (cur.count == cap)
+			flush();
+		if (!cur.count)
+			cur.first = k;
+		++cur.count;
+	}
+	flush();
+}
+
+// The passes of a batched call: folded runs, each split in halves until its buffers fit the
+// sync-free budget; dependency groups when a colored pass would rebuild its color leaves from
+// dense per-voxel colors (nested pushes need the leaves rebuilt in LDS).
+static void batch_plans(const Pool &p, const hd_edit_desc *edits, u32 n, std::vector<hd_edit_desc> &flat,
+                        std::vector<PassPlan> &plans) {
+	if (!lds_jobs(p.dc) || force_sync_path()) {
+		batched_plans(p, edits, n, flat, plans);
+		return;
+	}
+	std::vector<PassPlan> runs;
+	folded_plans(edits, n, flat, runs);
+	std::vector<PassPlan> todo(runs.rbegin(), runs.rend()); // a stack, first run on top
+	while (!todo.empty()) {
+		PassPlan pp = todo.back();
+		todo.pop_back();
+		plan_pass(p, flat.data(), pp);
+		if (pp.folded && !pp.async) { // over the budget: its halves, in order
+			PassPlan a, b;
+			a.first = pp.first, a.count = pp.count / 2, a.folded = a.count > 1;
+			b.first = pp.first + a.count, b.count = pp.count - a.count, b.folded = b.count > 1;
+			todo.push_back(b);
+			todo.push_back(a);
+			continue;
+		}
+		plans.push_back(pp);
+	}
+}
+
 int edit_batched(Pool &p, u32 root, u32 croot, const hd_edit_desc *edits, u32 n, u32 *root_out, u32 *croot_out) {
 	std::vector<hd_edit_desc> flat;
 	std::vector<PassPlan> plans;
-	batched_plans(p, edits, n, flat, plans);
+	batch_plans(p, edits, n, flat, plans);
 	p.last_groups = (u32)plans.size();
 	return edit_sequence(p, root, croot, flat, plans, root_out, croot_out);
 }
@@ -4534,7 +4862,7 @@ static int edit_reserve(Pool &p, const hd_edit_desc *edits, u32 n, u32 mode) {
 	std::vector<hd_edit_desc> flat;
 	std::vector<PassPlan> plans;
 	if (mode == HD_MODE_BATCHED && n > 1)
-		batched_plans(p, edits, n, flat, plans);
+		batch_plans(p, edits, n, flat, plans);
 	else
 		plain_plans(edits, n, flat, plans);
 	bool any_async = false;
