@@ -156,11 +156,13 @@ enum {
 	HD_MODE_PLAIN = 0,  /* one level-synchronous pass per edit, edits applied in order: the result
 	                       (pool words, bucket counters, color arrays, roots) equals NodePoolBase::Edit
 	                       applied to each edit in turn (NodePool.hpp:405-417) */
-	HD_MODE_BATCHED = 1 /* dependency-ordered passes: each edit goes to the pass after every earlier
-	                       edit whose voxels it may share, and one level-synchronous pass applies all
-	                       edits of a pass with per-node ordered edit lists (32 passes for 4096 mixed
-	                       edits on the 4096^3 terrain); the reachable DAG and the colors of occupied
-	                       voxels equal sequential application */
+	HD_MODE_BATCHED = 1 /* folded passes: runs of consecutive edits, overlapping ones included, each
+	                       applied by one level-synchronous pass whose nodes carry the edits that
+	                       touch them in submission order, folded from the last edit that fills or
+	                       clears the node whole (10 passes for 4096 mixed edits on the 4096^3
+	                       terrain); an edit with a weighted color, a column fill, and an uncolored
+	                       fill among colored edits run alone. The reachable DAG and the colors of
+	                       occupied voxels equal sequential application */
 };
 
 /* Apply n edits to (root, color_root); writes the new roots. Replaces NodePoolBase::Edit and
