@@ -1907,6 +1907,12 @@ __device__ inline uint2 child_result_fresh(const ALevel &C, const GroupTable &G,
 __device__ inline bool leaf_writes_colors(const EditParams &P, const ALevel *__restrict__ lv, const Entry &E) {
 	return (E.sf & kSfWriter) && !(E.sf & kSfFinal) && (E.sf & kSfActive) && E.job < lv[P.c.color_leaf].cap;
 }
+// kMode: kLeafAll = occupancy, find and colors; kLeafNodes = occupancy and find only; kLeafColors =
+// the colors of the leaves inside a color job only. A leaf's colors depend on its starting words and
+// its edit list alone, so in a colored pass the color stream runs its own kLeafColors launch right
+// after the top-down, beside the kLeafNodes launch of the node levels.
+enum : u32 { kLeafAll = 0, kLeafNodes = 1, kLeafColors = 2 };
+template <u32 kMode>
 __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
                                               Color *__restrict__ lcol, u64 *__restrict__ lmask) {
 	const u32 lane = threadIdx.x & 63u, level = P.c.L - 1;
@@ -1921,6 +1927,9 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		E.z = __builtin_amdgcn_readfirstlane(E.z), E.ptr = __builtin_amdgcn_readfirstlane(E.ptr);
 		E.sf = __builtin_amdgcn_readfirstlane(E.sf), E.job = __builtin_amdgcn_readfirstlane(E.job);
 		E.loff = __builtin_amdgcn_readfirstlane(E.loff), E.llen = __builtin_amdgcn_readfirstlane(E.llen);
+		const bool writer = kMode != kLeafNodes && leaf_writes_colors(P, lv, E);
+		if (kMode == kLeafColors && !writer)
+			continue;
 		u32 w0 = 0, w1 = 0;
 		if (E.ptr != kNull) {
 			w0 = ldg(P.words, E.ptr);
@@ -1930,7 +1939,6 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		u32 vx = (E.x << 2) | ((lane >> 2) & 2u) | (lane & 1u);
 		u32 vy = (E.y << 2) | ((lane >> 3) & 2u) | ((lane >> 1) & 1u);
 		u32 vz = (E.z << 2) | ((lane >> 4) & 2u) | ((lane >> 2) & 1u);
-		const bool writer = leaf_writes_colors(P, lv, E);
 		Color col = Color::none();
 		bool asg = false;
 		bool nv = v;
@@ -1948,6 +1956,8 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 			if (lane == 0)
 				lmask[e] = am;
 		}
+		if (kMode == kLeafColors)
+			continue;
 		unsigned long long m = __ballot(nv);
 		u32 n0 = (u32)m, n1 = (u32)(m >> 32);
 		bool pending = (n0 != w0 || n1 != w1) && m != 0ull; // wavefront-uniform
@@ -2492,7 +2502,7 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const
 // Colored passes whose color leaves hold at most kJobV voxels (so at most two node levels lie
 // between a job and its 4^3 leaves). a_job_ranges, one thread per writer job: its contiguous entry
 // ranges at those levels (from first_child / pmask) in jr[2 j], and the header of its current
-// chunk {M, B, block size, 1} (or zeros for a uniform color) in jr[2 j + 1].
+// chunk {M, B, block size, weight words} (or zeros for a uniform color) in jr[2 j + 1].
 constexpr u32 kJobV = 4096;
 __global__ void a_job_ranges(const ALevel *__restrict__ lv, const u32 *__restrict__ pm, u32 leafL, u32 L,
                              const u32 *__restrict__ cleaves, uint4 *__restrict__ jr) {
@@ -2505,7 +2515,7 @@ __global__ void a_job_ranges(const ALevel *__restrict__ lv, const u32 *__restric
 		uint4 hdr = make_uint4(0, 0, 0, 0);
 		if (ctag(oct) == kTagLeaf)
 			hdr = make_uint4(ldg(cleaves, (u64)cdata(oct) + 1u), ldg(cleaves, (u64)cdata(oct) + 2u),
-			                 ldg(cleaves, (u64)cdata(oct)), 1u);
+			                 ldg(cleaves, (u64)cdata(oct)), ldg(cleaves, (u64)cdata(oct) + 3u));
 		u32 lo = j, hi = j + 1u, r[4] = {0, 0, 0, 0};
 #pragma unroll
 		for (u32 k = 0; k < 2; ++k) {
@@ -2540,6 +2550,7 @@ constexpr u32 kJobHdr = 3072; // staged header words (larger chunks are decoded 
 __device__ inline u32 pack_wb16(u32 wb) { return (wb & 0xFFu) | (((wb >> 8) & 0x7Fu) << 8) | (((wb >> 16) & 1u) << 15); }
 __device__ inline u32 unpack_wb16(u32 w) { return (w & 0xFFu) | (((w >> 8) & 0x7Fu) << 8) | ((w >> 15) << 16); }
 static_assert(kJobHdr <= kEncTmp, "a staged chunk fits its kEncTmp-word slot");
+static_assert(kJobHdr % kEncThreads == 0 && kEncThreads >= 72, "staged words split evenly; one push per thread");
 struct JobLds {
 	u32 colc[kJobV];
 	unsigned short colw[kJobV];
@@ -2563,15 +2574,16 @@ __device__ inline void job_set(JobLds &S, u32 v, Color c) {
 	S.colw[v] = (unsigned short)pack_wb16(c.wb);
 }
 // enc_load over the job's LDS colors.
-__device__ inline void enc_load_job(const JobLds &S, u32 V, EncRegs &R) {
-	const u32 per = (V + kEncThreads - 1) / kEncThreads;
+// kFull: V == kJobV, so every thread holds exactly kEncPer voxels.
+template <bool kFull> __device__ inline void enc_load_job(const JobLds &S, u32 V, EncRegs &R) {
+	const u32 per = kFull ? kEncPer : (V + kEncThreads - 1) / kEncThreads;
 	R.v0 = threadIdx.x * per;
-	const u32 v1 = min(V, R.v0 + per);
+	const u32 v1 = kFull ? R.v0 + kEncPer : min(V, R.v0 + per);
 	R.n = R.v0 < v1 ? v1 - R.v0 : 0u;
 	u32 pc = 0, pb = 0;
 	if (R.n && R.v0)
 		pc = S.colc[R.v0 - 1], pb = (S.colw[R.v0 - 1] >> 8) & 0x7Fu;
-	if (R.n == kEncPer) {
+	if (kFull || R.n == kEncPer) {
 		const uint4 *qc = reinterpret_cast<const uint4 *>(S.colc + R.v0);
 		const uint4 *qw = reinterpret_cast<const uint4 *>(S.colw + R.v0);
 #pragma unroll
@@ -2601,7 +2613,7 @@ __device__ inline void enc_load_job(const JobLds &S, u32 V, EncRegs &R) {
 	R.x = EncScan{0, 0};
 #pragma unroll
 	for (u32 k = 0; k < kEncPer; ++k)
-		if (k < R.n) {
+		if (kFull || k < R.n) {
 			const u32 v = R.v0 + k, b = (R.w[k] >> 8) & 0xFFu;
 			if (v == 0 || (v & 0x3FFFu) == 0 || R.c[k] != pc || b != pb)
 				R.heads |= 1u << k;
@@ -2611,6 +2623,7 @@ __device__ inline void enc_load_job(const JobLds &S, u32 V, EncRegs &R) {
 		}
 	R.x.heads = (u32)__builtin_popcount(R.heads);
 }
+template <bool kFull>
 __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, const ALevel *__restrict__ lv,
                                                              u32 *__restrict__ pm, const uint4 *__restrict__ jrange,
                                                              const Color *__restrict__ lcol,
@@ -2636,47 +2649,68 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 		const uint4 jr = jrange[2ull * e], jh = jrange[2ull * e + 1u];
 		__syncthreads();
 		const bool chunk = ctag(E.octree) == kTagLeaf;
-		const u32 idx = cdata(E.octree), M = jh.x, B0 = jh.y, block = jh.z;
-		// pushes (the job's own 8 pairs, then the pairs of its entries one level down) and the
-		// leaves' records, each in one round trip
-		{
-			const u32 np0 = below ? 8u : 0u, np1 = below > 1u ? 8u * (jr.y - jr.x) : 0u;
-			for (u32 t = threadIdx.x; t < np0 + np1; t += kEncThreads) {
-				const u32 l = t < np0 ? leafL : leafL + 1u;
-				const u32 pe = t < np0 ? e : jr.x + ((t - np0) >> 3), i = t & 7u;
-				const u32 pmw = ldg(lv[l].pmask, pe);
-				const uint4 q = ldg(lv[l].pev, 8ull * pe + i); // read ahead: valid where the push bit is set
-				if ((pmw >> (16u + i)) & 1u)
-					S.fill[atomicAdd(&S.nfill, 1u)] = make_uint4(q.x, q.y, q.z, 1u << (3u * (vl - (l + 1u))));
-			}
+		const u32 idx = cdata(E.octree), M = jh.x, B0 = jh.y, block = jh.z, W0 = jh.w;
+		// pushes (the job's own 8 pairs, then the pairs of its entries one level down: at most 72),
+		// the leaves' records, the chunk's headers and its weight words when they fit beside them (at
+		// most 384 words for 4096 voxels of <= 3 bits) and the leaves' colors: every load is issued
+		// before the first LDS store, so they share one round trip
+		const u32 t0 = threadIdx.x;
+		const u32 np0 = below ? 8u : 0u, np1 = below > 1u ? 8u * (jr.y - jr.x) : 0u;
+		u32 pmw = 0, pl = leafL;
+		uint4 q = make_uint4(0, 0, 0, 0);
+		if (t0 < np0 + np1) {
+			pl = t0 < np0 ? leafL : leafL + 1u;
+			const u32 pe = t0 < np0 ? e : jr.x + ((t0 - np0) >> 3);
+			pmw = ldg(lv[pl].pmask, pe);
+			q = ldg(lv[pl].pev, 8ull * pe + (t0 & 7u)); // read ahead: valid where the push bit is set
 		}
 		const ALevel &LA = lv[L - 1u];
 		const u32 llo = below == 0u ? e : (below == 1u ? jr.x : jr.z);
 		const u32 nl = below == 0u ? 1u : (below == 1u ? jr.y - jr.x : jr.w - jr.z);
-		if (threadIdx.x < nl) {
-			const u32 le = llo + threadIdx.x, m2 = (1u << P.leaf_bits >> 2) - 1u;
-			Entry LE{};
+		Entry LE{};
+		u64 am = 0;
+		if (t0 < nl) {
+			const u32 le = llo + t0;
 			LE.sf = ldg(&LA.ent[le].sf, 0);
 			LE.job = ldg(&LA.ent[le].job, 0);
 			LE.x = ldg(&LA.ent[le].x, 0);
 			LE.y = ldg(&LA.ent[le].y, 0);
 			LE.z = ldg(&LA.ent[le].z, 0);
-			const u64 am = ldg(lmask, le);
-			// a leaf's lane is the Morton index of its voxel inside the 4^3 leaf
-			S.lbase[threadIdx.x] = leaf_writes_colors(P, lv, LE) ? morton3(LE.x & m2, LE.y & m2, LE.z & m2) << 6 : kNull;
-			S.lmask[threadIdx.x] = am;
+			am = ldg(lmask, le);
 		}
 		const bool staged_hdr = chunk && 2u * (M + B0) <= kJobHdr;
-		if (staged_hdr)
-			for (u32 k = threadIdx.x; k < 2u * (M + B0); k += kEncThreads)
-				S.hdr[k] = cleaves[idx + 4u + k];
-		constexpr u32 kPer = kJobV / kEncThreads;
-		Color c[kPer]; // the leaves' colors, read ahead of their masks
+		const bool staged_w = staged_hdr && W0 <= kJobHdr && 2u * (M + B0) + W0 <= kJobHdr;
+		const u32 nk = staged_hdr ? 2u * (M + B0) + (staged_w ? W0 : 0u) : 0u;
+		constexpr u32 kHdrPer = kJobHdr / kEncThreads;
+		u32 hw[kHdrPer];
 #pragma unroll
-		for (u32 r = 0; r < kPer; ++r) {
-			const u32 k = threadIdx.x + r * kEncThreads;
-			if ((k >> 6) < nl)
-				c[r] = ldg(lcol, 64ull * llo + k);
+		for (u32 r = 0; r < kHdrPer; ++r) {
+			const u32 k = t0 + r * kEncThreads;
+			if (k < nk)
+				hw[r] = cleaves[idx + 4u + k];
+		}
+		constexpr u32 kPer = kJobV / kEncThreads;
+		// the leaves' colors, read ahead of their masks: a thread's r-th voxel lies in leaf
+		// wave + 4 r (wave-uniform)
+		const u32 wv = __builtin_amdgcn_readfirstlane(t0 >> 6);
+		Color c[kPer];
+#pragma unroll
+		for (u32 r = 0; r < kPer; ++r)
+			if (wv + r * (kEncThreads / 64u) < nl)
+				c[r] = ldg(lcol, 64ull * llo + t0 + r * kEncThreads);
+		if (t0 < np0 + np1 && ((pmw >> (16u + (t0 & 7u))) & 1u))
+			S.fill[atomicAdd(&S.nfill, 1u)] = make_uint4(q.x, q.y, q.z, 1u << (3u * (vl - (pl + 1u))));
+		if (t0 < nl) {
+			const u32 m2 = (1u << P.leaf_bits >> 2) - 1u;
+			// a leaf's lane is the Morton index of its voxel inside the 4^3 leaf
+			S.lbase[t0] = leaf_writes_colors(P, lv, LE) ? morton3(LE.x & m2, LE.y & m2, LE.z & m2) << 6 : kNull;
+			S.lmask[t0] = am;
+		}
+#pragma unroll
+		for (u32 r = 0; r < kHdrPer; ++r) {
+			const u32 k = t0 + r * kEncThreads;
+			if (k < nk)
+				S.hdr[k] = hw[r];
 		}
 		__syncthreads();
 		// the pushed and assigned colors (loads of all of a thread's voxels first), marked covered. A
@@ -2701,7 +2735,7 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 		}
 #pragma unroll
 		for (u32 r = 0; r < kPer; ++r) {
-			const u32 k = threadIdx.x + r * kEncThreads, li = k >> 6, lane = k & 63u;
+			const u32 li = wv + r * (kEncThreads / 64u), lane = t0 & 63u;
 			if (li < nl && S.lbase[li] != kNull && ((S.lmask[li] >> lane) & 1ull)) {
 				const u32 at = S.lbase[li] | lane;
 				job_set(S, at, c[r]);
@@ -2747,9 +2781,17 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 						c = Color{hc, 1u << 16};
 					else {
 						const u32 bit = ws + (hp & 0xFFFFu) + (v - (hp >> 18)) * bpw, o = bit & 31u;
-						u32 w = cleaves[wo + (bit >> 5)] >> o;
-						if (o + bpw > 32u)
-							w |= cleaves[wo + (bit >> 5) + 1u] << (32u - o);
+						u32 w, w1 = 0u;
+						if (staged_w) {
+							w = S.hdr[2u * (M + B0) + (bit >> 5)];
+							if (o + bpw > 32u)
+								w1 = S.hdr[2u * (M + B0) + (bit >> 5) + 1u];
+						} else {
+							w = cleaves[wo + (bit >> 5)];
+							if (o + bpw > 32u)
+								w1 = cleaves[wo + (bit >> 5) + 1u];
+						}
+						w = (w >> o) | (o + bpw > 32u ? w1 << (32u - o) : 0u);
 						w &= (1u << bpw) - 1u;
 						c = Color{hc, w | (bpw << 8) | (1u << 16)};
 					}
@@ -2760,7 +2802,7 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 		__syncthreads();
 		// encode: a_encode_size's register path over the LDS colors
 		EncRegs R;
-		enc_load_job(S, V, R);
+		enc_load_job<kFull>(S, V, R);
 		EncScan tot;
 		EncScan run = block_excl_scan(R.x, S.sh, tot);
 		const u32 B = tot.heads, W = (tot.bits + 31u) >> 5, ds = 6u + 2u * B + W;
@@ -2780,7 +2822,7 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 			__syncthreads();
 #pragma unroll
 			for (u32 k = 0; k < kEncPer; ++k)
-				if (k < R.n) {
+				if (kFull || k < R.n) {
 					const u32 v = R.v0 + k, bpw = (R.w[k] >> 8) & 0xFFu;
 					if ((R.heads >> k) & 1u) {
 						S.stmp[bo + 2u * run.heads] = R.c[k];
@@ -4619,7 +4661,17 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 			                                                           B.lv[leafL].cap, B.dense);
 	}
 	const u32 leaf_grid = grid_for(pp.cap[L - 1], kAT / 64, 16384);
-	a_leaf<<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
+	// with LDS jobs, the leaves' colors start the color stream right after the top-down
+	const bool split = colors && ljobs;
+	if (split) {
+		HD_TRY(hipEventRecord(p.ev_top, p.stream));
+		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_top, 0));
+		a_leaf<kLeafColors><<<leaf_grid, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
+		a_leaf<kLeafNodes><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
+	} else if (colors)
+		a_leaf<kLeafAll><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
+	else
+		a_leaf<kLeafNodes><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
 	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
 	const u64 small_level = std::min<u32>(g_tuning.small_level, kSmallLevel);
 	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= small_level)
@@ -4630,14 +4682,17 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	bool colors_queued = !colors;
 	auto queue_colors = [&]() -> int {
 		colors_queued = true;
-		HD_TRY(hipEventRecord(p.ev_leaf, p.stream));
-		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_leaf, 0));
+		if (!split) {
+			HD_TRY(hipEventRecord(p.ev_leaf, p.stream));
+			HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_leaf, 0));
+		}
 		u32 G = grid_for(pp.cap[leafL], 1, 8192);
 		if (ljobs) {
 			a_job_ranges<<<grid_for(pp.cap[leafL], 256, 4096), 256, 0, p.cstream>>>(B.d_lv, B.pm, leafL, L, p.cleaves,
 			                                                                       B.jrange);
-			a_job_encode<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.jrange, B.lcol, B.lmask, B.dense, V,
-			                                               B.info, B.app, B.etmp, B.rest);
+			const auto job = V == kJobV ? a_job_encode<true> : a_job_encode<false>;
+			job<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.jrange, B.lcol, B.lmask, B.dense, V, B.info, B.app,
+			                                      B.etmp, B.rest);
 		}
 		else {
 			a_leafcolor<<<leaf_grid, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.dense, V);
@@ -4655,7 +4710,8 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		HD_TRY(hipEventRecord(p.ev_color, p.cstream));
 		return HD_OK;
 	};
-	const int color_after = (int)L - 1; // after the leaf level's walk (measured best)
+	// without the split, the color stream starts after the leaf level's walk (measured best)
+	const int color_after = split ? (int)L : (int)L - 1;
 	if (color_after >= (int)L)
 		HD_RET(queue_colors());
 	for (int l = (int)L - 1; l > l_from; --l) {
