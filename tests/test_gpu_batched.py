@@ -173,3 +173,34 @@ def test_reserve_changes_nothing_and_edits_match():
     assert np.array_equal(a.live_words()[1], b.live_words()[1])
     assert np.array_equal(a.color_nodes(), b.color_nodes())
     assert np.array_equal(a.color_leaves(), b.color_leaves())
+
+
+@pytest.mark.parametrize("depth", [3, 4])
+@pytest.mark.parametrize("words", [0, 24])
+def test_unstaged_color_jobs_match_staged(depth, words, monkeypatch):
+    # A color job whose encoded chunk does not fit the staging area is rebuilt and encoded straight
+    # into the color leaves after the append scan. With no staging (every job) or a little (some
+    # jobs) the edits give the same roots, node words, color nodes and color leaves as with the
+    # default staging. depth 3: 512-voxel color leaves; depth 4: 4096-voxel ones.
+    capi = _capi()
+    res = 128
+    h = W.value_noise_heightmap(res, 4, 2, 20, 100)
+    cfg = small_cfg(res, 8, 9, 2, depth)
+    t = W.terrain_edit(res, seed=3, skin=6)
+    e = W.mixed_batch(200, res, h, 2, 24, seed=5)
+    pools = []
+    for env in (None, str(words)):
+        if env is None:
+            monkeypatch.delenv("HD_EDIT_STAGING_WORDS", raising=False)
+        else:
+            monkeypatch.setenv("HD_EDIT_STAGING_WORDS", env)
+        p = capi.DevicePool(cfg, heights=h)
+        p.edit(t)
+        p.edit(e, capi.MODE_BATCHED)
+        pools.append(p)
+    monkeypatch.delenv("HD_EDIT_STAGING_WORDS", raising=False)
+    a, b = pools
+    assert (a.root, a.color_root) == (b.root, b.color_root)
+    assert np.array_equal(a.live_words()[1], b.live_words()[1])
+    assert np.array_equal(a.color_nodes(), b.color_nodes())
+    assert np.array_equal(a.color_leaves(), b.color_leaves())

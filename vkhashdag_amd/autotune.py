@@ -33,7 +33,7 @@ CANDIDATES = {
     "trace_tile": [8, 16],
     "top_pairs": [1024, 2048, 4096],
     "small_level": [256, 512, 1024],
-    "batch_pass_edits": [0, 256, 1024],
+    "batch_pass_edits": [0, 512, 1024],
 }
 
 
