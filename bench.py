@@ -442,6 +442,7 @@ def config4_section(args):
     counters.upload(np.zeros(4, np.uint64))
     capi.check(capi.lib().hd_trace_async(pool.h, pool.root, pool.color_root, dv.ptr, dt.ptr, len(views),
                                          int(tiles[-1]), rgba.ptr, None, counters.ptr))
+    pool.sync()  # the trace runs on the pool's stream; the copy below does not wait for it
     c = counters.download(np.uint64, 4)
     pool.timer_begin()
     capi.check(capi.lib().hd_trace_async(pool.h, pool.root, pool.color_root, dv.ptr, dt.ptr, len(views),

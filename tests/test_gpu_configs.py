@@ -105,7 +105,7 @@ def test_config3_batched_4096_mixed_equals_sequential(heights_4096):
     ora.edit(t)
     batch = W.mixed_batch(4096, res, h, 2, 256, seed=2)
     dev.edit(batch, capi.MODE_BATCHED)
-    assert dev.stats().last_groups > 1
+    assert 1 <= dev.stats().last_groups < len(batch)  # folded: far fewer passes than edits
     ora.edit(batch)
     assert ora.overflows() == 0 and dev.stats().overflow == 0
     mirror = O.OraclePool(cfg)  # the device's words, read by the oracle's checkers

@@ -67,7 +67,34 @@ struct EditParams {
 		e.voxel_level = c.voxel_level;
 		return e;
 	}
+	// An edit descriptor held across a wavefront's lanes: lane j loads the j-th edit of a list
+	// (every load of a list at once), and the loop over the list reads edit j back with readlane.
+	static constexpr u32 kDescWords = sizeof(hd_edit_desc) / 4;
+	union DescWords {
+		hd_edit_desc d;
+		u32 w[kDescWords];
+	};
+	__device__ void load_desc(u32 k, DescWords &out) const {
+		const uint2 *src = reinterpret_cast<const uint2 *>(edits + k);
+#pragma unroll
+		for (u32 i = 0; i < kDescWords / 2; ++i) {
+			const uint2 v = ldg(src, i);
+			out.w[2 * i] = v.x, out.w[2 * i + 1] = v.y;
+		}
+	}
+	__device__ DevEditor lane_editor(const DescWords &mine, u32 j) const {
+		DescWords t;
+#pragma unroll
+		for (u32 i = 0; i < kDescWords; ++i)
+			t.w[i] = (u32)__builtin_amdgcn_readlane((int)mine.w[i], (int)j);
+		DevEditor e;
+		e.d = t.d;
+		e.hm = hm;
+		e.voxel_level = c.voxel_level;
+		return e;
+	}
 };
+static_assert(sizeof(hd_edit_desc) % 8 == 0 && alignof(hd_edit_desc) >= 8, "descriptors load as uint2");
 
 __device__ inline u32 color_child(const u32 *cnodes, u32 p, u32 i) { // DAGColorPool::GetChild (:139-143)
 	u32 t = ctag(p);
@@ -1942,12 +1969,18 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		Color col = Color::none();
 		bool asg = false;
 		bool nv = v;
-		for (u32 k = 0; k < E.llen; ++k) {
-			DevEditor ed = P.editor(E.llen == 1 ? E.loff : ldg(A.lists, (u64)E.loff + k));
-			if (!ed.stateful())
-				nv = ed.edit_voxel(vx, vy, vz, nv);
-			else
-				nv = ed.edit_voxel_vbr_asg(vx, vy, vz, nv, col, asg);
+		for (u32 k0 = 0; k0 < E.llen; k0 += 64u) {
+			const u32 nk = min(64u, E.llen - k0);
+			EditParams::DescWords mine{}; // lane j: the list's (k0 + j)-th edit
+			if (lane < nk)
+				P.load_desc(E.llen == 1 ? E.loff : ldg(A.lists, (u64)E.loff + k0 + lane), mine);
+			for (u32 j = 0; j < nk; ++j) {
+				const DevEditor ed = P.lane_editor(mine, j);
+				if (!ed.stateful())
+					nv = ed.edit_voxel(vx, vy, vz, nv);
+				else
+					nv = ed.edit_voxel_vbr_asg(vx, vy, vz, nv, col, asg);
+			}
 		}
 		if (writer) {
 			const u64 am = __ballot(asg);
@@ -2570,8 +2603,14 @@ __device__ inline void job_set(JobLds &S, u32 v, Color c) {
 	S.colw[v] = (unsigned short)pack_wb16(c.wb);
 }
 // enc_load over the job's LDS colors.
+// A thread's voxels of a job (v0 .. v0 + n - 1): which start a header, and the totals for the
+// block scan. The colors stay in LDS; the writers read them again.
+struct JobEnc {
+	u32 v0, n, heads;
+	EncScan x;
+};
 // kFull: V == kJobV, so every thread holds exactly kEncPer voxels.
-template <bool kFull> __device__ inline void enc_load_job(const JobLds &S, u32 V, EncRegs &R) {
+template <bool kFull> __device__ inline void enc_scan_job(const JobLds &S, u32 V, JobEnc &R) {
 	const u32 per = kFull ? kEncPer : (V + kEncThreads - 1) / kEncThreads;
 	R.v0 = threadIdx.x * per;
 	const u32 v1 = kFull ? R.v0 + kEncPer : min(V, R.v0 + per);
@@ -2579,44 +2618,33 @@ template <bool kFull> __device__ inline void enc_load_job(const JobLds &S, u32 V
 	u32 pc = 0, pb = 0;
 	if (R.n && R.v0)
 		pc = S.colc[R.v0 - 1], pb = (S.colw[R.v0 - 1] >> 8) & 0x7Fu;
+	R.heads = 0;
+	R.x = EncScan{0, 0};
+	auto step = [&](u32 k, u32 c, u32 w16) {
+		const u32 v = R.v0 + k, b = (w16 >> 8) & 0x7Fu;
+		if (v == 0 || (v & 0x3FFFu) == 0 || c != pc || b != pb)
+			R.heads |= 1u << k;
+		R.x.bits += b;
+		pc = c;
+		pb = b;
+	};
 	if (kFull || R.n == kEncPer) {
 		const uint4 *qc = reinterpret_cast<const uint4 *>(S.colc + R.v0);
 		const uint4 *qw = reinterpret_cast<const uint4 *>(S.colw + R.v0);
 #pragma unroll
-		for (u32 k = 0; k < kEncPer / 4; ++k) {
-			const uint4 t = qc[k];
-			R.c[4 * k] = t.x, R.c[4 * k + 1] = t.y, R.c[4 * k + 2] = t.z, R.c[4 * k + 3] = t.w;
-		}
+		for (u32 q = 0; q < kEncPer / 8; ++q) { // 8 voxels a step: their colors and weight words
+			const uint4 c0 = qc[2 * q], c1 = qc[2 * q + 1], w = qw[q];
+			const u32 cs[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w}, ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-		for (u32 k = 0; k < kEncPer / 8; ++k) {
-			const uint4 t = qw[k];
-			const u32 h[4] = {t.x, t.y, t.z, t.w};
-#pragma unroll
-			for (u32 j = 0; j < 4; ++j) {
-				R.w[8 * k + 2 * j] = unpack_wb16(h[j] & 0xFFFFu);
-				R.w[8 * k + 2 * j + 1] = unpack_wb16(h[j] >> 16);
-			}
+			for (u32 j = 0; j < 8; ++j)
+				step(8 * q + j, cs[j], (ws[j >> 1] >> (16 * (j & 1))) & 0xFFFFu);
 		}
 	} else {
 #pragma unroll
 		for (u32 k = 0; k < kEncPer; ++k)
-			if (k < R.n) {
-				R.c[k] = S.colc[R.v0 + k];
-				R.w[k] = unpack_wb16(S.colw[R.v0 + k]);
-			}
+			if (k < R.n)
+				step(k, S.colc[R.v0 + k], S.colw[R.v0 + k]);
 	}
-	R.heads = 0;
-	R.x = EncScan{0, 0};
-#pragma unroll
-	for (u32 k = 0; k < kEncPer; ++k)
-		if (kFull || k < R.n) {
-			const u32 v = R.v0 + k, b = (R.w[k] >> 8) & 0xFFu;
-			if (v == 0 || (v & 0x3FFFu) == 0 || R.c[k] != pc || b != pb)
-				R.heads |= 1u << k;
-			R.x.bits += b;
-			pc = R.c[k];
-			pb = b;
-		}
 	R.x.heads = (u32)__builtin_popcount(R.heads);
 }
 // Rebuilds job e's colors in S (every thread of the block calls it): the colors pushed onto its
@@ -2803,7 +2831,7 @@ constexpr u32 kEtmpJobWords = 512;
 // workgroup's share of etmp (etmp_cap words); a larger one, or one that no longer fits, is listed for
 // a_job_rest, which rebuilds and encodes it in place once the append offsets are known.
 template <bool kFull>
-__global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, const ALevel *__restrict__ lv,
+__global__ void __launch_bounds__(kEncThreads, 4) a_job_encode(EditParams P, const ALevel *__restrict__ lv,
                                                              u32 *__restrict__ pm, const uint4 *__restrict__ jrange,
                                                              const Color *__restrict__ lcol,
                                                              const u64 *__restrict__ lmask, u32 V,
@@ -2829,8 +2857,8 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 		const bool chunk = ctag(E.octree) == kTagLeaf;
 		const u32 block = jh.z;
 		// encode: a_encode_size's register path over the LDS colors
-		EncRegs R;
-		enc_load_job<kFull>(S, V, R);
+		JobEnc R;
+		enc_scan_job<kFull>(S, V, R);
 		EncScan tot;
 		EncScan run = block_excl_scan(R.x, S.sh, tot);
 		const u32 B = tot.heads, W = (tot.bits + 31u) >> 5, ds = 6u + 2u * B + W;
@@ -2853,14 +2881,14 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_encode(EditParams P, con
 #pragma unroll
 			for (u32 k = 0; k < kEncPer; ++k)
 				if (kFull || k < R.n) {
-					const u32 v = R.v0 + k, bpw = (R.w[k] >> 8) & 0xFFu;
+					const u32 v = R.v0 + k, wb = unpack_wb16(S.colw[v]), bpw = (wb >> 8) & 0xFFu;
 					if ((R.heads >> k) & 1u) {
-						S.stmp[bo + 2u * run.heads] = R.c[k];
+						S.stmp[bo + 2u * run.heads] = S.colc[v];
 						S.stmp[bo + 2u * run.heads + 1u] = (v << 18) | (bpw << 16) | run.bits;
 						run.heads += 1u;
 					}
 					if (bpw) {
-						const u32 w = R.w[k] & ((1u << bpw) - 1u), o = run.bits & 31u;
+						const u32 w = wb & ((1u << bpw) - 1u), o = run.bits & 31u;
 						atomicOr(&S.stmp[wo + (run.bits >> 5)], w << o);
 						if (o + bpw > 32u)
 							atomicOr(&S.stmp[wo + (run.bits >> 5) + 1u], w >> (32u - o));
@@ -2912,8 +2940,8 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_rest(EditParams P, const
 		Entry E;
 		uint4 jh;
 		job_rebuild(P, lv, jrange, lcol, lmask, V, e, S, E, jh);
-		EncRegs R;
-		enc_load_job<kFull>(S, V, R);
+		JobEnc R;
+		enc_scan_job<kFull>(S, V, R);
 		EncScan tot;
 		EncScan run = block_excl_scan(R.x, S.sh, tot);
 		const u32 ds = info[4 * e], as = info[4 * e + 1], inplace = info[4 * e + 2] & 1u;
@@ -2938,14 +2966,14 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_rest(EditParams P, const
 #pragma unroll
 			for (u32 k = 0; k < kEncPer; ++k)
 				if (kFull || k < R.n) {
-					const u32 v = R.v0 + k, bpw = (R.w[k] >> 8) & 0xFFu;
+					const u32 v = R.v0 + k, wb = unpack_wb16(S.colw[v]), bpw = (wb >> 8) & 0xFFu;
 					if ((R.heads >> k) & 1u) {
-						Lw[bo + 2u * run.heads] = R.c[k];
+						Lw[bo + 2u * run.heads] = S.colc[v];
 						Lw[bo + 2u * run.heads + 1u] = (v << 18) | (bpw << 16) | run.bits;
 						run.heads += 1u;
 					}
 					if (bpw) {
-						const u32 w = R.w[k] & ((1u << bpw) - 1u), o = run.bits & 31u;
+						const u32 w = wb & ((1u << bpw) - 1u), o = run.bits & 31u;
 						atomicOr(&Lw[wo + (run.bits >> 5)], w << o);
 						if (o + bpw > 32u)
 							atomicOr(&Lw[wo + (run.bits >> 5) + 1u], w >> (32u - o));
@@ -4387,7 +4415,7 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 }
 
 // ---------------------------------------------------------------- sync-free pass: host side
-constexpr u64 kAsyncBudget = 32ull << 30; // scratch bytes a sync-free pass may use
+constexpr u64 kAsyncBudget = 72ull << 30; // scratch bytes a sync-free pass may use (a quarter of HBM)
 
 // Upper bounds on the proceeding nodes of a pass, per level. A node proceeds only where some
 // edit's boundary crosses it: for a sphere the closed node cube [lb, lb + S]^3 meets the sphere
@@ -4943,10 +4971,11 @@ static void plain_plans(const hd_edit_desc *edits, u32 n, std::vector<hd_edit_de
 // path the sequential edits take runs in a pass of its own, between the runs: a weighted color (a
 // whole-node fill stores it RGB8, a voxel keeps it exact), a column fill, and an uncolored fill
 // among colored edits (it refills voxels with the colors earlier edits left on them). Runs of up
-// to 2048 edits: a larger pass shares its fixed costs (the top-down's first levels, the small
+// to 4096 edits: a larger pass shares its fixed costs (the top-down's first levels, the small
 // levels, SetNode) among more edits and re-encodes a color leaf that several of its edits touch
-// once (4096 mixed edits on the 4096^3 terrain: runs of 512 43 ms, 1024 39 ms, 2048 35 ms).
-constexpr u32 kFoldedPassEdits = 2048;
+// once (4096 mixed edits on the 4096^3 terrain: runs of 512 43 ms, 1024 39 ms, 2048 34 ms, all
+// 4096 in one pass 28 ms).
+constexpr u32 kFoldedPassEdits = 4096;
 static bool folds(const hd_edit_desc &d, bool colored_batch) {
 	if (d.kind == HD_EDIT_COLUMN_FILL)
 		return false;
