@@ -4768,7 +4768,9 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	const u32 V = pp.stateful ? 1u << (3u * P.leaf_bits) : 0u;
 	// A pair of a multi-edit pass tests its parent's edit list: count those tests too (lists hold at
 	// most the pass's edits, and shrink with depth), so list-heavy levels go to the wide kernels
-	const u64 lf = (std::min<u32>(pp.count, 128) + 7) / 8;
+	// (a pass of thousands of edits leaves only the root to the one workgroup: the wide kernels take
+	// the level below's nodes in parallel, 4096 mixed edits 0.8 ms faster)
+	const u64 lf = (std::min<u32>(pp.count, 4096) + 7) / 8;
 	u32 top_end = 0;
 	const u64 top_pairs = std::min<u64>(g_tuning.top_pairs, kTopPairs);
 	while (top_end + 1 < L && 8 * pp.cap[top_end] * lf <= top_pairs)
