@@ -191,14 +191,15 @@ struct DevEditor {
 	const DevHeightmap *hm; // device copy of the pool's heightmap view
 	u32 voxel_level;
 
+	// the kinds in enum order: sphere fill, dig, paint, box fill, dig, paint, column fill
+	static_assert(HD_EDIT_SPHERE_FILL == 0 && HD_EDIT_SPHERE_DIG == 1 && HD_EDIT_SPHERE_PAINT == 2 &&
+	                  HD_EDIT_AABB_FILL == 3 && HD_EDIT_AABB_DIG == 4 && HD_EDIT_AABB_PAINT == 5 &&
+	                  HD_EDIT_COLUMN_FILL == 6,
+	              "kind ranges");
 	__device__ bool is_dig() const { return d.kind == HD_EDIT_SPHERE_DIG || d.kind == HD_EDIT_AABB_DIG; }
 	__device__ bool is_paint() const { return d.kind == HD_EDIT_SPHERE_PAINT || d.kind == HD_EDIT_AABB_PAINT; }
-	__device__ bool is_box() const {
-		return d.kind == HD_EDIT_AABB_FILL || d.kind == HD_EDIT_AABB_DIG || d.kind == HD_EDIT_AABB_PAINT;
-	}
-	__device__ bool is_sphere() const {
-		return d.kind == HD_EDIT_SPHERE_FILL || d.kind == HD_EDIT_SPHERE_DIG || d.kind == HD_EDIT_SPHERE_PAINT;
-	}
+	__device__ bool is_box() const { return d.kind - (u32)HD_EDIT_AABB_FILL <= 2u; }
+	__device__ bool is_sphere() const { return d.kind <= (u32)HD_EDIT_SPHERE_PAINT; }
 	__device__ bool stateful() const { return (d.colored || is_paint()) && !is_dig(); }
 	__device__ Color color() const { return from_hd(d.color); }
 
@@ -383,6 +384,15 @@ __host__ __device__ inline u32 morton_part(u32 u) {
 	return u;
 }
 __host__ __device__ inline u32 morton3(u32 x, u32 y, u32 z) { return morton_part(x) | (morton_part(y) << 1) | (morton_part(z) << 2); }
+// The inverse of morton_part: bits 0, 3, 6, ... of m, packed (x of a Morton index m; y of m >> 1).
+__host__ __device__ inline u32 morton_compact(u32 m) {
+	m &= 0x09249249u;
+	m = (m | (m >> 2u)) & 0x030C30C3u;
+	m = (m | (m >> 4u)) & 0x0300F00Fu;
+	m = (m | (m >> 8u)) & 0x030000FFu;
+	m = (m | (m >> 16u)) & 0x000003FFu;
+	return m;
+}
 
 
 // Loads and stores through a pointer read from memory (a struct field): the compiler cannot tell

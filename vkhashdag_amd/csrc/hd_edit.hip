@@ -61,32 +61,16 @@ struct EditParams {
 	u32 octree_root;
 	u32 leaf_bits; // voxel_level - color_leaf_level
 	__device__ DevEditor editor(u32 k) const {
-		DevEditor e;
-		e.d = edits[k];
-		e.hm = hm;
-		e.voxel_level = c.voxel_level;
-		return e;
-	}
-	// An edit descriptor held across a wavefront's lanes: lane j loads the j-th edit of a list
-	// (every load of a list at once), and the loop over the list reads edit j back with readlane.
-	static constexpr u32 kDescWords = sizeof(hd_edit_desc) / 4;
-	union DescWords {
-		hd_edit_desc d;
-		u32 w[kDescWords];
-	};
-	__device__ void load_desc(u32 k, DescWords &out) const {
-		const uint2 *src = reinterpret_cast<const uint2 *>(edits + k);
+		// whole words (scalar loads for a uniform k; the color's byte fields would otherwise take
+		// vector byte loads)
+		union {
+			hd_edit_desc d;
+			u32 w[sizeof(hd_edit_desc) / 4];
+		} t;
+		const u32 *src = reinterpret_cast<const u32 *>(edits + k);
 #pragma unroll
-		for (u32 i = 0; i < kDescWords / 2; ++i) {
-			const uint2 v = ldg(src, i);
-			out.w[2 * i] = v.x, out.w[2 * i + 1] = v.y;
-		}
-	}
-	__device__ DevEditor lane_editor(const DescWords &mine, u32 j) const {
-		DescWords t;
-#pragma unroll
-		for (u32 i = 0; i < kDescWords; ++i)
-			t.w[i] = (u32)__builtin_amdgcn_readlane((int)mine.w[i], (int)j);
+		for (u32 i = 0; i < sizeof(hd_edit_desc) / 4; ++i)
+			t.w[i] = src[i];
 		DevEditor e;
 		e.d = t.d;
 		e.hm = hm;
@@ -94,7 +78,6 @@ struct EditParams {
 		return e;
 	}
 };
-static_assert(sizeof(hd_edit_desc) % 8 == 0 && alignof(hd_edit_desc) >= 8, "descriptors load as uint2");
 
 __device__ inline u32 color_child(const u32 *cnodes, u32 p, u32 i) { // DAGColorPool::GetChild (:139-143)
 	u32 t = ctag(p);
@@ -1937,11 +1920,15 @@ __device__ inline bool leaf_writes_colors(const EditParams &P, const ALevel *__r
 // kMode: kLeafAll = occupancy, find and colors; kLeafNodes = occupancy and find only; kLeafColors =
 // the colors of the leaves inside a color job only. A leaf's colors depend on its starting words and
 // its edit list alone, so in a colored pass the color stream runs its own kLeafColors launch right
-// after the top-down, beside the kLeafNodes launch of the node levels.
+// after the top-down, beside the kLeafNodes launch of the node levels. kLeafColors records, per
+// voxel, which of the pass's edits assigned its color last (lidx, kNoEdit for none: 2 bytes instead
+// of the color and a mask bit); the color job reads the color from that edit.
 enum : u32 { kLeafAll = 0, kLeafNodes = 1, kLeafColors = 2 };
+constexpr u32 kNoEdit = 0xFFFFu; // a pass with LDS color jobs holds fewer edits
 template <u32 kMode>
 __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
-                                              Color *__restrict__ lcol, u64 *__restrict__ lmask) {
+                                              Color *__restrict__ lcol, u64 *__restrict__ lmask,
+                                              unsigned short *__restrict__ lidx) {
 	const u32 lane = threadIdx.x & 63u, level = P.c.L - 1;
 	const ALevel A = lv[level];
 	const u32 n = pm[kPmHdr + level];
@@ -1969,18 +1956,22 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 		Color col = Color::none();
 		bool asg = false;
 		bool nv = v;
-		for (u32 k0 = 0; k0 < E.llen; k0 += 64u) {
-			const u32 nk = min(64u, E.llen - k0);
-			EditParams::DescWords mine{}; // lane j: the list's (k0 + j)-th edit
-			if (lane < nk)
-				P.load_desc(E.llen == 1 ? E.loff : ldg(A.lists, (u64)E.loff + k0 + lane), mine);
-			for (u32 j = 0; j < nk; ++j) {
-				const DevEditor ed = P.lane_editor(mine, j);
-				if (!ed.stateful())
-					nv = ed.edit_voxel(vx, vy, vz, nv);
-				else
-					nv = ed.edit_voxel_vbr_asg(vx, vy, vz, nv, col, asg);
-			}
+		u32 last = kNoEdit; // kLeafColors: the last edit that assigned this voxel's color
+		for (u32 k = 0; k < E.llen; ++k) {
+			const u32 idx = E.llen == 1 ? E.loff : ldg(A.lists, (u64)E.loff + k);
+			DevEditor ed = P.editor(idx);
+			if (!ed.stateful())
+				nv = ed.edit_voxel(vx, vy, vz, nv);
+			else if (kMode == kLeafColors) {
+				bool a = false;
+				nv = ed.edit_voxel_vbr_asg(vx, vy, vz, nv, col, a);
+				last = a ? idx : last;
+			} else
+				nv = ed.edit_voxel_vbr_asg(vx, vy, vz, nv, col, asg);
+		}
+		if (kMode == kLeafColors) {
+			lidx[(u64)e * 64u + lane] = (unsigned short)last;
+			continue;
 		}
 		if (writer) {
 			const u64 am = __ballot(asg);
@@ -1989,8 +1980,6 @@ __global__ void __launch_bounds__(kAT) a_leaf(EditParams P, const ALevel *__rest
 			if (lane == 0)
 				lmask[e] = am;
 		}
-		if (kMode == kLeafColors)
-			continue;
 		unsigned long long m = __ballot(nv);
 		u32 n0 = (u32)m, n1 = (u32)(m >> 32);
 		bool pending = (n0 != w0 || n1 != w1) && m != 0ull; // wavefront-uniform
@@ -2537,10 +2526,18 @@ __global__ void __launch_bounds__(kEncThreads) a_encode_size(EditParams P, const
 // ranges at those levels (from first_child / pmask) in jr[2 j], and the header of its current
 // chunk {M, B, block size, weight words} (or zeros for a uniform color) in jr[2 j + 1].
 constexpr u32 kJobV = 4096;
+// Also fills the pass's edit color table (ecol[k]: the color a_leaf's kLeafColors records of edit k
+// stand for; a column fill's, which depends on the voxel, is flagged in bit 24 of the weight word).
 __global__ void a_job_ranges(const ALevel *__restrict__ lv, const u32 *__restrict__ pm, u32 leafL, u32 L,
-                             const u32 *__restrict__ cleaves, uint4 *__restrict__ jr) {
+                             const u32 *__restrict__ cleaves, uint4 *__restrict__ jr, const hd_edit_desc *__restrict__ edits,
+                             u32 n_edits, uint2 *__restrict__ ecol) {
 	const Entry *ent = lv[leafL].ent;
 	const u32 n = pm[kPmHdr + leafL];
+	for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n_edits; k += gridDim.x * blockDim.x) {
+		const hd_edit_desc d = edits[k];
+		const Color c = from_hd(d.color);
+		ecol[k] = make_uint2(c.colors, c.wb | (d.kind == HD_EDIT_COLUMN_FILL ? 1u << 24 : 0u));
+	}
 	for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
 		if (!(ldg(&ent[j].sf, 0) & kSfWriter))
 			continue;
@@ -2593,7 +2590,6 @@ struct JobLds {
 			uint4 fill[kJobV / 64 + 8]; // pushes: {voxel offset, colors, wb, voxel count}
 			u32 nfill;
 			u32 lbase[kJobV / 64];      // per leaf of the job: its first voxel, or kNull without assigned colors
-			u64 lmask[kJobV / 64];
 		};
 		EncScan sh[kEncThreads / 64];
 	};
@@ -2654,7 +2650,7 @@ template <bool kFull> __device__ inline void enc_scan_job(const JobLds &S, u32 V
 // headers staged in LDS) fill the rest. Returns with S complete and visible to the block; E and jh
 // are the job's entry and the header of its current chunk.
 __device__ inline void job_rebuild(const EditParams &P, const ALevel *__restrict__ lv, const uint4 *__restrict__ jrange,
-                                   const Color *__restrict__ lcol, const u64 *__restrict__ lmask, u32 V, u32 e,
+                                   const unsigned short *__restrict__ lidx, const uint2 *__restrict__ ecol, u32 V, u32 e,
                                    JobLds &S, Entry &E, uint4 &jh) {
 	const u32 leafL = P.c.color_leaf, L = P.c.L, vl = P.c.voxel_level;
 	const u32 *cleaves = P.cleaves;
@@ -2689,7 +2685,6 @@ __device__ inline void job_rebuild(const EditParams &P, const ALevel *__restrict
 	const u32 llo = below == 0u ? e : (below == 1u ? jr.x : jr.z);
 	const u32 nl = below == 0u ? 1u : (below == 1u ? jr.y - jr.x : jr.w - jr.z);
 	Entry LE{};
-	u64 am = 0;
 	if (t0 < nl) {
 		const u32 le = llo + t0;
 		LE.sf = ldg(&LA.ent[le].sf, 0);
@@ -2697,7 +2692,6 @@ __device__ inline void job_rebuild(const EditParams &P, const ALevel *__restrict
 		LE.x = ldg(&LA.ent[le].x, 0);
 		LE.y = ldg(&LA.ent[le].y, 0);
 		LE.z = ldg(&LA.ent[le].z, 0);
-		am = ldg(lmask, le);
 	}
 	const bool staged_hdr = chunk && 2u * (M + B0) <= kJobHdr;
 	const bool staged_w = staged_hdr && W0 <= kJobHdr && 2u * (M + B0) + W0 <= kJobHdr;
@@ -2711,21 +2705,19 @@ __device__ inline void job_rebuild(const EditParams &P, const ALevel *__restrict
 			hw[r] = cleaves[idx + 4u + k];
 	}
 	constexpr u32 kPer = kJobV / kEncThreads;
-	// the leaves' colors, read ahead of their masks: a thread's r-th voxel lies in leaf
+	// which edit assigned each leaf voxel's color last: a thread's r-th voxel lies in leaf
 	// wave + 4 r (wave-uniform)
 	const u32 wv = __builtin_amdgcn_readfirstlane(t0 >> 6);
-	Color c[kPer];
+	u32 ix[kPer];
 #pragma unroll
 	for (u32 r = 0; r < kPer; ++r)
-		if (wv + r * (kEncThreads / 64u) < nl)
-			c[r] = ldg(lcol, 64ull * llo + t0 + r * kEncThreads);
+		ix[r] = wv + r * (kEncThreads / 64u) < nl ? (u32)ldg(lidx, 64ull * llo + t0 + r * kEncThreads) : kNoEdit;
 	if (t0 < np0 + np1 && ((pmw >> (16u + (t0 & 7u))) & 1u))
 		S.fill[atomicAdd(&S.nfill, 1u)] = make_uint4(q.x, q.y, q.z, 1u << (3u * (vl - (pl + 1u))));
 	if (t0 < nl) {
 		const u32 m2 = (1u << P.leaf_bits >> 2) - 1u;
 		// a leaf's lane is the Morton index of its voxel inside the 4^3 leaf
 		S.lbase[t0] = leaf_writes_colors(P, lv, LE) ? morton3(LE.x & m2, LE.y & m2, LE.z & m2) << 6 : kNull;
-		S.lmask[t0] = am;
 	}
 #pragma unroll
 	for (u32 r = 0; r < kHdrPer; ++r) {
@@ -2734,9 +2726,18 @@ __device__ inline void job_rebuild(const EditParams &P, const ALevel *__restrict
 			S.hdr[k] = hw[r];
 	}
 	__syncthreads();
-	// the pushed and assigned colors (loads of all of a thread's voxels first), marked covered. A
-	// folded pass can push a color on a proceeding child (the edit that reset it) and again on its
-	// children or assign leaf colors inside it: the larger pushes go first, the leaves last.
+	// the assigning edits' colors from the pass's table (their loads overlap the pushes below)
+	uint2 ac[kPer];
+#pragma unroll
+	for (u32 r = 0; r < kPer; ++r) {
+		const u32 li = wv + r * (kEncThreads / 64u);
+		ac[r] = make_uint2(0u, 0u);
+		if (ix[r] != kNoEdit && li < nl && S.lbase[li] != kNull)
+			ac[r] = ldg(ecol, ix[r]);
+	}
+	// the pushed and assigned colors, marked covered. A folded pass can push a color on a proceeding
+	// child (the edit that reset it) and again on its children or assign leaf colors inside it: the
+	// larger pushes go first, the leaves last.
 	const u32 big = 1u << (3u * (vl - (leafL + 1u)));
 	for (u32 pass = 0; pass < 2; ++pass) {
 		for (u32 f = 0; f < S.nfill; ++f) {
@@ -2757,9 +2758,16 @@ __device__ inline void job_rebuild(const EditParams &P, const ALevel *__restrict
 #pragma unroll
 	for (u32 r = 0; r < kPer; ++r) {
 		const u32 li = wv + r * (kEncThreads / 64u), lane = t0 & 63u;
-		if (li < nl && S.lbase[li] != kNull && ((S.lmask[li] >> lane) & 1ull)) {
+		if (ix[r] != kNoEdit && li < nl && S.lbase[li] != kNull) {
 			const u32 at = S.lbase[li] | lane;
-			job_set(S, at, c[r]);
+			Color c{ac[r].x, ac[r].y};
+			if (ac[r].y >> 24) { // a column fill: the voxel's absolute position from its Morton index in the job
+				const u32 bits = P.leaf_bits;
+				const u32 vx = (E.x << bits) | morton_compact(at), vy = (E.y << bits) | morton_compact(at >> 1),
+				          vz = (E.z << bits) | morton_compact(at >> 2);
+				c = Color::rgb8(P.editor(ix[r]).column_rgb8(vx, vy, vz));
+			}
+			job_set(S, at, c);
 			atomicOr(&S.cov[at >> 5], 1u << (at & 31u));
 		}
 	}
@@ -2833,8 +2841,8 @@ constexpr u32 kEtmpJobWords = 512;
 template <bool kFull>
 __global__ void __launch_bounds__(kEncThreads, 4) a_job_encode(EditParams P, const ALevel *__restrict__ lv,
                                                              u32 *__restrict__ pm, const uint4 *__restrict__ jrange,
-                                                             const Color *__restrict__ lcol,
-                                                             const u64 *__restrict__ lmask, u32 V,
+                                                             const unsigned short *__restrict__ lidx,
+                                                             const uint2 *__restrict__ ecol, u32 V,
                                                              u32 *__restrict__ info, u32 *__restrict__ app,
                                                              u32 *__restrict__ etmp, u32 etmp_cap,
                                                              u32 *__restrict__ rest) {
@@ -2853,7 +2861,7 @@ __global__ void __launch_bounds__(kEncThreads, 4) a_job_encode(EditParams P, con
 		__syncthreads(); // the previous job's readers are done with S
 		Entry E;
 		uint4 jh;
-		job_rebuild(P, lv, jrange, lcol, lmask, V, e, S, E, jh);
+		job_rebuild(P, lv, jrange, lidx, ecol, V, e, S, E, jh);
 		const bool chunk = ctag(E.octree) == kTagLeaf;
 		const u32 block = jh.z;
 		// encode: a_encode_size's register path over the LDS colors
@@ -2925,8 +2933,8 @@ __global__ void __launch_bounds__(kEncThreads, 4) a_job_encode(EditParams P, con
 template <bool kFull>
 __global__ void __launch_bounds__(kEncThreads, 3) a_job_rest(EditParams P, const ALevel *__restrict__ lv,
                                                            const u32 *__restrict__ pm, const uint4 *__restrict__ jrange,
-                                                           const Color *__restrict__ lcol,
-                                                           const u64 *__restrict__ lmask, u32 V,
+                                                           const unsigned short *__restrict__ lidx,
+                                                           const uint2 *__restrict__ ecol, u32 V,
                                                            const u32 *__restrict__ info, const u32 *__restrict__ appo,
                                                            u64 leaf_cap, const u32 *__restrict__ rest) {
 	__shared__ JobLds S;
@@ -2939,7 +2947,7 @@ __global__ void __launch_bounds__(kEncThreads, 3) a_job_rest(EditParams P, const
 		__syncthreads(); // the previous job's readers are done with S
 		Entry E;
 		uint4 jh;
-		job_rebuild(P, lv, jrange, lcol, lmask, V, e, S, E, jh);
+		job_rebuild(P, lv, jrange, lidx, ecol, V, e, S, E, jh);
 		JobEnc R;
 		enc_scan_job<kFull>(S, V, R);
 		EncScan tot;
@@ -4415,7 +4423,7 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 }
 
 // ---------------------------------------------------------------- sync-free pass: host side
-constexpr u64 kAsyncBudget = 72ull << 30; // scratch bytes a sync-free pass may use (a quarter of HBM)
+constexpr u64 kAsyncBudget = 48ull << 30; // scratch bytes a sync-free pass may use (a sixth of HBM)
 
 // Upper bounds on the proceeding nodes of a pass, per level. A node proceeds only where some
 // edit's boundary crosses it: for a sphere the closed node cube [lb, lb + S]^3 meets the sphere
@@ -4549,7 +4557,7 @@ static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
 		b += (lds_jobs(p.dc) ? pp.cap[leafL] * (60 + 4 * (u64)kEtmpJobWords)
 		                     : pp.cap[leafL] * ((u64)sizeof(Color) << (3 * (vl - leafL))) +
 		                           pp.cap[leafL] * (60 + 4 * (u64)kEncTmp)) +
-		     pp.cap[L - 1] * (64 * sizeof(Color) + 8);
+		     pp.cap[L - 1] * (lds_jobs(p.dc) ? 64 * sizeof(unsigned short) : 64 * sizeof(Color) + 8);
 	pp.bytes = b;
 	if (b > kAsyncBudget || (pp.stateful && vl - leafL > 7))
 		pp.async = false;
@@ -4574,6 +4582,8 @@ struct AsyncBufs {
 	PushEvent *ev = nullptr;
 	Color *dense = nullptr, *lcol = nullptr;
 	u64 *lmask = nullptr;
+	unsigned short *lidx = nullptr; // LDS jobs: per leaf voxel, the pass edit that assigned its color
+	uint2 *ecol = nullptr;          // LDS jobs: per pass edit, the color lidx stands for
 	hd_edit_desc *edits = nullptr;
 	GroupTable G{};
 	u64 tab_slots = 0;
@@ -4659,8 +4669,13 @@ static int async_setup(Pool &p, const std::vector<PassPlan> &plans, const std::v
 		HD_RET(p.scratch.get<u32>(pre + "rest", jobs, &B.rest));
 		HD_RET(p.scratch.get<uint4>(pre + "jrange", 2 * jobs, &B.jrange));
 		const u64 leaves = std::max<u64>(cap[L - 1], 1);
-		HD_RET(p.scratch.get<Color>(pre + "lcol", leaves * 64, &B.lcol));
-		HD_RET(p.scratch.get<u64>(pre + "lmask", leaves, &B.lmask));
+		HD_RET(p.scratch.get<Color>(pre + "lcol", lj ? 1 : leaves * 64, &B.lcol));
+		HD_RET(p.scratch.get<u64>(pre + "lmask", lj ? 1 : leaves, &B.lmask));
+		HD_RET(p.scratch.get<unsigned short>(pre + "lidx", lj ? leaves * 64 : 1, &B.lidx));
+		u32 max_count = 1;
+		for (auto &pp : plans)
+			max_count = std::max(max_count, pp.count);
+		HD_RET(p.scratch.get<uint2>(pre + "ecol", max_count, &B.ecol));
 	}
 	HD_RET(p.scratch.get<hd_edit_desc>(pre + "edits", std::max<size_t>(flat.size(), 1), &B.edits));
 	HD_TRY(hipMemcpyAsync(B.edits, flat.data(), flat.size() * sizeof(hd_edit_desc), hipMemcpyHostToDevice, p.stream));
@@ -4764,6 +4779,8 @@ static void queue_setnode(const Pool &p, const PassPlan &pp, const EditParams &P
 // Queues one sync-free pass (no host synchronization).
 static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = true) {
 	const u32 L = p.dc.L, leafL = p.dc.color_leaf;
+	if (pp.stateful && lds_jobs(p.dc) && pp.count >= kNoEdit) // leaves record edits by 16-bit index
+		return set_error("hd_edit: a colored pass holds at most %u edits", kNoEdit - 1u), HD_ERR_ARG;
 	const EditParams P = pass_params(p, B.edits + pp.first, pp.count, pp.stateful, pp.folded);
 	const u32 V = pp.stateful ? 1u << (3u * P.leaf_bits) : 0u;
 	// A pair of a multi-edit pass tests its parent's edit list: count those tests too (lists hold at
@@ -4806,12 +4823,12 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 	if (split) {
 		HD_TRY(hipEventRecord(p.ev_top, p.stream));
 		HD_TRY(hipStreamWaitEvent(p.cstream, p.ev_top, 0));
-		a_leaf<kLeafColors><<<leaf_grid, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
-		a_leaf<kLeafNodes><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
+		a_leaf<kLeafColors><<<leaf_grid, kAT, 0, p.cstream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.lidx);
+		a_leaf<kLeafNodes><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.lidx);
 	} else if (colors)
-		a_leaf<kLeafAll><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
+		a_leaf<kLeafAll><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.lidx);
 	else
-		a_leaf<kLeafNodes><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask);
+		a_leaf<kLeafNodes><<<leaf_grid, kAT, 0, p.stream>>>(P, B.d_lv, B.pm, B.lcol, B.lmask, B.lidx);
 	int l_from = -1; // levels [0, l_from] are small enough for a_bottom
 	const u64 small_level = std::min<u32>(g_tuning.small_level, kSmallLevel);
 	while (l_from + 1 < (int)L && pp.cap[l_from + 1] <= small_level)
@@ -4828,10 +4845,10 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		}
 		u32 G = grid_for(pp.cap[leafL], 1, 8192);
 		if (ljobs) {
-			a_job_ranges<<<grid_for(pp.cap[leafL], 256, 4096), 256, 0, p.cstream>>>(B.d_lv, B.pm, leafL, L, p.cleaves,
-			                                                                       B.jrange);
+			a_job_ranges<<<grid_for(std::max<u64>(pp.cap[leafL], pp.count), 256, 4096), 256, 0, p.cstream>>>(
+			    B.d_lv, B.pm, leafL, L, p.cleaves, B.jrange, P.edits, pp.count, B.ecol);
 			const auto job = V == kJobV ? a_job_encode<true> : a_job_encode<false>;
-			job<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.jrange, B.lcol, B.lmask, V, B.info, B.app, B.etmp,
+			job<<<G, kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.jrange, B.lidx, B.ecol, V, B.info, B.app, B.etmp,
 			                                      B.etmp_cap, B.rest);
 		}
 		else {
@@ -4846,7 +4863,7 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		                                                 B.etmp);
 		if (ljobs) {
 			const auto rest = V == kJobV ? a_job_rest<true> : a_job_rest<false>;
-			rest<<<std::min(G, 1024u), kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.jrange, B.lcol, B.lmask, V, B.info,
+			rest<<<std::min(G, 1024u), kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.jrange, B.lidx, B.ecol, V, B.info,
 			                                                        B.appo, p.cleaf_cap, B.rest);
 		} else
 			a_encode_rest<<<std::min(G, 1024u), kEncThreads, 0, p.cstream>>>(P, B.d_lv, B.pm, B.dense, V, B.info,
@@ -4992,7 +5009,8 @@ static void folded_plans(const hd_edit_desc *edits, u32 n, std::vector<hd_edit_d
 	bool colored = false;
 	for (u32 k = 0; k < n; ++k)
 		colored |= is_stateful(edits[k]);
-	const u32 cap = g_tuning.batch_pass_edits ? g_tuning.batch_pass_edits : kFoldedPassEdits;
+	// (a pass's leaves record its edits by 16-bit index)
+	const u32 cap = std::min<u32>(g_tuning.batch_pass_edits ? g_tuning.batch_pass_edits : kFoldedPassEdits, kNoEdit - 1u);
 	PassPlan cur;
 	auto flush = [&]() {
 		if (cur.count) {
