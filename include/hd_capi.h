@@ -156,13 +156,14 @@ enum {
 	HD_MODE_PLAIN = 0,  /* one level-synchronous pass per edit, edits applied in order: the result
 	                       (pool words, bucket counters, color arrays, roots) equals NodePoolBase::Edit
 	                       applied to each edit in turn (NodePool.hpp:405-417) */
-	HD_MODE_BATCHED = 1 /* folded passes: runs of consecutive edits, overlapping ones included, each
-	                       applied by one level-synchronous pass whose nodes carry the edits that
-	                       touch them in submission order, folded from the last edit that fills or
-	                       clears the node whole (10 passes for 4096 mixed edits on the 4096^3
-	                       terrain); an edit with a weighted color, a column fill, and an uncolored
-	                       fill among colored edits run alone. The reachable DAG and the colors of
-	                       occupied voxels equal sequential application */
+	HD_MODE_BATCHED = 1 /* folded passes: runs of up to 4096 consecutive edits, overlapping ones
+	                       included, each applied by one level-synchronous pass whose nodes carry the
+	                       edits that touch them in submission order, folded from the last edit that
+	                       fills or clears the node whole; a run whose pass buffers exceed the 48 GB
+	                       sync-free budget is split in halves (4096 mixed edits on the 4096^3 terrain
+	                       take one pass); an edit with a weighted color, a column fill, and an
+	                       uncolored fill among colored edits run alone. The reachable DAG and the
+	                       colors of occupied voxels equal sequential application */
 };
 
 /* Apply n edits to (root, color_root); writes the new roots. Replaces NodePoolBase::Edit and
@@ -263,8 +264,9 @@ int hd_traversal(hd_pool *pool, uint32_t root, const float *o, const float *d, u
  * multi-view trace (n_views views up to max_width x max_height, pixels packed view after view) of
  * the roots the passes leave on the device. The sequence is captured once at creation; each frame
  * sets its edits and views (hd_frame_set) and replays the graph (hd_frame_launch). Edits are grouped
- * as HD_MODE_BATCHED groups them, so a frame's DAG equals hd_edit(..., HD_MODE_BATCHED) word for
- * word, and one edit per frame equals HD_MODE_PLAIN. With HD_FRAME_GC the GC runs inside the graph
+ * into passes of pairwise-disjoint edits, in order, so a frame's reachable DAG and the colors of its
+ * occupied voxels equal applying the edits one by one (HD_MODE_BATCHED's bar), and one edit per
+ * frame equals HD_MODE_PLAIN word for word. With HD_FRAME_GC the GC runs inside the graph
  * (the frame's words then equal hd_edit batched + hd_gc); without it, after an hd_gc hand the
  * remapped roots back with hd_frame_set_roots. */
 enum {
