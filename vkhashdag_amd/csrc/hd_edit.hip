@@ -806,7 +806,8 @@ __global__ void k_add_meta(u32 *meta, u32 which, const u32 *a, const u32 *b, u32
 constexpr u32 kAT = 256;       // threads per block of the sync-free kernels
 constexpr u32 kTopPairs = 4096; // a level whose 8 n pairs fit this bound expands inside a_top
 constexpr u32 kGather = 32;    // bucket lists up to this length are sorted in registers
-constexpr u32 kWalkList = 16;  // groups per bucket listed by arrival for the walker
+constexpr u32 kWalkList = 64;  // groups per bucket listed by arrival for the walker
+constexpr u32 kWalkShort = 16; // at most this many: one thread walks the bucket from registers
 
 struct Pair {
 	u32 a, b;
@@ -2280,9 +2281,68 @@ __device__ inline u32 walk_short(const EditParams &P, bool leaf, const ALevel &A
 	return appended;
 }
 
-// Returns the nodes appended (0 unless e walks its bucket).
+// A bucket of kWalkShort < k <= kWalkList new groups, walked by a whole wavefront (all 64 lanes,
+// b and k uniform): lane x holds the x-th group by arrival; its rank by first entry counts the
+// smaller keys across the wave, the positions follow in rank order from one uniform pass (page
+// padding makes them sequential; the padding words are cleared by the wave), and each lane writes
+// its group's result and node words. Returns the nodes appended on lane 0 (0 on the others).
+__device__ inline u32 walk_wave(const EditParams &P, bool leaf, const ALevel &A, const GroupTable &G, u32 b, u32 k,
+                                const u32 *__restrict__ blist, u32 lane) {
+	const u32 stride = leaf ? 2u : 9u, wpp = P.c.words_per_page(), wpb = P.c.words_per_bucket();
+	const bool in = lane < k;
+	const u32 sl = in ? ldg(blist, (u64)b * kWalkList + lane) : 0u;
+	const u32 key = in ? (u32)ldg(G.tab, sl) : 0xFFFFFFFFu; // the group's first entry
+	const u32 nz = in ? (leaf ? 2u : ldg(A.csize, key)) : 0u;
+	u32 rk = 0;
+	for (u32 y = 0; y < k; ++y) // distinct keys
+		rk += (u32)__builtin_amdgcn_readlane((int)key, (int)y) < key ? 1u : 0u;
+	u32 nz_r = 0; // lane r: the size of the group ranked r
+	for (u32 y = 0; y < k; ++y)
+		if ((u32)__builtin_amdgcn_readlane((int)rk, (int)y) == lane)
+			nz_r = (u32)__builtin_amdgcn_readlane((int)nz, (int)y);
+	u32 bw = ldg(P.bucket_words, b), at_r = kNull, appended = 0;
+	bool any_fail = false;
+	const u64 bb = (u64)b << P.c.wbpb();
+	for (u32 r = 0; r < k; ++r) { // append_to_bucket in rank order
+		const u32 n = (u32)__builtin_amdgcn_readlane((int)nz_r, (int)r);
+		u32 at = kNull;
+		if (bw + n <= wpb) {
+			u32 slot = bw >> P.c.wbpp, off = bw & (wpp - 1u);
+			if (off + n > wpp) {
+				for (u32 q = off + lane; q < wpp; q += 64u)
+					stg(P.words, bb + ((u64)slot << P.c.wbpp) + q, 0u);
+				++slot;
+				off = 0;
+			}
+			const u32 rel = (slot << P.c.wbpp) | off;
+			at = (u32)(bb + rel);
+			bw = rel + n;
+			++appended;
+		} else
+			any_fail = true;
+		at_r = lane == r ? at : at_r;
+	}
+	const u32 pos = (u32)__shfl((int)at_r, (int)rk);
+	if (in) {
+		stg(G.gres, sl, pos);
+		stg(G.gfail, sl, pos == kNull ? 1u : 0u);
+		if (pos != kNull)
+			for (u32 j = 0; j < nz; ++j)
+				stg(P.words, (u64)pos + j, ldg(A.content, (u64)key * stride + j));
+	}
+	if (lane == 0) {
+		stg(P.bucket_words, b, bw);
+		if (any_fail)
+			atomicOr(&P.meta[kMetaOverflow], 1u);
+	}
+	return lane == 0 ? appended : 0u;
+}
+
+// Returns the nodes appended (0 unless e walks its bucket). A bucket of more than kWalkShort and at
+// most kWalkList new groups is left to the caller's wavefront (wb, wk).
 __device__ inline u32 walk_entry(const EditParams &P, u32 l, const ALevel &A, const GroupTable &G, u32 e,
-                                  u32 *__restrict__ bcnt, u32 *__restrict__ head, const u32 *__restrict__ blist) {
+                                  u32 *__restrict__ bcnt, u32 *__restrict__ head, const u32 *__restrict__ blist,
+                                  u32 &wb, u32 &wk) {
 	if (A.lpos[e] != 0)
 		return 0;
 	const bool leaf = l == P.c.L - 1;
@@ -2292,9 +2352,12 @@ __device__ inline u32 walk_entry(const EditParams &P, u32 l, const ALevel &A, co
 	u32 appended = 0;
 	if (k <= 4u)
 		appended = walk_short<4>(P, leaf, A, G, b, k, bw0, blist);
-	else if (k <= kWalkList)
-		appended = walk_short<kWalkList>(P, leaf, A, G, b, k, bw0, blist);
-	else { // long lists (distinct new nodes of one bucket): repeated minimum search over the chain
+	else if (k <= kWalkShort)
+		appended = walk_short<kWalkShort>(P, leaf, A, G, b, k, bw0, blist);
+	else if (k <= kWalkList) {
+		wb = b, wk = k;
+		return 0;
+	} else { // long lists (distinct new nodes of one bucket): repeated minimum search over the chain
 		u32 bw = bw0;
 		bool any_fail = false;
 		u32 last = 0;
@@ -2363,10 +2426,25 @@ __global__ void __launch_bounds__(kAT) a_walk(EditParams P, u32 l, const ALevel 
                                               u32 *__restrict__ head, const u32 *__restrict__ blist,
                                               u32 *__restrict__ acc) {
 	const ALevel A = lv[l];
-	const u32 n = pm[kPmHdr + l];
+	const u32 n = pm[kPmHdr + l], lane = threadIdx.x & 63u;
+	const bool leaf = l == P.c.L - 1;
 	u32 app = 0;
-	for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
-		app += walk_entry(P, l, A, G, e, bcnt, head, blist);
+	for (u32 e0 = blockIdx.x * blockDim.x; e0 < n; e0 += gridDim.x * blockDim.x) { // uniform trip count
+		const u32 e = e0 + threadIdx.x;
+		u32 wb = kNull, wk = 0;
+		if (e < n)
+			app += walk_entry(P, l, A, G, e, bcnt, head, blist, wb, wk);
+		// the buckets of kWalkShort..kWalkList new groups its lanes found: one at a time, by the wave
+		for (u64 m = __ballot(wb != kNull); m; m &= m - 1ull) {
+			const int src = (int)__builtin_ctzll(m);
+			const u32 b = (u32)__builtin_amdgcn_readlane((int)wb, src), k = (u32)__builtin_amdgcn_readlane((int)wk, src);
+			app += walk_wave(P, leaf, A, G, b, k, blist, lane);
+			if (lane == 0) {
+				bcnt[b] = 0;
+				head[b] = kNull;
+			}
+		}
+	}
 	// one counter update per wavefront: a bump per walker would serialise on the one address
 #pragma unroll
 	for (u32 d = 32; d > 0; d >>= 1)
