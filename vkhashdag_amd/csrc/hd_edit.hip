@@ -4896,7 +4896,8 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 			                                                           B.lv[leafL].cap, B.dense);
 	}
 	const u32 leaf_grid = grid_for(pp.cap[L - 1], kAT / 64, 16384);
-	// with LDS jobs, the leaves' colors start the color stream right after the top-down
+	// with LDS jobs, the leaves' colors start the color stream right after the top-down (always: such
+	// a pass has the edit-index buffer lidx, and no lcol / lmask for kLeafAll)
 	const bool split = colors && ljobs;
 	if (split) {
 		HD_TRY(hipEventRecord(p.ev_top, p.stream));
