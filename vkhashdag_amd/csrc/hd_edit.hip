@@ -1912,9 +1912,9 @@ __device__ inline uint2 child_result_fresh(const ALevel &C, const GroupTable &G,
 }
 
 // edit_leaf (k_leaf) + the find among the bucket's existing leaves: one wavefront per leaf. A
-// leaf inside a color job records the colors its edits assign (lcol, by lane) and which voxels
-// they are (lmask); a_leafcolor moves them into the job's dense colors, so this kernel does not
-// wait for the dense colors to be decoded.
+// leaf inside a color job records what its edits assign: on the dense path the colors (lcol, by
+// lane) and which voxels they are (lmask), which a_leafcolor moves into the job's dense colors (so
+// this kernel does not wait for them to be decoded); with LDS jobs the assigning edit per voxel.
 __device__ inline bool leaf_writes_colors(const EditParams &P, const ALevel *__restrict__ lv, const Entry &E) {
 	return (E.sf & kSfWriter) && !(E.sf & kSfFinal) && (E.sf & kSfActive) && E.job < lv[P.c.color_leaf].cap;
 }
