@@ -4502,6 +4502,18 @@ void schedule_groups(const hd_edit_desc *edits, u32 n, u32 res, std::vector<std:
 
 // ---------------------------------------------------------------- sync-free pass: host side
 constexpr u64 kAsyncBudget = 48ull << 30; // scratch bytes a sync-free pass may use (a sixth of HBM)
+// The budget a large pass gets: kAsyncBudget, or less when the device holds less free memory (the
+// scratch this pool already holds counts as free: its buffers are reused; buffers grow with a
+// quarter of headroom, hence 3/4).
+static u64 async_budget(const Pool &p) {
+	size_t free_b = 0, total_b = 0;
+	if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+		return kAsyncBudget;
+	u64 held = 0;
+	for (const auto &kv : p.scratch.bufs)
+		held += kv.second.bytes;
+	return std::min<u64>(kAsyncBudget, ((u64)free_b + held) / 4 * 3);
+}
 
 // Upper bounds on the proceeding nodes of a pass, per level. A node proceeds only where some
 // edit's boundary crosses it: for a sphere the closed node cube [lb, lb + S]^3 meets the sphere
@@ -4637,7 +4649,7 @@ static void plan_pass(const Pool &p, const hd_edit_desc *edits, PassPlan &pp) {
 		                           pp.cap[leafL] * (60 + 4 * (u64)kEncTmp)) +
 		     pp.cap[L - 1] * (lds_jobs(p.dc) ? 64 * sizeof(unsigned short) : 64 * sizeof(Color) + 8);
 	pp.bytes = b;
-	if (b > kAsyncBudget || (pp.stateful && vl - leafL > 7))
+	if (b > kAsyncBudget || (b > (1ull << 30) && b > async_budget(p)) || (pp.stateful && vl - leafL > 7))
 		pp.async = false;
 }
 
