@@ -50,6 +50,27 @@ def test_plain_edits_bit_exact(case):
     assert np.array_equal(dev.filled(), ora.filled())
 
 
+@pytest.mark.parametrize("bits", [2, 0])
+def test_plain_walkers_of_every_size_bit_exact(bits):
+    # levels of more than 64 entries leave the one-workgroup kernel (small_level 64) for the bucket
+    # walkers; with 4 buckets per level (bits 2) the large edits give 16..31 new leaves per bucket (a
+    # whole wavefront walks them), with 1 bucket (bits 0) up to 93 (the chain of minimum searches),
+    # and the small edits 1..4 (one thread): every placement must stay the reference's, word for word
+    capi = _capi()
+    cfg = small_cfg(64, bits, 9, 2, 3)
+    e = random_edits(24, 64, 9, True)
+    old = capi.set_tuning(small_level=64)
+    try:
+        dev, ora = capi.DevicePool(cfg), O.OraclePool(cfg)
+        for i in range(len(e)):
+            dev.edit(e[i:i + 1])
+            ora.edit(e[i:i + 1])
+            assert_same_pool(dev, ora, f"bits {bits} edit {i} kind {int(e[i]['kind'])}")
+        assert ora.overflows() == 0
+    finally:
+        capi.set_tuning(**old)
+
+
 def test_fragmented_color_leaf_bit_exact():
     # 1500 one-voxel colored fills in one 16^3 color leaf: a color change at almost every voxel makes
     # the encoded chunk (2 header words per change) outgrow the encoder's 4096-word staging slot, so
