@@ -3,7 +3,8 @@
 Bar: bit-exact. trace_tile, top_pairs and small_level give the same roots, pool words, color pool
 and pixels as the default set on plain edits and a primary + shadow frame. batch_pass_edits splits
 batched passes, which renumbers nodes (the batched path's own bar): the DAG up to renumbering and
-the pixels are unchanged. The autotuner's search (small terrain, few edits) returns a valid set
+the pixels of a frame without LOD are unchanged (a LOD cut-off shows the color stored at its cube's
+corner voxel, which may be empty, and empty voxels' colors depend on the pass split). The autotuner's search (small terrain, few edits) returns a valid set
 and applies it."""
 import json
 
@@ -41,7 +42,7 @@ def _run(capi, res, h, tuning, batched):
     p.edit(W.terrain_sphere_edits(24, res, h, 2, 24, seed=5))
     if batched:
         p.edit(W.mixed_batch(160, res, h, 2, 20, seed=6), capi.MODE_BATCHED)
-    vb = W.orbit_view((res / 2, res / 2, res / 3), res * 0.9, 3, 72, 40, res=res)
+    vb = W.orbit_view((res / 2, res / 2, res / 3), res * 0.9, 3, 72, 40, res=res, lod=not batched)
     img = p.render(vb, 72, 40, shadow=True)
     return cfg, p, img
 

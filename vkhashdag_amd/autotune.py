@@ -9,7 +9,8 @@ Parameters and the workload each is measured on (seeded synthetic inputs, the be
                     passes (sphere add/remove, r log-uniform 4..128) from one checkpointed pool state;
   batch_pass_edits  the largest batched pass (queue depth): a batch of mixed edits from the same state.
 The search is coordinate-wise in that order; each measurement is the best of `repeats` runs. Every
-plain candidate must give the first candidate's roots, every batched one its rendered frame.
+plain candidate must give the first candidate's roots, every batched one its rendered frame
+(LOD off).
 
     python -m vkhashdag_amd.autotune --res 4096 --out vkhashdag_amd/tuning.json
     autotune.load("vkhashdag_amd/tuning.json")   # applies a stored set (hd_tuning_set)
@@ -31,7 +32,7 @@ DEFAULT_PATH = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {
     "trace_tile": [8, 16],
-    "top_pairs": [1024, 2048, 4096],
+    "top_pairs": [512, 1024, 2048, 4096],
     "small_level": [256, 512, 1024],
     "batch_pass_edits": [0, 512, 1024],
 }
@@ -154,7 +155,10 @@ def tune(res: int = 4096, seed: int = 1, plain_edits: int = 64, batch_edits: int
         capi.set_tuning(top_pairs=tp, small_level=sl)
 
         batch = W.mixed_batch(batch_edits, res, heights, 2, 256, seed=2)
-        view = (W.orbit_view((res / 2, res / 2, res * 0.3), 0.9 * res, 11, 256, 144, res=res), 256, 144)
+        # LOD off: a LOD cut-off shows the color stored at its cube's corner voxel, which may be
+        # empty, and empty voxels' colors depend on how the edits were split into passes (the
+        # batched bar is the DAG and the colors of occupied voxels)
+        view = (W.orbit_view((res / 2, res / 2, res * 0.3), 0.9 * res, 11, 256, 144, res=res, lod=False), 256, 144)
         ref = None
         for q in cand["batch_pass_edits"]:
             capi.set_tuning(batch_pass_edits=q)

@@ -4886,8 +4886,11 @@ static int queue_pass(Pool &p, const PassPlan &pp, AsyncBufs &B, bool reserve = 
 		HD_RET(gen_reserve(p, B, 1));
 	const auto top = pp.count > 1 ? a_top<true> : a_top<false>;
 	top<<<1, kAT, 0, p.stream>>>(P, B.d_lv, B.roots, B.pm, kPmHdr + 2 * L, top_end, B.ev, (u32)pp.ev_cap, B.gen_ctr);
+	// one 256-pair tile per block up to the block cap: a block's tiles are a chain of dependent round
+	// trips (entry, node words, color child), so a mid-size level spread over few blocks waits on
+	// them in turn (256 sphere edits on the 4096^3 terrain: 44.1 -> 41.0 ms per step vs 2048 pairs)
 	for (u32 l = top_end; l + 1 < L; ++l) {
-		u32 G = grid_for(8 * pp.cap[l] * lf, 2048, kMaxEmitBlocks);
+		u32 G = grid_for(8 * pp.cap[l] * lf, kAT, kMaxEmitBlocks);
 		a_count<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.ev, (u32)pp.ev_cap, B.bsum);
 		a_emit<<<G, kAT, 0, p.stream>>>(P, l, B.d_lv, B.pm, B.flag, B.lcnt, B.cstate, B.bsum);
 	}
