@@ -3779,8 +3779,8 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 }
 
 // Single workgroup: the levels [0, l_from] (each at most kSmallLevel entries) bottom-up -- build,
-// dedup and walk with the bookkeeping in LDS -- the root result, the pass statistics and, for a
-// colored pass, SetNode.
+// dedup and walk with the bookkeeping in LDS -- then the root result and the pass statistics (a
+// colored pass's SetNode runs on the color stream, a_setnode).
 __global__ void __launch_bounds__(kAT) a_bottom(EditParams P, const ALevel *__restrict__ lv, const u32 *__restrict__ pm,
                                                 int l_from, GroupTable G, u32 *__restrict__ roots,
                                                 u32 *__restrict__ acc, u64 node_cap) {
