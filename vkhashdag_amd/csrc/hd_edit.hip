@@ -3727,36 +3727,56 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 			pre = load_join(s_lv[l - 1], lane);
 		if (!leaf)
 			jbase += n;
-		u64 pending = __ballot(need);
-		u32 appended = 0;
-		bool full = false;
+		// groups of identical new contents: each lane's leader (the group's lowest lane)
+		u64 pending = __ballot(need), leaders = 0;
+		u32 lead = lane;
 		while (pending) {
 			const u32 ld = (u32)__builtin_ctzll(pending);
 			const u32 lh = __builtin_amdgcn_readlane(h, ld), lk = __builtin_amdgcn_readlane(k, ld);
-			const u32 lb = __builtin_amdgcn_readlane(b, ld);
 			bool same = need && h == lh && k == lk;
 #pragma unroll
 			for (u32 j = 0; j < 9; ++j)
 				same = same && c[j] == __builtin_amdgcn_readlane(c[j], ld);
-			const u64 members = __ballot(same);
-			u32 f = kNull, nbw = bw;
-			if (lane == ld) {
-				f = append_regs(P.words, P.c, lb, nbw, c, k);
-				stg(P.bucket_words, lb, nbw);
+			if (same)
+				lead = ld;
+			leaders |= 1ull << ld;
+			pending &= ~__ballot(same);
+		}
+		const bool leader = (leaders >> lane) & 1ull;
+		// leaders in distinct buckets append at once; a bucket with several (rare: a level here has at
+		// most 64 entries) takes its groups in lane order, as the reference appends them
+		bool shared = false;
+		for (u64 m = leaders; m; m &= m - 1ull) {
+			const u32 y = (u32)__builtin_ctzll(m);
+			shared |= leader && y < lane && __builtin_amdgcn_readlane(b, y) == b;
+		}
+		u32 f = kNull;
+		if (__ballot(shared) == 0ull) {
+			if (leader) {
+				u32 nbw = bw;
+				f = append_regs(P.words, P.c, b, nbw, c, k);
+				stg(P.bucket_words, b, nbw);
 			}
-			f = __builtin_amdgcn_readlane(f, ld);
-			nbw = __builtin_amdgcn_readlane(nbw, ld);
-			if (f == kNull)
-				full = true;
-			else
-				++appended;
-			if (same) {
-				fin = f == kNull ? optr : f;
-				fresh_out = f != kNull;
+		} else
+			for (u64 m = leaders; m; m &= m - 1ull) {
+				const u32 ld = (u32)__builtin_ctzll(m);
+				const u32 lb = __builtin_amdgcn_readlane(b, ld);
+				u32 g = kNull, nbw = bw;
+				if (lane == ld) {
+					g = append_regs(P.words, P.c, lb, nbw, c, k);
+					stg(P.bucket_words, lb, nbw);
+				}
+				nbw = __builtin_amdgcn_readlane(nbw, ld);
+				f = lane == ld ? g : f;
+				if (need && b == lb)
+					bw = nbw; // later groups of this bucket append after it
 			}
-			if (need && b == lb)
-				bw = nbw; // later groups of this bucket append after it
-			pending &= ~members;
+		const u32 appended = (u32)__popcll(__ballot(leader && f != kNull));
+		const bool full = __ballot(leader && f == kNull) != 0ull;
+		const u32 fl = (u32)__shfl((int)f, (int)lead); // the group's result
+		if (need) {
+			fin = fl == kNull ? optr : fl;
+			fresh_out = fl != kNull;
 		}
 		if (lane == 0) {
 			if (appended)
