@@ -3727,13 +3727,20 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 			pre = load_join(s_lv[l - 1], lane);
 		if (!leaf)
 			jbase += n;
-		// groups of identical new contents: each lane's leader (the group's lowest lane)
-		u64 pending = __ballot(need), leaders = 0;
+		// groups of identical new contents: each lane's leader (the group's lowest lane); a lane whose
+		// hash no other new content shares leads its own group, the others are grouped by content
+		const u64 news = __ballot(need);
+		bool dup = false;
+		for (u64 m = news; m; m &= m - 1ull) {
+			const u32 y = (u32)__builtin_ctzll(m);
+			dup |= need && y != lane && __builtin_amdgcn_readlane(h, y) == h;
+		}
+		u64 pending = __ballot(dup), leaders = news & ~pending;
 		u32 lead = lane;
 		while (pending) {
 			const u32 ld = (u32)__builtin_ctzll(pending);
 			const u32 lh = __builtin_amdgcn_readlane(h, ld), lk = __builtin_amdgcn_readlane(k, ld);
-			bool same = need && h == lh && k == lk;
+			bool same = dup && h == lh && k == lk;
 #pragma unroll
 			for (u32 j = 0; j < 9; ++j)
 				same = same && c[j] == __builtin_amdgcn_readlane(c[j], ld);
