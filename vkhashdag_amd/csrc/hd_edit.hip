@@ -3729,11 +3729,13 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 			jbase += n;
 		// groups of identical new contents: each lane's leader (the group's lowest lane); a lane whose
 		// hash no other new content shares leads its own group, the others are grouped by content
+		// (below: whether a lower lane's new content goes to this lane's bucket)
 		const u64 news = __ballot(need);
-		bool dup = false;
+		bool dup = false, below = false;
 		for (u64 m = news; m; m &= m - 1ull) {
 			const u32 y = (u32)__builtin_ctzll(m);
 			dup |= need && y != lane && __builtin_amdgcn_readlane(h, y) == h;
+			below |= need && y < lane && __builtin_amdgcn_readlane(b, y) == b;
 		}
 		u64 pending = __ballot(dup), leaders = news & ~pending;
 		u32 lead = lane;
@@ -3751,14 +3753,11 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 		}
 		const bool leader = (leaders >> lane) & 1ull;
 		// leaders in distinct buckets append at once; a bucket with several (rare: a level here has at
-		// most 64 entries) takes its groups in lane order, as the reference appends them
-		bool shared = false;
-		for (u64 m = leaders; m; m &= m - 1ull) {
-			const u32 y = (u32)__builtin_ctzll(m);
-			shared |= leader && y < lane && __builtin_amdgcn_readlane(b, y) == b;
-		}
+		// most 64 entries) takes its groups in lane order, as the reference appends them. A leader
+		// shares its bucket with a lower leader exactly when a lower lane's new content goes there (that
+		// lane's leader is lower still).
 		u32 f = kNull;
-		if (__ballot(shared) == 0ull) {
+		if (__ballot(leader && below) == 0ull) {
 			if (leader) {
 				u32 nbw = bw;
 				f = append_regs(P.words, P.c, b, nbw, c, k);
