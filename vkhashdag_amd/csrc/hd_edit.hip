@@ -3476,6 +3476,9 @@ static_assert(kSTab == 1u << 11 && kSmallLevel * 2u <= kSTab);
 constexpr u32 kColorLds = 4096;   // SetNode entries (all levels above the color leaves) kept in LDS
 
 constexpr u32 kStage = 5 * kSTab; // bucket staging words of the build (shares the dedup tables' LDS)
+// the wave-0 levels' slot counts in the stage: hash slots, then bucket slots
+constexpr u32 kDupSlots = 2048, kDupBucketSlots = 8192;
+static_assert(kDupSlots + kDupBucketSlots <= kStage, "the slot counts fit the stage");
 
 struct BottomLds {
 	// per entry; [2] = this level / its child level, whose results the build resolves
@@ -3728,14 +3731,34 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 		if (!leaf)
 			jbase += n;
 		// groups of identical new contents: each lane's leader (the group's lowest lane); a lane whose
-		// hash no other new content shares leads its own group, the others are grouped by content
-		// (below: whether a lower lane's new content goes to this lane's bucket)
+		// hash no other new content shares leads its own group, the others are grouped by content.
+		// dup: another new content has this lane's hash slot; below: a lower lane's new content has this
+		// lane's bucket slot. Slots are counted in the stage, free until the next level's finds: equal
+		// values always share a slot, so a collision of distinct values only sends lanes to the exact
+		// steps below (the content grouping, the ordered appends).
 		const u64 news = __ballot(need);
 		bool dup = false, below = false;
-		for (u64 m = news; m; m &= m - 1ull) {
-			const u32 y = (u32)__builtin_ctzll(m);
-			dup |= need && y != lane && __builtin_amdgcn_readlane(h, y) == h;
-			below |= need && y < lane && __builtin_amdgcn_readlane(b, y) == b;
+		if (news) {
+			u32 *hc = S.stage, *bl = S.stage + kDupSlots;
+			const u32 hs = h & (kDupSlots - 1u), bs = b & (kDupBucketSlots - 1u);
+			if (need) {
+				hc[hs] = 0u;
+				bl[bs] = 64u;
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			if (need) {
+				atomicAdd(&hc[hs], 1u);
+				atomicMin(&bl[bs], lane);
+			}
+			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+			__builtin_amdgcn_wave_barrier();
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+			if (need) {
+				dup = hc[hs] > 1u;
+				below = bl[bs] < lane;
+			}
 		}
 		u64 pending = __ballot(dup), leaders = news & ~pending;
 		u32 lead = lane;
