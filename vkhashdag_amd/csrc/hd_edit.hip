@@ -1537,14 +1537,15 @@ __global__ void __launch_bounds__(kAT) a_top(EditParams P0, const ALevel *__rest
 			}
 			const u64 bp = __ballot(o.flag & 1u), bt = __ballot((o.flag >> 1) & 1u), bu = __ballot((o.flag >> 2) & 1u);
 			const u64 below = (1ull << lane) - 1ull;
-			u32 lc = o.lcnt; // inclusive wave scan of the list lengths
+			u32 lc = o.lcnt; // inclusive wave scan of the list lengths (a single-edit pass has none)
+			if (kLists)
 #pragma unroll
-			for (u32 d = 1; d < 64; d <<= 1) {
-				const u32 y = __shfl_up(lc, d);
-				if (lane >= d)
-					lc += y;
-			}
-			const Pair tot{(u32)__popcll(bp), __shfl(lc, 63)};
+				for (u32 d = 1; d < 64; d <<= 1) {
+					const u32 y = __shfl_up(lc, d);
+					if (lane >= d)
+						lc += y;
+				}
+			const Pair tot{(u32)__popcll(bp), kLists ? (u32)__shfl(lc, 63) : 0u};
 			if (in) {
 				const u32 m8 = (lane & 7u) == 0 ? (u32)((bp >> lane) & 0xFFu) | ((u32)((bt >> lane) & 0xFFu) << 8) |
 				                                      ((u32)((bu >> lane) & 0xFFu) << 16)
