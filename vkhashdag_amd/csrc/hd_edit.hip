@@ -3777,9 +3777,10 @@ __device__ inline void tiny_levels(const EditParams &P, int l_top, int l_from, c
 		}
 		const bool leader = (leaders >> lane) & 1ull;
 		// leaders in distinct buckets append at once; a bucket with several (rare: a level here has at
-		// most 64 entries) takes its groups in lane order, as the reference appends them. A leader
-		// shares its bucket with a lower leader exactly when a lower lane's new content goes there (that
-		// lane's leader is lower still).
+		// most 64 entries) takes its groups in lane order, as the reference appends them. A leader that
+		// shares its bucket with a lower leader has below set (that leader's content is in a lower lane,
+		// in the same slot); below may also be set by a slot collision, which only costs the ordered
+		// appends.
 		u32 f = kNull;
 		if (__ballot(leader && below) == 0ull) {
 			if (leader) {
